@@ -1,0 +1,241 @@
+"""Benchmark: plain texture-space path tracer on one MI355X.
+
+Default workload (one "step" = one progressive iteration over every texel):
+the seeded synthetic 1.21M-triangle scene (512x512-quad value-noise
+heightfield, 64x64 extruded boxes, 500 level-3 icospheres; sun at 35 deg +
+sky 0.3), texels_per_unit 8, min_resolution 2, max_resolution 16, budget
+16,777,216 slots, 1 sample/texel, 3 bounces, FP32.  ``--workload cornell``
+runs the small Cornell box instead.
+
+Prints one JSON line.  ``value`` is texel-path samples/s (slot updates per
+second) with the scene, layout and cache resident in HBM.  Work counts come
+from a counted replay of the same iterations; the roofline entry prices the
+tracer kernel at its algorithmic bytes (counted BVH node and triangle
+fetches, slot read/write, texel->face map, layout tables and per-hit shading
+records) over its measured launch time.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "texel-path samples/s (Mrays/s) at equal RMSE vs 4096-spp bake; learned vs plain"
+
+# algorithmic bytes (see the module docstring)
+BYTES_PER_NODE = 64
+BYTES_PER_TRI = 48
+BYTES_PER_SLOT_UPDATE = 32         # 16-byte slot read + write
+BYTES_PER_TEXEL_SETUP = 4 + 4 + 16 + 8 + 4 + 8   # texel->face, face->instance, instance, side, prefix, face
+# per shaded hit: tri id map 8 + instance 16 + mesh 16 + 3 indices 12 + 3 vertices x 32
+# + transform 48 + inverse transpose 64 + material 32
+BYTES_PER_SHADED_HIT = 8 + 16 + 16 + 12 + 96 + 48 + 64 + 32
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(name: str):
+    from nan_amd.mesh_colors import MeshColorsLayout
+    from nan_amd.scenes import cornell_box, synthetic_scene
+
+    if name == "cornell":
+        node = cornell_box()
+        kw = dict(texels_per_unit=16.0, min_resolution=4, max_resolution=64, max_total_texels=16_777_216)
+        desc = ("plain texel path tracer, Cornell box (36 tris), texels_per_unit 16, res 4-64, "
+                "1 spp, 3 bounces, FP32")
+    else:
+        node = synthetic_scene(0)
+        kw = dict(texels_per_unit=8.0, min_resolution=2, max_resolution=16, max_total_texels=16_777_216)
+        desc = ("plain texel path tracer, synthetic 1.21M-tri scene seed 0 (heightfield + 4096 boxes + "
+                "500 icospheres, sun 35 deg + sky 0.3), texels_per_unit 8, res 2-16, budget 16,777,216, "
+                "1 spp, 3 bounces, FP32")
+    layout = MeshColorsLayout.build(node, **kw)
+    return node, layout, kw, desc
+
+
+def cpu_baseline(node, layout, threads: int, target_seconds: float = 12.0) -> dict:
+    """Oracle tracer (C, pthreads) on a strided sample of front texels."""
+    import numpy as np
+
+    import oracle
+    from nan_amd.scene_arrays import flatten_scene
+
+    t0 = time.perf_counter()
+    orc = oracle.OracleScene(flatten_scene(node))
+    build_s = time.perf_counter() - t0
+    payload = np.zeros((layout.total_payload_count, 4), dtype=np.float32)
+    total = layout.total_surface_texels
+    stride = max(1, total // 4096)
+    work = np.arange(0, total, stride, dtype=np.uint64)
+    t0 = time.perf_counter()
+    cnt = orc.trace_texels(layout.face_table, layout.instance_table, layout.side_table, payload, 0, 1, 3, True,
+                           work=work, threads=threads)
+    dt = time.perf_counter() - t0
+    # scale the sample so the timed run is ~target_seconds
+    per_texel = dt / max(1, len(work))
+    n = int(min(total, max(len(work), target_seconds / max(per_texel, 1e-9))))
+    stride = max(1, total // n)
+    work = np.arange(0, total, stride, dtype=np.uint64)
+    t0 = time.perf_counter()
+    cnt = orc.trace_texels(layout.face_table, layout.instance_table, layout.side_table, payload, 0, 1, 3, True,
+                           work=work, threads=threads)
+    dt = time.perf_counter() - t0
+    samples = int(cnt[3])
+    rays = int(cnt[0] + cnt[1])
+    return {
+        "value": samples / dt,
+        "unit": "texel-path samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle C tracer, iteration 0, every {stride}th front texel ({len(work)} texels, "
+                   f"{samples} slot updates, {rays} rays) in {dt:.1f} s; oracle BVH build {build_s:.1f} s"),
+        "mrays_per_s": rays / dt / 1e6,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["synthetic", "cornell"], default="synthetic")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; the path shards by texel range with no "
+                                   "exchange during tracing, and multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from nan_amd import _native
+    from nan_amd.device import Device
+    from nan_amd.scene import Scene
+    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
+    t0 = time.perf_counter()
+    node, layout, layout_kw, desc = build_workload(args.workload)
+    log(f"scene {node.triangle_count} tris, {len(node.instances)} instances; layout "
+        f"{layout.total_surface_texels} surface + {layout.total_payload_count - layout.total_surface_texels} back "
+        f"slots ({time.perf_counter() - t0:.1f} s)")
+    device = Device(0)
+    t0 = time.perf_counter()
+    scene = Scene(device, node)
+    log(f"BVH: {scene.info.node_count} nodes, {scene.info.leaf_count} leaves, max stack {scene.info.max_stack}, "
+        f"build {scene.info.build_ms:.0f} ms, upload total {time.perf_counter() - t0:.1f} s")
+    tracer = TextureSpacePathTracer(device, scene, layout, samples_per_texel=1, max_bounces=3)
+    payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=device.torch_device)
+    enc = device.create_command_encoder()
+    stream = torch.cuda.current_stream(device.torch_device)
+
+    it = 0
+    for _ in range(args.warmup):
+        tracer.execute(enc, payload, it, reset=(it == 0))
+        it += 1
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} iterations done")
+
+    first_timed = it
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        tracer.execute(enc, payload, it, reset=False)
+        ends[k].record(stream)
+        it += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    total_ms = starts[0].elapsed_time(ends[-1])
+    avg_launch_ms = float(np.mean(launch_ms))
+    log(f"timed {args.steps} iterations: {total_ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)")
+
+    # counted replay of the same iterations on a scratch copy (counts depend
+    # only on the iteration, not on the accumulated values)
+    scratch = payload.clone()
+    counters = torch.zeros(8, dtype=torch.int64, device=device.torch_device)
+    for k in range(args.steps):
+        tracer.execute(enc, scratch, first_timed + k, reset=False, counters=counters)
+    torch.cuda.synchronize()
+    c = counters.cpu().numpy().astype(np.int64) // args.steps
+    del scratch
+    ext, shd, nodes, tris, slots = (int(c[_native.COUNTER_EXTEND_RAYS]), int(c[_native.COUNTER_SHADOW_RAYS]),
+                                    int(c[_native.COUNTER_NODE_VISITS]), int(c[_native.COUNTER_TRI_TESTS]),
+                                    int(c[_native.COUNTER_SLOT_UPDATES]))
+    front = layout.total_surface_texels
+    # shaded hits: every texel surface (front/back) + every extend ray that hit
+    algo_bytes = (nodes * BYTES_PER_NODE + tris * BYTES_PER_TRI + slots * BYTES_PER_SLOT_UPDATE
+                  + front * BYTES_PER_TEXEL_SETUP + (slots + ext) * BYTES_PER_SHADED_HIT)
+    samples_per_s = slots / (avg_launch_ms * 1e-3)
+    value = slots * args.steps / (total_ms * 1e-3)
+    mrays = (ext + shd) * args.steps / (total_ms * 1e-3) / 1e6
+    achieved = algo_bytes / (avg_launch_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "texel-path samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "triangles": node.triangle_count,
+            "instances": len(node.instances),
+            "payload_slots": layout.total_payload_count,
+            "surface_texels": layout.total_surface_texels,
+            "layout": layout_kw,
+            "samples_per_texel": 1,
+            "max_bounces": 3,
+            "parallelism": "single GPU",
+        },
+        "mrays_per_s": mrays,
+        "rays_per_step": {"extend": ext, "shadow": shd},
+        "traversal_per_step": {"node_visits": nodes, "tri_tests": tris},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "tst_megakernel",
+            "algorithmic_bytes_per_launch": algo_bytes,
+            "avg_launch_ms": avg_launch_ms,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, 16)
+        log(f"cpu baseline on {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(node, layout, threads, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,156 @@
+/*
+ * nan_hip.h - C ABI of libnan_hip.so, the MI355X (gfx950) texture-space
+ * path tracer.
+ *
+ * Conventions follow the reference's native helpers: extern "C", int status
+ * (0 = success), an optional caller error buffer (char* err, size_t cap;
+ * surface_probe_sampler.h:120-129) plus nan_last_error()
+ * (vertex_baking_utils.h:11-23), opaque handles with explicit destroy
+ * functions (static_shadow_tree_encoder.h:93-115).  Device pointers are
+ * plain pointers into HIP device memory (for instance torch tensors);
+ * `stream` is a hipStream_t passed as void* (NULL = default stream).  The
+ * library never frees caller memory.
+ */
+#ifndef NAN_HIP_H
+#define NAN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NAN_API __attribute__((visibility("default")))
+
+#define NAN_HIP_ABI_VERSION 1
+
+/* Scene input: the buffers the reference's Scene uploads (scene.py:330-437).
+ *   vertices        float[V][8]  object-space position, normal, uv
+ *   indices         uint32[3T]   mesh-local, meshes concatenated
+ *   mesh_descs      uint32[M][4] vertex_count, index_count, vertex_offset, index_offset
+ *   instance_descs  uint32[I][4] mesh_id, material_id, transform_id, flags (bit0 odd-negative)
+ *   transforms      float[X][16] row-major object-to-world
+ *   inv_transposes  float[X][16] row-major transpose(inverse(M))
+ *   materials       uint32[Mat][8] base_color.rgb (f32), flags (bit0 double-sided),
+ *                                  emissive.rgb (f32), alpha_mode
+ *   lighting        float[16]    sun dir.xyz, cos_half_angle, color*intensity.rgb, pad,
+ *                                transmittance.rgb, pad, sky radiance.rgb, pad          */
+typedef struct NanSceneDesc {
+    const float *vertices;
+    uint32_t vertex_count;
+    const uint32_t *indices;
+    uint32_t index_count;
+    const uint32_t *mesh_descs;
+    uint32_t mesh_count;
+    const uint32_t *instance_descs;
+    uint32_t instance_count;
+    const float *transforms;
+    const float *inv_transposes;
+    uint32_t transform_count;
+    const uint32_t *materials;
+    uint32_t material_count;
+    const float *lighting;
+} NanSceneDesc;
+
+/* BVH build options (0 selects the default). */
+typedef struct NanBvhOptions {
+    uint32_t sah_bins;       /* binned SAH bins, default 16 */
+    uint32_t max_leaf_size;  /* triangles per leaf, 1..8, default 4 */
+    uint32_t build_threads;  /* host threads for the build, default hardware concurrency */
+} NanBvhOptions;
+
+typedef struct NanSceneInfo {
+    uint64_t triangle_count;
+    uint64_t node_count;       /* 4-wide 64-byte nodes */
+    uint64_t leaf_count;
+    uint64_t device_bytes;     /* BVH + triangle + shading buffers */
+    uint32_t max_stack;        /* worst-case traversal stack entries */
+    uint32_t stack_depth;      /* stack compiled into the kernels in use */
+    double build_ms;
+    float sah_cost;
+} NanSceneInfo;
+
+typedef struct NanScene NanScene;
+typedef struct NanTracer NanTracer;
+
+/* Counter slots written by the *_counted entry points (uint64 each). */
+enum {
+    NAN_COUNTER_EXTEND_RAYS = 0,
+    NAN_COUNTER_SHADOW_RAYS = 1,
+    NAN_COUNTER_NODE_VISITS = 2,
+    NAN_COUNTER_TRI_TESTS = 3,
+    NAN_COUNTER_SLOT_UPDATES = 4,
+    NAN_COUNTER_COUNT = 8
+};
+
+NAN_API int nan_version(void);
+NAN_API const char *nan_last_error(void);
+
+/* Builds the BVH on the host and uploads scene buffers (replaces Scene.__init__
+ * BLAS/TLAS builds, scene.py:440-533). */
+NAN_API int nan_scene_create(const NanSceneDesc *desc, const NanBvhOptions *options, NanScene **out,
+                             char *err, size_t cap);
+NAN_API void nan_scene_destroy(NanScene *scene);
+NAN_API int nan_scene_info(const NanScene *scene, NanSceneInfo *out);
+/* Replaces the lighting record (scene.slang:7-17 uniforms). */
+NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16);
+
+/* Closest hit and occlusion for rays float[n][8] (origin, tmin, dir, tmax) in device
+ * memory; writes hits float[n][4] (t, b1, b2, 0; t < 0 on a miss), ids uint32[n][2]
+ * (instance, primitive; 0xFFFFFFFF on a miss) and occluded uint32[n].  Replaces
+ * Scene.intersect / get_visibility (scene.slang:1017-1172). */
+NAN_API int nan_trace_rays(NanScene *scene, const float *rays, uint64_t n, float *hits, uint32_t *ids,
+                           uint32_t *occluded, void *stream);
+
+/* Uploads a Mesh Colors layout (mesh_colors.py:299-337: face uint2, instance uint4,
+ * side uint2 tables) and creates a tracer over scene (TextureSpacePathTracer.__init__,
+ * texture_space_path_tracer.py:8-31). */
+NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint32_t face_count,
+                              const uint32_t *instance_table, const uint32_t *side_table,
+                              uint32_t instance_count, uint64_t total_surface_texels,
+                              uint64_t total_payload_count, NanTracer **out, char *err, size_t cap);
+NAN_API void nan_tracer_destroy(NanTracer *tracer);
+/* Device copies of the layout tables, for the resolve and filter passes
+ * (texture_space_path_tracer.py:26-31 face/instance/side buffers). */
+NAN_API int nan_tracer_tables(const NanTracer *tracer, const void **face_table, const void **instance_table,
+                              const void **side_table);
+
+/* One progressive iteration over every texel of every instance
+ * (TextureSpacePathTracer.execute, texture_space_path_tracer.py:33-59; kernel
+ * texture_space_path_tracer.slang:140-216).  payload: 16-byte slots
+ * {f32 r,g,b; u32 count}[total_payload_count] in device memory. */
+NAN_API int nan_tst_execute(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,
+                            uint32_t max_bounces, uint32_t reset, void *stream);
+/* Same, adding per-launch work counters (NAN_COUNTER_*) into counters
+ * (uint64[NAN_COUNTER_COUNT], device memory, caller zeroes). */
+NAN_API int nan_tst_execute_counted(NanTracer *tracer, void *payload, uint32_t iteration,
+                                    uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
+                                    uint64_t *counters, void *stream);
+/* Texel ray generation only (texture_space_path_tracer.slang:142-199): per flat front
+ * texel writes uint32[4] = instance, primitive, bits of bary.x, bits of bary.y and
+ * uint32 RNG state after the jitter draws (rng_state may be NULL). */
+NAN_API int nan_tst_raygen(NanTracer *tracer, uint32_t iteration, uint32_t samples_per_texel, uint32_t *out,
+                           uint32_t *rng_state, void *stream);
+
+/* RGB9E5 freeze of n slots (mesh_colors_rgb9e5_pack.slang:7-16). */
+NAN_API int nan_rgb9e5_pack(const void *payload, uint32_t *packed, uint64_t n, void *stream);
+/* Decode n words to float[n][4] (mesh_colors.slang:246-253). */
+NAN_API int nan_rgb9e5_unpack(const uint32_t *packed, float *out, uint64_t n, void *stream);
+
+/* Primary-ray resolve into out float[h][w][4] (mesh_colors_resolve.slang:29-73;
+ * frozen != 0 reads RGB9E5 words, mesh_colors_frozen_resolve.slang:92-135).
+ * camera16 (host memory): position, jitter_x, image_u, jitter_y, image_v, near, image_w, far. */
+NAN_API int nan_resolve(NanTracer *tracer, const void *cache, int frozen, const float *camera16, float *out,
+                        uint32_t width, uint32_t height, void *stream);
+
+/* Screen accumulation (accumulator.slang:3-29) and LPM tone map + Rec.709
+ * (tone_mapper.slang:8-90) over n float4 pixels. */
+NAN_API int nan_accumulate(const float *input, float *accum, float *output, uint64_t n, int reset, void *stream);
+NAN_API int nan_tonemap(const float *input, float *output, uint64_t n, float exposure, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NAN_HIP_H */

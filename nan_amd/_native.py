@@ -1,0 +1,129 @@
+"""ctypes binding of ``libnan_hip.so`` (declared in ``include/nan_hip.h``).
+
+Loading follows the reference's native helpers (surface_probe_sampler.py:
+251-350): a CDLL with typed signatures and a typed Python exception carrying
+the library's error text.  There is no fallback: if the library is missing
+or cannot be loaded, :func:`lib` raises :class:`NativeLibraryError`.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libnan_hip.so"
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+f32p = ctypes.POINTER(ctypes.c_float)
+vp = ctypes.c_void_p
+
+
+class NativeLibraryError(RuntimeError):
+    """libnan_hip.so is missing or failed to load."""
+
+
+class NanError(RuntimeError):
+    """A libnan_hip call returned a non-zero status."""
+
+
+class NanSceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("vertices", f32p), ("vertex_count", ctypes.c_uint32),
+        ("indices", u32p), ("index_count", ctypes.c_uint32),
+        ("mesh_descs", u32p), ("mesh_count", ctypes.c_uint32),
+        ("instance_descs", u32p), ("instance_count", ctypes.c_uint32),
+        ("transforms", f32p), ("inv_transposes", f32p), ("transform_count", ctypes.c_uint32),
+        ("materials", u32p), ("material_count", ctypes.c_uint32),
+        ("lighting", f32p),
+    ]
+
+
+class NanBvhOptions(ctypes.Structure):
+    _fields_ = [("sah_bins", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
+                ("build_threads", ctypes.c_uint32)]
+
+
+class NanSceneInfo(ctypes.Structure):
+    _fields_ = [
+        ("triangle_count", ctypes.c_uint64), ("node_count", ctypes.c_uint64), ("leaf_count", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64), ("max_stack", ctypes.c_uint32), ("stack_depth", ctypes.c_uint32),
+        ("build_ms", ctypes.c_double), ("sah_cost", ctypes.c_float),
+    ]
+
+
+COUNTER_EXTEND_RAYS = 0
+COUNTER_SHADOW_RAYS = 1
+COUNTER_NODE_VISITS = 2
+COUNTER_TRI_TESTS = 3
+COUNTER_SLOT_UPDATES = 4
+COUNTER_COUNT = 8
+
+# every symbol include/nan_hip.h declares
+EXPORTED_SYMBOLS = (
+    "nan_version", "nan_last_error", "nan_scene_create", "nan_scene_destroy", "nan_scene_info",
+    "nan_scene_set_lighting", "nan_trace_rays", "nan_tracer_create", "nan_tracer_destroy", "nan_tracer_tables",
+    "nan_tst_execute", "nan_tst_execute_counted", "nan_tst_raygen", "nan_rgb9e5_pack", "nan_rgb9e5_unpack",
+    "nan_resolve", "nan_accumulate", "nan_tonemap",
+)
+
+_LIB = None
+
+
+def _declare(lib) -> None:
+    I = ctypes.c_int
+    U32 = ctypes.c_uint32
+    U64 = ctypes.c_uint64
+    sig = {
+        "nan_version": (I, []),
+        "nan_last_error": (ctypes.c_char_p, []),
+        "nan_scene_create": (I, [ctypes.POINTER(NanSceneDesc), ctypes.POINTER(NanBvhOptions), ctypes.POINTER(vp),
+                                 ctypes.c_char_p, ctypes.c_size_t]),
+        "nan_scene_destroy": (None, [vp]),
+        "nan_scene_info": (I, [vp, ctypes.POINTER(NanSceneInfo)]),
+        "nan_scene_set_lighting": (I, [vp, f32p]),
+        "nan_trace_rays": (I, [vp, vp, U64, vp, vp, vp, vp]),
+        "nan_tracer_create": (I, [vp, u32p, U32, u32p, u32p, U32, U64, U64, ctypes.POINTER(vp), ctypes.c_char_p,
+                                  ctypes.c_size_t]),
+        "nan_tracer_destroy": (None, [vp]),
+        "nan_tracer_tables": (I, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "nan_tst_execute": (I, [vp, vp, U32, U32, U32, U32, vp]),
+        "nan_tst_execute_counted": (I, [vp, vp, U32, U32, U32, U32, vp, vp]),
+        "nan_tst_raygen": (I, [vp, U32, U32, vp, vp, vp]),
+        "nan_rgb9e5_pack": (I, [vp, vp, U64, vp]),
+        "nan_rgb9e5_unpack": (I, [vp, vp, U64, vp]),
+        "nan_resolve": (I, [vp, vp, I, f32p, vp, U32, U32, vp]),
+        "nan_accumulate": (I, [vp, vp, vp, U64, I, vp]),
+        "nan_tonemap": (I, [vp, vp, U64, ctypes.c_float, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """Loads libnan_hip.so once; raises NativeLibraryError if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise NativeLibraryError(
+            f"{LIB_PATH} is not built; run `python -m nan_amd.build` (hipcc, gfx950)")
+    try:
+        handle = ctypes.CDLL(str(LIB_PATH))
+    except OSError as exc:
+        raise NativeLibraryError(f"cannot load {LIB_PATH}: {exc}") from exc
+    _declare(handle)
+    _LIB = handle
+    return handle
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().nan_last_error()
+        raise NanError(f"{what}: {msg.decode() if msg else 'unknown error'}")
+
+
+def err_buffer():
+    return ctypes.create_string_buffer(512)

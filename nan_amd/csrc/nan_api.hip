@@ -1,0 +1,960 @@
+// libnan_hip: kernels and C ABI of the MI355X texture-space path tracer.
+//
+// Kernels (one HIP thread per unit of work, wave64, LDS traversal stacks):
+//   tst_megakernel  one progressive iteration over all front texels
+//                   (texture_space_path_tracer.slang:140-216)
+//   tst_raygen      texel ray generation only (slang:142-199)
+//   trace_rays      closest hit + occlusion for a ray batch (scene.slang:1017-1172)
+//   rgb9e5_pack     float3 slot -> uint32 (mesh_colors_rgb9e5_pack.slang:7-16)
+//   resolve         primary rays into the camera (mesh_colors_resolve.slang:29-73)
+//   accumulate / tonemap (accumulator.slang, tone_mapper.slang)
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "nan_bvh.h"
+#include "nan_device.h"
+#include "../../include/nan_hip.h"
+
+using namespace nan;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(char *err, size_t cap, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    if (err && cap) {
+        strncpy(err, buf, cap - 1);
+        err[cap - 1] = '\0';
+    }
+    return -1;
+}
+
+#define HIP_TRY(expr, err, cap)                                                                  \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess) return fail(err, cap, "%s: %s", #expr, hipGetErrorString(_e));     \
+    } while (0)
+
+template <typename T>
+int upload(T **dst, const void *src, size_t bytes, char *err, size_t cap) {
+    *dst = nullptr;
+    if (bytes == 0) bytes = 16;
+    HIP_TRY(hipMalloc((void **)dst, bytes), err, cap);
+    if (src) HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), err, cap);
+    return 0;
+}
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kTraceBlock = 128;
+constexpr int kFlatBlock = 256;
+
+}  // namespace
+
+struct NanScene {
+    DevScene dev{};
+    std::vector<void *> allocations;
+    NanSceneInfo info{};
+    uint32_t stack_depth = 0;
+    std::vector<uint32_t> instance_triangles;  // triangles of each instance's mesh
+    ~NanScene() {
+        for (void *p : allocations) hipFree(p);
+    }
+};
+
+struct NanTracer {
+    NanScene *scene = nullptr;
+    uint2 *faces = nullptr;
+    uint4 *instances = nullptr;
+    uint2 *sides = nullptr;
+    uint32_t *texel_face = nullptr;
+    uint32_t *face_inst = nullptr;
+    uint32_t *front_prefix = nullptr;
+    uint32_t face_count = 0, instance_count = 0;
+    uint64_t total_front = 0, total_payload = 0;
+    ~NanTracer() {
+        hipFree(faces);
+        hipFree(instances);
+        hipFree(sides);
+        hipFree(texel_face);
+        hipFree(face_inst);
+        hipFree(front_prefix);
+    }
+};
+
+struct DevLayout {
+    const uint2 *faces;
+    const uint4 *instances;
+    const uint2 *sides;
+    const uint32_t *texel_face;
+    const uint32_t *face_inst;
+    const uint32_t *front_prefix;
+    uint64_t total_front;
+};
+
+// ---------------------------------------------------------------------------
+// path tracing device functions
+// ---------------------------------------------------------------------------
+struct PathCounters {
+    uint32_t extend, shadow, nodes, tris, slots;
+};
+
+template <int STACK, int BLOCK, bool COUNT>
+__device__ __forceinline__ bool shadow_visible(const DevScene &S, f3 origin, f3 dir, uint32_t *col,
+                                               PathCounters &pc) {
+    Ray r;
+    r.o = origin;
+    r.d = dir;
+    r.tmin = 0.0f;
+    r.tmax = 1000.0f;
+    ray_setup(r);
+    Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+    TraversalStats st{0, 0};
+    bool occluded = traverse<STACK, BLOCK, true, COUNT>(S, r, h, col, st);
+    if (COUNT) {
+        pc.shadow++;
+        pc.nodes += st.nodes;
+        pc.tris += st.tris;
+    }
+    return !occluded;
+}
+
+// trace_radiance (texture_space_path_tracer.slang:57-84)
+template <int STACK, int BLOCK, bool COUNT>
+__device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_bounces, Rng &rng, uint32_t *col,
+                             PathCounters &pc) {
+    const Light &L = S.light;
+    f3 T = mk(1.0f, 1.0f, 1.0f), rad = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t bounce = 0; bounce < max_bounces; ++bounce) {
+        Ray r;
+        r.o = origin;
+        r.d = dir;
+        r.tmin = 0.0f;
+        r.tmax = 1000.0f;
+        ray_setup(r);
+        Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+        TraversalStats st{0, 0};
+        bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
+        if (COUNT) {
+            pc.extend++;
+            pc.nodes += st.nodes;
+            pc.tris += st.tris;
+        }
+        if (!hit) return rad + T * L.sky;
+        uint2 ids = S.tri_ids[h.id];
+        Shading sd;
+        shading_data(S, ids.x, ids.y, h.b1, h.b2, -dir, sd);
+        rad = rad + T * sd.emissive;
+        float ux = rng.next_1d(), uy = rng.next_1d();
+        f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+        float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc)) {
+            f3 f = sd.diffuse * kInvPi;
+            f3 li = (f * L.sun_illum) * ndl;
+            rad = rad + T * li;
+        }
+        float ex = rng.next_1d(), ey = rng.next_1d();
+        f3 sdir;
+        float pdf;
+        cosine_sample(ex, ey, sdir, pdf);
+        if (pdf <= 1e-6f) return rad;
+        f3 f = sd.diffuse * kInvPi;
+        float az = fabsf(sdir.z);
+        T = T * mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf);
+        origin = sd.origin_front;
+        dir = to_global(sd.frame, sdir);
+    }
+    return rad;
+}
+
+// estimate_surface_irradiance (texture_space_path_tracer.slang:86-117)
+template <int STACK, int BLOCK, bool COUNT>
+__device__ f3 estimate_irradiance(const DevScene &S, uint32_t instance, uint32_t prim, float b1, float b2, f3 view,
+                                  uint32_t spp, uint32_t max_bounces, Rng &rng, uint32_t *col, PathCounters &pc) {
+    const Light &L = S.light;
+    Shading sd;
+    shading_data(S, instance, prim, b1, b2, view, sd);
+    f3 acc = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = 0; k < spp; ++k) {
+        f3 sample = mk(0.0f, 0.0f, 0.0f);
+        float ux = rng.next_1d(), uy = rng.next_1d();
+        f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+        float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+            sample = L.sun_illum * ndl;
+        float ex = rng.next_1d(), ey = rng.next_1d();
+        f3 sdir;
+        float pdf;
+        cosine_sample(ex, ey, sdir, pdf);
+        if (pdf > 1e-6f) {
+            f3 rad = trace_radiance<STACK, BLOCK, COUNT>(S, sd.origin_front, to_global(sd.frame, sdir), max_bounces,
+                                                         rng, col, pc);
+            float az = fabsf(sdir.z);
+            sample = sample + mk(rad.x * az / pdf, rad.y * az / pdf, rad.z * az / pdf);
+        }
+        acc = acc + sample;
+    }
+    float fs = (float)spp;
+    return mk(acc.x / fs, acc.y / fs, acc.z / fs);
+}
+
+// store_accumulated_irradiance (texture_space_path_tracer.slang:119-138)
+__device__ __forceinline__ void store_accumulated(float4 *payload, uint32_t slot, f3 e, uint32_t reset) {
+    float4 p = payload[slot];
+    uint32_t prev_count = reset ? 0u : __float_as_uint(p.w);
+    f3 prev = prev_count == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(p.x, p.y, p.z);
+    uint32_t next = prev_count + 1 < kMaxTexelSamples ? prev_count + 1 : kMaxTexelSamples;
+    float w = 1.0f / (float)(prev_count + 1);
+    float4 o;
+    o.x = prev.x + w * (e.x - prev.x);
+    o.y = prev.y + w * (e.y - prev.y);
+    o.z = prev.z + w * (e.z - prev.z);
+    o.w = __uint_as_float(next);
+    payload[slot] = o;
+}
+
+struct TexelSurface {
+    uint32_t instance, prim, global_texel, local;
+    float b1, b2;
+    uint2 side;
+};
+
+// compute_main prologue (texture_space_path_tracer.slang:142-199) for flat
+// front texel w; leaves rng after the jitter draws.
+__device__ __forceinline__ TexelSurface texel_surface(const DevLayout &Ly, uint64_t w, uint32_t iteration,
+                                                      uint32_t spp, Rng &rng) {
+    TexelSurface ts;
+    uint32_t f = Ly.texel_face[w];
+    uint32_t inst = Ly.face_inst[f];
+    uint4 ii = Ly.instances[inst];
+    uint2 face = Ly.faces[f];
+    ts.side = Ly.sides[inst];
+    ts.instance = inst;
+    ts.prim = f - ii.x;
+    ts.local = (uint32_t)(w - Ly.front_prefix[inst]);
+    ts.global_texel = ii.z + ts.local;
+    uint32_t i, j;
+    address_to_ij(ts.local - face.x, face.y, i, j);
+    float fr = (float)face.y;
+    float base_x = (float)i / fr, base_y = (float)j / fr;
+    rng.seed(ts.global_texel, inst, iteration * (spp > 1u ? spp : 1u));
+    float bx = base_x, by = base_y;
+    for (int attempt = 0; attempt < 32; ++attempt) {
+        float ux = rng.next_1d();
+        float uy = rng.next_1d();
+        float cx = base_x + (ux - 0.5f) / fr;
+        float cy = base_y + (uy - 0.5f) / fr;
+        if (cx < 0.0f || cy < 0.0f || cx + cy > 1.0f) continue;
+        f3 cb = mk(cx, cy, 1.0f - cx - cy);
+        if (nearest_texel(cb, ii.z, face) == ts.global_texel) {
+            bx = cx;
+            by = cy;
+            break;
+        }
+    }
+    ts.b1 = by;
+    ts.b2 = fmaxf(0.0f, 1.0f - bx - by);
+    return ts;
+}
+
+__device__ __forceinline__ void flush_counters(const PathCounters &pc, unsigned long long *counters) {
+    __shared__ unsigned int block_sum[5];
+    if (threadIdx.x < 5) block_sum[threadIdx.x] = 0;
+    __syncthreads();
+    atomicAdd(&block_sum[0], pc.extend);
+    atomicAdd(&block_sum[1], pc.shadow);
+    atomicAdd(&block_sum[2], pc.nodes);
+    atomicAdd(&block_sum[3], pc.tris);
+    atomicAdd(&block_sum[4], pc.slots);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&counters[NAN_COUNTER_EXTEND_RAYS], (unsigned long long)block_sum[0]);
+        atomicAdd(&counters[NAN_COUNTER_SHADOW_RAYS], (unsigned long long)block_sum[1]);
+        atomicAdd(&counters[NAN_COUNTER_NODE_VISITS], (unsigned long long)block_sum[2]);
+        atomicAdd(&counters[NAN_COUNTER_TRI_TESTS], (unsigned long long)block_sum[3]);
+        atomicAdd(&counters[NAN_COUNTER_SLOT_UPDATES], (unsigned long long)block_sum[4]);
+    }
+}
+
+template <int STACK, int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void tst_megakernel(DevScene S, DevLayout Ly, float4 *payload,
+                                                        uint32_t iteration, uint32_t spp, uint32_t max_bounces,
+                                                        uint32_t reset, unsigned long long *counters) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    uint32_t *col = stack + threadIdx.x;
+    PathCounters pc{0, 0, 0, 0, 0};
+    uint64_t w = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (w < Ly.total_front) {
+        Rng rng;
+        TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
+        uint32_t nspp = spp > 1u ? spp : 1u;
+        f3 n = world_flat_normal(S, ts.instance, ts.prim);
+        f3 e = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.instance, ts.prim, ts.b1, ts.b2, n, nspp, max_bounces,
+                                                        rng, col, pc);
+        store_accumulated(payload, ts.global_texel, e, reset);
+        if (COUNT) pc.slots++;
+        if (ts.side.y & 1u) {
+            f3 eb = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.instance, ts.prim, ts.b1, ts.b2, -n, nspp,
+                                                             max_bounces, rng, col, pc);
+            store_accumulated(payload, ts.side.x + ts.local, eb, reset);
+            if (COUNT) pc.slots++;
+        }
+    }
+    if (COUNT) flush_counters(pc, counters);
+}
+
+__global__ void tst_raygen_kernel(DevLayout Ly, uint32_t iteration, uint32_t spp, uint4 *out, uint32_t *rng_out) {
+    uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= Ly.total_front) return;
+    Rng rng;
+    TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
+    out[w] = make_uint4(ts.instance, ts.prim, __float_as_uint(ts.b1), __float_as_uint(ts.b2));
+    if (rng_out) rng_out[w] = rng.state;
+}
+
+template <int STACK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void trace_rays_kernel(DevScene S, const float4 *rays, uint64_t n, float4 *hits,
+                                                           uint2 *ids, uint32_t *occluded) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    uint32_t *col = stack + threadIdx.x;
+    uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    float4 a = rays[2 * k], b = rays[2 * k + 1];
+    Ray r;
+    r.o = mk(a.x, a.y, a.z);
+    r.tmin = a.w;
+    r.d = mk(b.x, b.y, b.z);
+    r.tmax = b.w;
+    ray_setup(r);
+    Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+    TraversalStats st{0, 0};
+    bool hit = traverse<STACK, BLOCK, false, false>(S, r, h, col, st);
+    if (hit) {
+        hits[k] = make_float4(h.t, h.b1, h.b2, 0.0f);
+        ids[k] = S.tri_ids[h.id];
+    } else {
+        hits[k] = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+        ids[k] = make_uint2(kInvalid, kInvalid);
+    }
+    if (occluded) {
+        Hit o{r.tmax, 0.0f, 0.0f, kInvalid};
+        occluded[k] = traverse<STACK, BLOCK, true, false>(S, r, o, col, st) ? 1u : 0u;
+    }
+}
+
+__global__ void rgb9e5_pack_kernel(const float4 *payload, uint32_t *out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = payload[i];
+    out[i] = pack_rgb9e5(p.x, p.y, p.z);
+}
+
+__global__ void rgb9e5_unpack_kernel(const uint32_t *in, float4 *out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 v = unpack_rgb9e5(in[i]);
+    out[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// resolve
+// ---------------------------------------------------------------------------
+struct DevCamera {
+    f3 pos, u, v, w;
+    float jx, jy, near, far;
+};
+
+__device__ __forceinline__ uint32_t side_texel(uint32_t front, uint32_t texel_offset, uint2 side, bool back) {
+    return back ? side.x + (front - texel_offset) : front;
+}
+
+template <bool FROZEN>
+__device__ __forceinline__ f3 load_cache(const void *cache, uint32_t slot) {
+    if (FROZEN) return unpack_rgb9e5(static_cast<const uint32_t *>(cache)[slot]);
+    float4 p = static_cast<const float4 *>(cache)[slot];
+    return mk(p.x, p.y, p.z);
+}
+
+// mesh_colors_sample_irradiance (mesh_colors.slang:365-426)
+template <bool FROZEN>
+__device__ f3 sample_irradiance(const void *cache, f3 b, uint32_t texel_offset, uint2 face, uint2 side, bool back) {
+    uint32_t R = face.y;
+    float fr = (float)R;
+    f3 sc = mk(saturate(b.x) * fr, saturate(b.y) * fr, saturate(b.z) * fr);
+    uint32_t i = (uint32_t)sc.x, j = (uint32_t)sc.y;
+    i = i > R ? R : i;
+    j = j > R - i ? R - i : j;
+    f3 w = mk(fracf(sc.x), fracf(sc.y), fracf(sc.z));
+    bool upper = fabsf(w.x + w.y + w.z - 2.0f) <= 1e-3f;
+    if (upper) w = mk(1.0f - w.x, 1.0f - w.y, 1.0f - w.z);
+    if (w.x + w.y + w.z <= 1e-6f || i >= R || j >= R - i)
+        return load_cache<FROZEN>(cache, side_texel(lattice_address(i, j, texel_offset, face), texel_offset, side, back));
+    uint32_t ta, tb, tc;
+    if (upper) {
+        ta = lattice_address(i, j + 1, texel_offset, face);
+        tb = lattice_address(i + 1, j, texel_offset, face);
+        tc = lattice_address(i + 1, j + 1, texel_offset, face);
+    } else {
+        ta = lattice_address(i + 1, j, texel_offset, face);
+        tb = lattice_address(i, j + 1, texel_offset, face);
+        tc = lattice_address(i, j, texel_offset, face);
+    }
+    f3 A = load_cache<FROZEN>(cache, side_texel(ta, texel_offset, side, back));
+    f3 B = load_cache<FROZEN>(cache, side_texel(tb, texel_offset, side, back));
+    f3 C = load_cache<FROZEN>(cache, side_texel(tc, texel_offset, side, back));
+    return A * w.x + B * w.y + C * w.z;
+}
+
+__device__ f3 sky_with_sun_disk(const Light &L, f3 dir) {
+    f3 out = L.sky;
+    const float ground = 6.360f, vy = 6.360f + 0.0002f;
+    f3 oc = mk(0.0f, vy, 0.0f);
+    float a = dot(dir, dir), bq = 2.0f * dot(oc, dir), c = dot(oc, oc) - ground * ground;
+    float disc = bq * bq - 4.0f * a * c;
+    bool hits_ground = false;
+    if (disc >= 0.0f) {
+        float sq = sqrtf(disc);
+        float t0 = (-bq - sq) / (2.0f * a), t1 = (-bq + sq) / (2.0f * a);
+        hits_ground = t0 > 0.0f || t1 > 0.0f;
+    }
+    if (!hits_ground && L.sun_on) {
+        float vdl = dot(dir, L.sun_dir);
+        if (vdl > L.cos_half) {
+            float solid = 2.0f * kPi * (1.0f - L.cos_half);
+            float soft = saturate(2.0f * (vdl - L.cos_half) / (1.0f - L.cos_half));
+            f3 disk = (L.transmittance * mk(L.outer.x / solid, L.outer.y / solid, L.outer.z / solid)) * soft;
+            out = out + disk;
+        }
+    }
+    return out;
+}
+
+template <int STACK, int BLOCK, bool FROZEN>
+__global__ __launch_bounds__(BLOCK) void resolve_kernel(DevScene S, DevLayout Ly, const void *cache, DevCamera cam,
+                                                        float4 *out, uint32_t width, uint32_t height) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    uint32_t *col = stack + threadIdx.x;
+    uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= (uint64_t)width * height) return;
+    uint32_t x = (uint32_t)(k % width), y = (uint32_t)(k / width);
+    float px = ((float)x + 0.5f) / (float)width;
+    float py = ((float)y + 0.5f) / (float)height;
+    px = px + -cam.jx;
+    py = py + cam.jy;
+    float nx = 2.0f * px + -1.0f, ny = -2.0f * py + 1.0f;
+    f3 d = normalize(cam.u * nx + cam.v * ny + cam.w);
+    float inv_cos = 1.0f / dot(normalize(cam.w), d);
+    Ray r;
+    r.o = cam.pos;
+    r.d = d;
+    r.tmin = cam.near * inv_cos;
+    r.tmax = cam.far * inv_cos;
+    ray_setup(r);
+    Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+    TraversalStats st{0, 0};
+    f3 color;
+    if (!traverse<STACK, BLOCK, false, false>(S, r, h, col, st)) {
+        color = sky_with_sun_disk(S.light, d);
+    } else {
+        uint2 ids = S.tri_ids[h.id];
+        f3 fn = world_flat_normal(S, ids.x, ids.y);
+        bool back = dot(fn, -d) <= 0.0f;
+        uint4 ii = Ly.instances[ids.x];
+        uint2 side = Ly.sides[ids.x];
+        if (back && !(side.y & 1u)) {
+            color = mk(0.0f, 0.0f, 0.0f);
+        } else {
+            uint2 face = Ly.faces[ii.x + ids.y];
+            f3 bc = mk(1.0f - h.b1 - h.b2, h.b1, h.b2);
+            f3 E = sample_irradiance<FROZEN>(cache, bc, ii.z, face, side, back);
+            Shading sd;
+            shading_data(S, ids.x, ids.y, h.b1, h.b2, -d, sd);
+            color = sd.emissive + (sd.diffuse * kInvPi) * E;
+        }
+    }
+    out[k] = make_float4(color.x, color.y, color.z, 1.0f);
+}
+
+__global__ void accumulate_kernel(const float4 *in, float4 *accum, float4 *out, uint64_t n, int reset) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 v = reset ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : accum[i];
+    float4 x = in[i];
+    v.x += x.x;
+    v.y += x.y;
+    v.z += x.z;
+    v.w += 1.0f;
+    accum[i] = v;
+    out[i] = make_float4(v.x / v.w, v.y / v.w, v.z / v.w, 1.0f);
+}
+
+// LpmToneMap + LinearToRec709 (tone_mapper.slang:8-81)
+__device__ f3 lpm_tonemap(f3 color, float hdr_max, float exposure, float contrast) {
+    const f3 luma709 = mk(0.2126f, 0.7152f, 0.0722f);
+    const f3 crosstalk = mk(1.0f, 0.5f, 0.0625f);
+    float mid_in = hdr_max * 0.18f * exp2f(-exposure);
+    float mid_out = 0.18f;
+    float cs = contrast;
+    float z0 = -powf(mid_in, contrast);
+    float z1 = powf(hdr_max, cs) * powf(mid_in, contrast);
+    float z2 = powf(hdr_max, contrast) * powf(mid_in, cs) * mid_out;
+    float z3 = powf(hdr_max, cs) * mid_out;
+    float z4 = powf(mid_in, cs) * mid_out;
+    float bias_x = -((z0 + (mid_out * (z1 - z2)) / (z3 - z4)) / z4);
+    float w0 = powf(hdr_max, cs) * powf(mid_in, contrast);
+    float w1 = powf(hdr_max, contrast) * powf(mid_in, cs) * mid_out;
+    float w2 = powf(hdr_max, cs) * mid_out;
+    float w3 = powf(mid_in, cs) * mid_out;
+    float bias_y = (w0 - w1) / (w2 - w3);
+    float maxc = fmaxf(fmaxf(color.x, color.y), color.z);
+    if (maxc <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    float rcp = 1.0f / maxc;
+    f3 ratio = color * rcp;
+    float luma = dot(color, luma709);
+    luma = powf(fmaxf(luma, 0.0f), contrast);
+    luma = luma / (luma * bias_x + bias_y);
+    float luma_ratio = dot(ratio, luma709);
+    float ratio_scale = saturate(luma / fmaxf(luma_ratio, 0.00001f));
+    f3 o = mk(saturate(ratio.x * ratio_scale), saturate(ratio.y * ratio_scale), saturate(ratio.z * ratio_scale));
+    f3 cap = crosstalk * mk(1.0f - o.x, 1.0f - o.y, 1.0f - o.z);
+    float luma_add = saturate(luma - dot(o, luma709));
+    float cap_luma = dot(cap, luma709);
+    if (cap_luma > 0.00001f) {
+        float t = luma_add / cap_luma;
+        o = mk(saturate(o.x + t * cap.x), saturate(o.y + t * cap.y), saturate(o.z + t * cap.z));
+    }
+    luma_add = saturate(luma - dot(o, luma709));
+    o = mk(saturate(o.x + luma_add * (1.0f / luma709.x)), saturate(o.y + luma_add * (1.0f / luma709.y)),
+           saturate(o.z + luma_add * (1.0f / luma709.z)));
+    return o;
+}
+
+__device__ __forceinline__ float rec709(float x) { return x < 0.018f ? 4.5f * x : 1.099f * powf(x, 0.45f) - 0.099f; }
+
+__global__ void tonemap_kernel(const float4 *in, float4 *out, uint64_t n, float exposure) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 x = in[i];
+    f3 t = lpm_tonemap(mk(x.x, x.y, x.z), 12.0f, exposure, 1.0f);
+    out[i] = make_float4(rec709(t.x), rec709(t.y), rec709(t.z), 1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+DevLayout dev_layout(const NanTracer *t) {
+    DevLayout L;
+    L.faces = t->faces;
+    L.instances = t->instances;
+    L.sides = t->sides;
+    L.texel_face = t->texel_face;
+    L.face_inst = t->face_inst;
+    L.front_prefix = t->front_prefix;
+    L.total_front = t->total_front;
+    return L;
+}
+
+inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
+
+Light make_light(const float *L) {
+    Light l;
+    l.sun_dir = mk(L[0], L[1], L[2]);
+    l.cos_half = L[3];
+    l.outer = mk(L[4], L[5], L[6]);
+    l.transmittance = mk(L[8], L[9], L[10]);
+    l.sun_illum = mk(L[4] * L[8], L[5] * L[9], L[6] * L[10]);
+    l.sky = mk(L[12], L[13], L[14]);
+    l.sun_on = l.sun_illum.x > 0.0f || l.sun_illum.y > 0.0f || l.sun_illum.z > 0.0f;
+    return l;
+}
+
+template <int STACK>
+void launch_megakernel(const NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
+                       uint32_t reset, unsigned long long *counters, hipStream_t stream) {
+    DevLayout L = dev_layout(t);
+    uint32_t grid = grid_for(t->total_front, kTraceBlock);
+    if (counters)
+        hipLaunchKernelGGL((tst_megakernel<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, stream,
+                           t->scene->dev, L, payload, iteration, spp, bounces, reset, counters);
+    else
+        hipLaunchKernelGGL((tst_megakernel<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, stream,
+                           t->scene->dev, L, payload, iteration, spp, bounces, reset, counters);
+}
+
+template <int STACK>
+void launch_trace_rays(const NanScene *s, const float4 *rays, uint64_t n, float4 *hits, uint2 *ids, uint32_t *occ,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL((trace_rays_kernel<STACK, kTraceBlock>), dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), 0,
+                       stream, s->dev, rays, n, hits, ids, occ);
+}
+
+template <int STACK>
+void launch_resolve(const NanTracer *t, const void *cache, int frozen, const DevCamera &cam, float4 *out,
+                    uint32_t w, uint32_t h, hipStream_t stream) {
+    DevLayout L = dev_layout(t);
+    uint64_t n = (uint64_t)w * h;
+    if (frozen)
+        hipLaunchKernelGGL((resolve_kernel<STACK, kTraceBlock, true>), dim3(grid_for(n, kTraceBlock)),
+                           dim3(kTraceBlock), 0, stream, t->scene->dev, L, cache, cam, out, w, h);
+    else
+        hipLaunchKernelGGL((resolve_kernel<STACK, kTraceBlock, false>), dim3(grid_for(n, kTraceBlock)),
+                           dim3(kTraceBlock), 0, stream, t->scene->dev, L, cache, cam, out, w, h);
+}
+
+int check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(nullptr, 0, "%s launch failed: %s", what, hipGetErrorString(e));
+    return 0;
+}
+
+// Kernel stack depths compiled in; a scene whose worst-case traversal needs
+// more than the largest is rejected at creation.
+constexpr uint32_t kStackDepths[] = {32, 64};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+NAN_API int nan_version(void) { return NAN_HIP_ABI_VERSION; }
+
+NAN_API const char *nan_last_error(void) { return g_last_error.c_str(); }
+
+NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options, NanScene **out, char *err,
+                             size_t cap) {
+    if (!d || !out) return fail(err, cap, "nan_scene_create: null argument");
+    *out = nullptr;
+    if (!d->vertices || !d->indices || !d->mesh_descs || !d->instance_descs || !d->transforms ||
+        !d->inv_transposes || !d->materials || !d->lighting)
+        return fail(err, cap, "nan_scene_create: null scene buffer");
+    if (d->instance_count == 0) return fail(err, cap, "nan_scene_create: scene has no instances");
+    auto t0 = std::chrono::steady_clock::now();
+    // validate descriptors and count triangles
+    uint64_t tri_total = 0;
+    for (uint32_t m = 0; m < d->mesh_count; ++m) {
+        const uint32_t *md = d->mesh_descs + 4 * (size_t)m;
+        if ((uint64_t)md[2] + md[0] > d->vertex_count || (uint64_t)md[3] + md[1] > d->index_count || md[1] % 3)
+            return fail(err, cap, "nan_scene_create: mesh %u descriptor out of range", m);
+    }
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const uint32_t *id = d->instance_descs + 4 * (size_t)i;
+        if (id[0] >= d->mesh_count || id[1] >= d->material_count || id[2] >= d->transform_count)
+            return fail(err, cap, "nan_scene_create: instance %u references a missing mesh/material/transform", i);
+        tri_total += d->mesh_descs[4 * (size_t)id[0] + 1] / 3;
+    }
+    for (uint32_t m = 0; m < d->mesh_count; ++m) {
+        const uint32_t *md = d->mesh_descs + 4 * (size_t)m;
+        for (uint32_t k = 0; k < md[1]; ++k)
+            if (d->indices[md[3] + k] >= md[0])
+                return fail(err, cap, "nan_scene_create: mesh %u index out of range", m);
+    }
+    if (tri_total == 0) return fail(err, cap, "nan_scene_create: scene has no triangles");
+    if (tri_total > nanbvh::kLeafFirstMask) return fail(err, cap, "nan_scene_create: too many triangles");
+    // world-space triangles: M (x, y, z, 1) in float64, rounded once
+    std::vector<float> world((size_t)tri_total * 9);
+    std::vector<uint32_t> tri_ids((size_t)tri_total * 2);
+    uint64_t g = 0;
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const uint32_t *id = d->instance_descs + 4 * (size_t)i;
+        const uint32_t *md = d->mesh_descs + 4 * (size_t)id[0];
+        const float *m = d->transforms + 16 * (size_t)id[2];
+        uint32_t ntri = md[1] / 3;
+        for (uint32_t p = 0; p < ntri; ++p, ++g) {
+            tri_ids[2 * g] = i;
+            tri_ids[2 * g + 1] = p;
+            for (int k = 0; k < 3; ++k) {
+                const float *v = d->vertices + 8 * (size_t)(md[2] + d->indices[md[3] + 3 * p + k]);
+                for (int r = 0; r < 3; ++r) {
+                    double w = (double)m[4 * r + 0] * v[0] + (double)m[4 * r + 1] * v[1];
+                    w = w + (double)m[4 * r + 2] * v[2];
+                    w = w + (double)m[4 * r + 3];
+                    world[9 * g + 3 * k + r] = (float)w;
+                }
+            }
+        }
+    }
+    nanbvh::BuildOptions bo;
+    if (options) {
+        if (options->sah_bins) bo.bins = options->sah_bins;
+        if (options->max_leaf_size) bo.max_leaf = options->max_leaf_size;
+        bo.threads = options->build_threads;
+    }
+    nanbvh::Bvh4 bvh;
+    const char *berr = nullptr;
+    if (!nanbvh::build_bvh4(world.data(), (uint32_t)tri_total, bo, &bvh, &berr))
+        return fail(err, cap, "nan_scene_create: %s", berr ? berr : "BVH build failed");
+    uint32_t depth = 0;
+    for (uint32_t s : kStackDepths)
+        if (bvh.max_stack <= s) { depth = s; break; }
+    if (!depth) return fail(err, cap, "nan_scene_create: BVH needs a %u-entry stack (max 64)", bvh.max_stack);
+    auto t1 = std::chrono::steady_clock::now();
+
+    NanScene *s = new NanScene();
+    auto up = [&](auto **dst, const void *src, size_t bytes) -> int {
+        int rc = upload(dst, src, bytes, err, cap);
+        if (rc == 0) s->allocations.push_back((void *)*dst);
+        return rc;
+    };
+    // vertices: 32-byte stride copied as-is (pos.xyz n.x | n.yz uv)
+    uint4 *nodes;
+    float4 *tris, *verts, *xf, *itf, *mats;
+    uint2 *ids;
+    uint32_t *indices;
+    uint4 *mesh_descs, *inst_descs;
+    size_t bytes_total = 0;
+    int rc = 0;
+    rc |= up(&nodes, bvh.nodes.data(), bvh.nodes.size() * 4);
+    rc |= up(&tris, bvh.tris.data(), bvh.tris.size() * 4);
+    rc |= up(&ids, tri_ids.data(), tri_ids.size() * 4);
+    rc |= up(&verts, d->vertices, (size_t)d->vertex_count * 32);
+    rc |= up(&indices, d->indices, (size_t)d->index_count * 4);
+    rc |= up(&mesh_descs, d->mesh_descs, (size_t)d->mesh_count * 16);
+    rc |= up(&inst_descs, d->instance_descs, (size_t)d->instance_count * 16);
+    rc |= up(&xf, d->transforms, (size_t)d->transform_count * 64);
+    rc |= up(&itf, d->inv_transposes, (size_t)d->transform_count * 64);
+    rc |= up(&mats, d->materials, (size_t)d->material_count * 32);
+    if (rc) {
+        delete s;
+        return -1;
+    }
+    bytes_total = bvh.nodes.size() * 4 + bvh.tris.size() * 4 + tri_ids.size() * 4 + (size_t)d->vertex_count * 32 +
+                  (size_t)d->index_count * 4 + (size_t)d->mesh_count * 16 + (size_t)d->instance_count * 16 +
+                  (size_t)d->transform_count * 128 + (size_t)d->material_count * 32;
+    s->dev.nodes = nodes;
+    s->dev.tris = tris;
+    s->dev.tri_ids = ids;
+    s->dev.vertices = verts;
+    s->dev.indices = indices;
+    s->dev.mesh_descs = mesh_descs;
+    s->dev.instance_descs = inst_descs;
+    s->dev.transforms = xf;
+    s->dev.inv_transposes = itf;
+    s->dev.materials = mats;
+    s->dev.root = bvh.root_entry;
+    s->dev.light = make_light(d->lighting);
+    s->stack_depth = depth;
+    s->instance_triangles.resize(d->instance_count);
+    for (uint32_t i = 0; i < d->instance_count; ++i)
+        s->instance_triangles[i] = d->mesh_descs[4 * (size_t)d->instance_descs[4 * (size_t)i] + 1] / 3;
+    s->info.triangle_count = tri_total;
+    s->info.node_count = bvh.node_count;
+    s->info.leaf_count = bvh.leaf_count;
+    s->info.device_bytes = bytes_total;
+    s->info.max_stack = bvh.max_stack;
+    s->info.stack_depth = depth;
+    s->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    s->info.sah_cost = bvh.sah_cost;
+    *out = s;
+    return 0;
+}
+
+NAN_API void nan_scene_destroy(NanScene *scene) { delete scene; }
+
+NAN_API int nan_scene_info(const NanScene *scene, NanSceneInfo *out) {
+    if (!scene || !out) return fail(nullptr, 0, "nan_scene_info: null argument");
+    *out = scene->info;
+    return 0;
+}
+
+NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16) {
+    if (!scene || !lighting16) return fail(nullptr, 0, "nan_scene_set_lighting: null argument");
+    scene->dev.light = make_light(lighting16);
+    return 0;
+}
+
+NAN_API int nan_trace_rays(NanScene *scene, const float *rays, uint64_t n, float *hits, uint32_t *ids,
+                           uint32_t *occluded, void *stream) {
+    if (!scene || (n && (!rays || !hits || !ids))) return fail(nullptr, 0, "nan_trace_rays: null argument");
+    if (n == 0) return 0;
+    const float4 *r = reinterpret_cast<const float4 *>(rays);
+    if (scene->stack_depth <= 32)
+        launch_trace_rays<32>(scene, r, n, (float4 *)hits, (uint2 *)ids, occluded, as_stream(stream));
+    else
+        launch_trace_rays<64>(scene, r, n, (float4 *)hits, (uint2 *)ids, occluded, as_stream(stream));
+    return check_launch("trace_rays");
+}
+
+NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint32_t face_count,
+                              const uint32_t *instance_table, const uint32_t *side_table, uint32_t instance_count,
+                              uint64_t total_surface_texels, uint64_t total_payload_count, NanTracer **out,
+                              char *err, size_t cap) {
+    if (!scene || !face_table || !instance_table || !side_table || !out)
+        return fail(err, cap, "nan_tracer_create: null argument");
+    *out = nullptr;
+    if (instance_count != scene->instance_triangles.size())
+        return fail(err, cap, "nan_tracer_create: layout has %u instances, scene has %zu", instance_count,
+                    scene->instance_triangles.size());
+    if (total_payload_count >= (1ull << 32)) return fail(err, cap, "nan_tracer_create: payload exceeds 2^32 slots");
+    // validate the tables against each other: faces tile each instance's range
+    std::vector<uint32_t> texel_face;
+    texel_face.reserve(total_surface_texels);
+    std::vector<uint32_t> face_inst(face_count);
+    std::vector<uint32_t> prefix(instance_count);
+    uint64_t front = 0, payload_end = 0;
+    for (uint32_t i = 0; i < instance_count; ++i) {
+        const uint32_t *ii = instance_table + 4 * (size_t)i;
+        const uint32_t *sd = side_table + 2 * (size_t)i;
+        if ((uint64_t)ii[0] + ii[1] > face_count)
+            return fail(err, cap, "nan_tracer_create: instance %u face range out of bounds", i);
+        if (ii[1] != scene->instance_triangles[i])
+            return fail(err, cap, "nan_tracer_create: instance %u face_count differs from its mesh", i);
+        prefix[i] = (uint32_t)front;
+        uint64_t local = 0;
+        for (uint32_t f = ii[0]; f < ii[0] + ii[1]; ++f) {
+            const uint32_t *fi = face_table + 2 * (size_t)f;
+            if (fi[1] == 0 || fi[1] > 4096) return fail(err, cap, "nan_tracer_create: face %u bad resolution", f);
+            if (fi[0] != local) return fail(err, cap, "nan_tracer_create: face %u address is not contiguous", f);
+            uint64_t pc = (uint64_t)(fi[1] + 1) * (fi[1] + 2) / 2;
+            face_inst[f] = i;
+            for (uint64_t k = 0; k < pc; ++k) texel_face.push_back(f);
+            local += pc;
+        }
+        if (local != ii[3]) return fail(err, cap, "nan_tracer_create: instance %u texel_count mismatch", i);
+        if ((uint64_t)ii[2] + ii[3] > total_payload_count)
+            return fail(err, cap, "nan_tracer_create: instance %u slots out of bounds", i);
+        if (sd[1] & 1u) {
+            if ((uint64_t)sd[0] + ii[3] > total_payload_count)
+                return fail(err, cap, "nan_tracer_create: instance %u back slots out of bounds", i);
+            payload_end = std::max<uint64_t>(payload_end, (uint64_t)sd[0] + ii[3]);
+        }
+        payload_end = std::max<uint64_t>(payload_end, (uint64_t)ii[2] + ii[3]);
+        front += ii[3];
+    }
+    if (front != total_surface_texels) return fail(err, cap, "nan_tracer_create: total_surface_texels mismatch");
+    if (front >= (1ull << 32)) return fail(err, cap, "nan_tracer_create: too many texels");
+    // every instance must exist in the scene
+    NanTracer *t = new NanTracer();
+    t->scene = scene;
+    t->face_count = face_count;
+    t->instance_count = instance_count;
+    t->total_front = front;
+    t->total_payload = total_payload_count;
+    int rc = 0;
+    rc |= upload(&t->faces, face_table, (size_t)face_count * 8, err, cap);
+    rc |= upload(&t->instances, instance_table, (size_t)instance_count * 16, err, cap);
+    rc |= upload(&t->sides, side_table, (size_t)instance_count * 8, err, cap);
+    rc |= upload(&t->texel_face, texel_face.data(), texel_face.size() * 4, err, cap);
+    rc |= upload(&t->face_inst, face_inst.data(), face_inst.size() * 4, err, cap);
+    rc |= upload(&t->front_prefix, prefix.data(), prefix.size() * 4, err, cap);
+    if (rc) {
+        delete t;
+        return -1;
+    }
+    *out = t;
+    return 0;
+}
+
+NAN_API void nan_tracer_destroy(NanTracer *tracer) { delete tracer; }
+
+NAN_API int nan_tracer_tables(const NanTracer *tracer, const void **face_table, const void **instance_table,
+                              const void **side_table) {
+    if (!tracer) return fail(nullptr, 0, "nan_tracer_tables: null tracer");
+    if (face_table) *face_table = tracer->faces;
+    if (instance_table) *instance_table = tracer->instances;
+    if (side_table) *side_table = tracer->sides;
+    return 0;
+}
+
+static int tst_execute(NanTracer *t, void *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
+                       uint32_t reset, uint64_t *counters, void *stream) {
+    if (!t || !payload) return fail(nullptr, 0, "nan_tst_execute: null argument");
+    if (t->total_front == 0) return 0;
+    unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
+    if (t->scene->stack_depth <= 32)
+        launch_megakernel<32>(t, (float4 *)payload, iteration, spp, bounces, reset, c, as_stream(stream));
+    else
+        launch_megakernel<64>(t, (float4 *)payload, iteration, spp, bounces, reset, c, as_stream(stream));
+    return check_launch("tst_megakernel");
+}
+
+NAN_API int nan_tst_execute(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,
+                            uint32_t max_bounces, uint32_t reset, void *stream) {
+    return tst_execute(tracer, payload, iteration, samples_per_texel, max_bounces, reset, nullptr, stream);
+}
+
+NAN_API int nan_tst_execute_counted(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,
+                                    uint32_t max_bounces, uint32_t reset, uint64_t *counters, void *stream) {
+    if (!counters) return fail(nullptr, 0, "nan_tst_execute_counted: null counters");
+    return tst_execute(tracer, payload, iteration, samples_per_texel, max_bounces, reset, counters, stream);
+}
+
+NAN_API int nan_tst_raygen(NanTracer *tracer, uint32_t iteration, uint32_t spp, uint32_t *out, uint32_t *rng_state,
+                           void *stream) {
+    if (!tracer || !out) return fail(nullptr, 0, "nan_tst_raygen: null argument");
+    if (tracer->total_front == 0) return 0;
+    hipLaunchKernelGGL(tst_raygen_kernel, dim3(grid_for(tracer->total_front, kFlatBlock)), dim3(kFlatBlock), 0,
+                       as_stream(stream), dev_layout(tracer), iteration, spp, (uint4 *)out, rng_state);
+    return check_launch("tst_raygen");
+}
+
+NAN_API int nan_rgb9e5_pack(const void *payload, uint32_t *packed, uint64_t n, void *stream) {
+    if (n == 0) return 0;
+    if (!payload || !packed) return fail(nullptr, 0, "nan_rgb9e5_pack: null argument");
+    hipLaunchKernelGGL(rgb9e5_pack_kernel, dim3(grid_for(n, kFlatBlock)), dim3(kFlatBlock), 0, as_stream(stream),
+                       (const float4 *)payload, packed, n);
+    return check_launch("rgb9e5_pack");
+}
+
+NAN_API int nan_rgb9e5_unpack(const uint32_t *packed, float *out, uint64_t n, void *stream) {
+    if (n == 0) return 0;
+    if (!packed || !out) return fail(nullptr, 0, "nan_rgb9e5_unpack: null argument");
+    hipLaunchKernelGGL(rgb9e5_unpack_kernel, dim3(grid_for(n, kFlatBlock)), dim3(kFlatBlock), 0, as_stream(stream),
+                       packed, (float4 *)out, n);
+    return check_launch("rgb9e5_unpack");
+}
+
+NAN_API int nan_resolve(NanTracer *tracer, const void *cache, int frozen, const float *c, float *out,
+                        uint32_t width, uint32_t height, void *stream) {
+    if (!tracer || !cache || !c || !out) return fail(nullptr, 0, "nan_resolve: null argument");
+    if (width == 0 || height == 0) return 0;
+    DevCamera cam;
+    cam.pos = mk(c[0], c[1], c[2]);
+    cam.jx = c[3];
+    cam.u = mk(c[4], c[5], c[6]);
+    cam.jy = c[7];
+    cam.v = mk(c[8], c[9], c[10]);
+    cam.near = c[11];
+    cam.w = mk(c[12], c[13], c[14]);
+    cam.far = c[15];
+    if (tracer->scene->stack_depth <= 32)
+        launch_resolve<32>(tracer, cache, frozen, cam, (float4 *)out, width, height, as_stream(stream));
+    else
+        launch_resolve<64>(tracer, cache, frozen, cam, (float4 *)out, width, height, as_stream(stream));
+    return check_launch("resolve");
+}
+
+NAN_API int nan_accumulate(const float *input, float *accum, float *output, uint64_t n, int reset, void *stream) {
+    if (n == 0) return 0;
+    if (!input || !accum || !output) return fail(nullptr, 0, "nan_accumulate: null argument");
+    hipLaunchKernelGGL(accumulate_kernel, dim3(grid_for(n, kFlatBlock)), dim3(kFlatBlock), 0, as_stream(stream),
+                       (const float4 *)input, (float4 *)accum, (float4 *)output, n, reset);
+    return check_launch("accumulate");
+}
+
+NAN_API int nan_tonemap(const float *input, float *output, uint64_t n, float exposure, void *stream) {
+    if (n == 0) return 0;
+    if (!input || !output) return fail(nullptr, 0, "nan_tonemap: null argument");
+    hipLaunchKernelGGL(tonemap_kernel, dim3(grid_for(n, kFlatBlock)), dim3(kFlatBlock), 0, as_stream(stream),
+                       (const float4 *)input, (float4 *)output, n, exposure);
+    return check_launch("tonemap");
+}
+
+}  // extern "C"

@@ -1,0 +1,510 @@
+// Device-side building blocks of the texture-space tracer for gfx950.
+//
+// Arithmetic contract (shared with the CPU oracle): FP32 with no implicit
+// contraction (this translation unit is compiled with -ffp-contract=off),
+// __builtin_fmaf exactly where the reference writes mad(), correctly
+// rounded division and sqrt (hipcc's default for FP32), and sin/cos of
+// 2*pi*u from sincos_2pi().  Under this contract every payload slot equals
+// the oracle's bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nan {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kMaxTexelSamples = 65535u;
+constexpr float kPi = 0x1.921fb6p+1f;
+constexpr float kTwoPi = 0x1.921fb6p+2f;
+constexpr float kInvPi = 0x1.45f306p-2f;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafCountShift = 27;
+constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
+// Ray-box slack: the far distance is inflated by 1 + 2^-20 (covers the
+// 2*gamma_3 bound of Ize, "Robust BVH Ray Traversal", JCGT 2013) and the
+// current hit distance by 1 + 2^-12 so near-ties are never culled.
+constexpr float kBoxPad = 0x1.000008p+0f;
+constexpr float kTPad = 0x1.001p+0f;
+
+struct f3 {
+    float x, y, z;
+};
+__host__ __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ __forceinline__ f3 operator*(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__host__ __device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ __forceinline__ f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ __forceinline__ f3 fabs3(f3 a) { return mk(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+__host__ __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 normalize(f3 a) {
+    float inv = 1.0f / sqrtf(dot(a, a));
+    return a * inv;
+}
+__device__ __forceinline__ float comp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+__device__ __forceinline__ float saturate(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+__device__ __forceinline__ float fracf(float x) { return x - floorf(x); }
+__device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
+
+// sin(2*pi*u), cos(2*pi*u) for u in [0, 1]: exact quarter-turn reduction on
+// u, then degree-9/10 Taylor polynomials on [-pi/4, pi/4].
+__device__ __forceinline__ void sincos_2pi(float u, float *s_out, float *c_out) {
+    float k = rintf(u * 4.0f);
+    float v = u - k * 0.25f;
+    float a = v * kTwoPi;
+    float a2 = a * a;
+    float p = __builtin_fmaf(a2, 0x1.71de3ap-19f, -0x1.a01a02p-13f);
+    p = __builtin_fmaf(a2, p, 0x1.111112p-7f);
+    p = __builtin_fmaf(a2, p, -0x1.555556p-3f);
+    float s = __builtin_fmaf(a * a2, p, a);
+    float q = __builtin_fmaf(a2, -0x1.27e4fcp-22f, 0x1.a01a02p-16f);
+    q = __builtin_fmaf(a2, q, -0x1.6c16c2p-10f);
+    q = __builtin_fmaf(a2, q, 0x1.555556p-5f);
+    q = __builtin_fmaf(a2, q, -0.5f);
+    float c = __builtin_fmaf(a2, q, 1.0f);
+    int quadrant = ((int)k) & 3;
+    float rs = quadrant == 0 ? s : (quadrant == 1 ? c : (quadrant == 2 ? -s : -c));
+    float rc = quadrant == 0 ? c : (quadrant == 1 ? -s : (quadrant == 2 ? -c : s));
+    *s_out = rs;
+    *c_out = rc;
+}
+
+// ---------------------------------------------------------------------------
+// RNG: 32-bit LCG (common.slang:164-210)
+// ---------------------------------------------------------------------------
+struct Rng {
+    uint32_t state;
+    __device__ __forceinline__ void seed(uint32_t x, uint32_t y, uint32_t s) {
+        state = (x * 100591u) ^ (y * 102191u) ^ (s * 103643u);
+    }
+    __device__ __forceinline__ float next_1d() {
+        state = 1664525u * state + 1013904223u;
+        return (float)(state >> 8) * 0x1p-24f;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// scene as seen by the kernels
+// ---------------------------------------------------------------------------
+struct Light {
+    f3 sun_dir;
+    float cos_half;
+    f3 sun_illum;     // color * intensity * transmittance
+    f3 outer;         // color * intensity
+    f3 transmittance;
+    f3 sky;
+    int sun_on;
+};
+
+struct DevScene {
+    const uint4 *nodes;           // 64-byte BVH4 nodes
+    const float4 *tris;           // 48-byte triangle records, 3 float4 each
+    const uint2 *tri_ids;         // global triangle id -> (instance, primitive)
+    const float4 *vertices;       // 2 float4 per vertex: pos.xyz n.x | n.yz uv
+    const uint32_t *indices;
+    const uint4 *mesh_descs;
+    const uint4 *instance_descs;
+    const float4 *transforms;     // 4 float4 rows
+    const float4 *inv_transposes; // 4 float4 rows
+    const float4 *materials;      // 2 float4: base.rgb flags | emissive.rgb alpha
+    uint32_t root;
+    Light light;
+};
+
+struct Ray {
+    f3 o, d;
+    float tmin, tmax;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    f3 inv;
+};
+
+__device__ __forceinline__ void ray_setup(Ray &r) {
+    float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
+    int kz = 0;
+    float am = ax;
+    if (ay > am) { kz = 1; am = ay; }
+    if (az > am) { kz = 2; }
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = comp(r.d, kz);
+    if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = comp(r.d, kx) / dz;
+    r.Sy = comp(r.d, ky) / dz;
+    r.Sz = 1.0f / dz;
+    float dx = fabsf(r.d.x) < 1e-30f ? copysignf(1e-30f, r.d.x) : r.d.x;
+    float dy = fabsf(r.d.y) < 1e-30f ? copysignf(1e-30f, r.d.y) : r.d.y;
+    float dzz = fabsf(r.d.z) < 1e-30f ? copysignf(1e-30f, r.d.z) : r.d.z;
+    r.inv = mk(1.0f / dx, 1.0f / dy, 1.0f / dzz);
+}
+
+struct Hit {
+    float t, b1, b2;
+    uint32_t id;  // global triangle id, kInvalid on a miss
+};
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013).
+__device__ __forceinline__ bool tri_intersect(const Ray &r, f3 v0, f3 v1, f3 v2, float &t_out, float &b1_out,
+                                              float &b2_out) {
+    f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    float Ax = comp(A, r.kx) - r.Sx * Akz;
+    float Ay = comp(A, r.ky) - r.Sy * Akz;
+    float Bx = comp(B, r.kx) - r.Sx * Bkz;
+    float By = comp(B, r.ky) - r.Sy * Bkz;
+    float Cx = comp(C, r.kx) - r.Sx * Ckz;
+    float Cy = comp(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= r.tmin && t <= r.tmax)) return false;
+    t_out = t;
+    b1_out = V * rcp;
+    b2_out = W * rcp;
+    return true;
+}
+
+struct TraversalStats {
+    uint32_t nodes, tris;
+};
+
+// Stack of child entries in LDS: STACK rows of BLOCK lanes, lane-major so
+// the 64 lanes of a wave hit 64 different banks.
+template <int STACK, int BLOCK, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &hit, uint32_t *lds_col,
+                                         TraversalStats &st) {
+    uint32_t cur = S.root;
+    int sp = 0;
+    while (true) {
+        if (!(cur & kLeafBit)) {
+            const uint4 *np = S.nodes + 4 * (size_t)cur;
+            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+            if (COUNT) st.nodes++;
+            f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
+            float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
+            float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
+            float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
+            uint32_t valid = w0.w >> 24;
+            uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
+            uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
+            float limit = (ANY_HIT ? r.tmax : hit.t) * kTPad;
+            float tn[4];
+            uint32_t ent[4];
+            int nhit = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float lox = origin.x + (float)((qlx >> (8 * c)) & 0xFFu) * sx;
+                float loy = origin.y + (float)((qly >> (8 * c)) & 0xFFu) * sy;
+                float loz = origin.z + (float)((qlz >> (8 * c)) & 0xFFu) * sz;
+                float hix = origin.x + (float)((qhx >> (8 * c)) & 0xFFu) * sx;
+                float hiy = origin.y + (float)((qhy >> (8 * c)) & 0xFFu) * sy;
+                float hiz = origin.z + (float)((qhz >> (8 * c)) & 0xFFu) * sz;
+                float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
+                float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
+                float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
+                float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+                float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxPad;
+                far = fminf(far, limit);
+                bool ok = ((valid >> c) & 1u) && near <= far;
+                // insertion into a sorted list (ascending near), unrolled
+                if (ok) {
+                    float key = near;
+                    uint32_t e = child[c];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (k < nhit && key < tn[k]) {
+                            float tk = tn[k];
+                            uint32_t ek = ent[k];
+                            tn[k] = key;
+                            ent[k] = e;
+                            key = tk;
+                            e = ek;
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (k == nhit) { tn[k] = key; ent[k] = e; }
+                    nhit++;
+                }
+            }
+            if (nhit == 0) {
+                if (sp == 0) break;
+                cur = lds_col[(--sp) * BLOCK];
+                continue;
+            }
+            // push the far children (farthest first), continue with the nearest
+#pragma unroll
+            for (int k = 3; k >= 1; --k)
+                if (k < nhit) lds_col[(sp++) * BLOCK] = ent[k];
+            cur = ent[0];
+            continue;
+        }
+        // leaf
+        uint32_t first = cur & kLeafFirstMask;
+        uint32_t count = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+        for (uint32_t i = 0; i < count; ++i) {
+            const float4 *tp = S.tris + 3 * (size_t)(first + i);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            if (COUNT) st.tris++;
+            float t, b1, b2;
+            if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
+                uint32_t id = __float_as_uint(c.y);
+                if (ANY_HIT) {
+                    hit.t = t;
+                    hit.id = id;
+                    return true;
+                }
+                if (t < hit.t || (t == hit.t && id < hit.id)) {
+                    hit.t = t;
+                    hit.b1 = b1;
+                    hit.b2 = b2;
+                    hit.id = id;
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = lds_col[(--sp) * BLOCK];
+    }
+    return hit.id != kInvalid;
+}
+
+// ---------------------------------------------------------------------------
+// shading (scene.slang:178-406, common.slang:54-162, 452-490)
+// ---------------------------------------------------------------------------
+struct Frame {
+    f3 n, t, b;
+};
+__device__ __forceinline__ Frame make_frame(f3 n) {
+    Frame f;
+    f.n = n;
+    f.t = normalize(cross(n, mk(-n.z, n.x, -n.y)));
+    f.b = cross(n, f.t);
+    return f;
+}
+__device__ __forceinline__ f3 to_global(const Frame &f, f3 w) { return f.t * w.x + f.b * w.y + f.n * w.z; }
+
+struct Shading {
+    f3 origin_front;
+    Frame frame;
+    f3 diffuse, emissive;
+};
+
+__device__ __forceinline__ f3 row3(float4 r) { return mk(r.x, r.y, r.z); }
+__device__ __forceinline__ f3 mul33(const float4 *m, f3 v) {
+    return mk(dot(row3(m[0]), v), dot(row3(m[1]), v), dot(row3(m[2]), v));
+}
+
+__device__ __forceinline__ void load_triangle(const DevScene &S, uint32_t instance, uint32_t prim, uint4 &inst,
+                                              f3 &v0, f3 &v1, f3 &v2, float4 (&vb)[3]) {
+    inst = S.instance_descs[instance];
+    uint4 mesh = S.mesh_descs[inst.x];
+    const uint32_t *idx = S.indices + mesh.w + 3 * (size_t)prim;
+    uint32_t i0 = mesh.z + idx[0], i1 = mesh.z + idx[1], i2 = mesh.z + idx[2];
+    float4 a0 = S.vertices[2 * (size_t)i0], a1 = S.vertices[2 * (size_t)i1], a2 = S.vertices[2 * (size_t)i2];
+    vb[0] = S.vertices[2 * (size_t)i0 + 1];
+    vb[1] = S.vertices[2 * (size_t)i1 + 1];
+    vb[2] = S.vertices[2 * (size_t)i2 + 1];
+    v0 = mk(a0.x, a0.y, a0.z);
+    v1 = mk(a1.x, a1.y, a1.z);
+    v2 = mk(a2.x, a2.y, a2.z);
+    // normals: (a.w, vb.x, vb.y)
+    vb[0] = make_float4(a0.w, vb[0].x, vb[0].y, 0.0f);
+    vb[1] = make_float4(a1.w, vb[1].x, vb[1].y, 0.0f);
+    vb[2] = make_float4(a2.w, vb[2].x, vb[2].y, 0.0f);
+}
+
+__device__ __forceinline__ void shading_data(const DevScene &S, uint32_t instance, uint32_t prim, float b1, float b2,
+                                             f3 view, Shading &sd) {
+    uint4 inst;
+    f3 v0, v1, v2;
+    float4 nrm[3];
+    load_triangle(S, instance, prim, inst, v0, v1, v2, nrm);
+    const float4 *M = S.transforms + 4 * (size_t)inst.z;
+    const float4 *IT = S.inv_transposes + 4 * (size_t)inst.z;
+    float4 m0 = M[0], m1 = M[1], m2 = M[2];
+    float4 it0 = IT[0], it1 = IT[1], it2 = IT[2], it3 = IT[3];
+    f3 bc = mk(1.0f - b1 - b2, b1, b2);
+
+    const float c0 = 5.9604644775390625e-8f, c1 = 1.7881397695873602e-7f, c2 = 1.1920931797249068e-7f;
+    f3 e1 = v1 - v0, e2 = v2 - v0;
+    f3 op = mk(v0.x + __builtin_fmaf(b1, e1.x, b2 * e2.x), v0.y + __builtin_fmaf(b1, e1.y, b2 * e2.y),
+               v0.z + __builtin_fmaf(b1, e1.z, b2 * e2.z));
+    f3 on = cross(e1, e2);
+    f3 wp = mk(m0.w + __builtin_fmaf(m0.x, op.x, __builtin_fmaf(m0.y, op.y, m0.z * op.z)),
+               m1.w + __builtin_fmaf(m1.x, op.x, __builtin_fmaf(m1.y, op.y, m1.z * op.z)),
+               m2.w + __builtin_fmaf(m2.x, op.x, __builtin_fmaf(m2.y, op.y, m2.z * op.z)));
+    f3 wnu = mk(dot(row3(it0), on), dot(row3(it1), on), dot(row3(it2), on));
+    float world_scale = 1.0f / sqrtf(fmaxf(dot(wnu, wnu), 1e-30f));
+    f3 world_normal = wnu * world_scale;
+    f3 ae1 = fabs3(e1), ae2 = fabs3(e2);
+    f3 ext3 = (ae1 + ae2) + fabs3(ae1 - ae2);
+    float extent = fmaxf(fmaxf(ext3.x, ext3.y), ext3.z);
+    f3 object_error = fabs3(v0) * c0 + mk(c1 * extent, c1 * extent, c1 * extent);
+    f3 aop = fabs3(op);
+    f3 world_error = mk(c1 * dot(fabs3(row3(m0)), aop) + c2 * fabsf(m0.w),
+                        c1 * dot(fabs3(row3(m1)), aop) + c2 * fabsf(m1.w),
+                        c1 * dot(fabs3(row3(m2)), aop) + c2 * fabsf(m2.w));
+    f3 awp = fabs3(wp);
+    f3 te = mk(fabsf(it0.x) * awp.x + fabsf(it1.x) * awp.y + fabsf(it2.x) * awp.z + fabsf(it3.x) * 1.0f,
+               fabsf(it0.y) * awp.x + fabsf(it1.y) * awp.y + fabsf(it2.y) * awp.z + fabsf(it3.y) * 1.0f,
+               fabsf(it0.z) * awp.x + fabsf(it1.z) * awp.y + fabsf(it2.z) * awp.z + fabsf(it3.z) * 1.0f);
+    object_error = object_error + te * c2;
+    float object_offset = dot(object_error, fabs3(on));
+    float world_offset = dot(world_error, fabs3(world_normal));
+    world_offset = __builtin_fmaf(world_scale, object_offset, world_offset);
+
+    f3 n0 = mk(nrm[0].x, nrm[0].y, nrm[0].z), n1 = mk(nrm[1].x, nrm[1].y, nrm[1].z),
+       n2 = mk(nrm[2].x, nrm[2].y, nrm[2].z);
+    f3 ni = n0 * bc.x + n1 * bc.y + n2 * bc.z;
+    f3 geometry_normal = normalize(mk(dot(row3(it0), ni), dot(row3(it1), ni), dot(row3(it2), ni)));
+    f3 face_normal = world_normal;
+    float4 ma = S.materials[2 * (size_t)inst.y], mb = S.materials[2 * (size_t)inst.y + 1];
+    bool double_sided = (__float_as_uint(ma.w) & 1u) != 0;
+    bool front = dot(face_normal, view) > 0.0f;
+    float sign = (double_sided && !front) ? -1.0f : 1.0f;
+    face_normal = face_normal * sign;
+    geometry_normal = geometry_normal * sign;
+    f3 shading_normal = geometry_normal;
+    f3 D = -view;
+    float dn = 2.0f * dot(shading_normal, D);
+    f3 R = D - shading_normal * dn;
+    float k = dot(R, face_normal);
+    if (k < 0.0f) shading_normal = normalize(normalize(R - face_normal * k) - D);
+    sd.origin_front = wp + face_normal * world_offset;
+    sd.frame = make_frame(shading_normal);
+    sd.diffuse = mk(ma.x, ma.y, ma.z);
+    sd.emissive = mk(mb.x, mb.y, mb.z);
+}
+
+// texture_space_path_tracer.slang:39-55
+__device__ __forceinline__ f3 world_flat_normal(const DevScene &S, uint32_t instance, uint32_t prim) {
+    uint4 inst;
+    f3 p0, p1, p2;
+    float4 nrm[3];
+    load_triangle(S, instance, prim, inst, p0, p1, p2, nrm);
+    f3 n = normalize(cross(p1 - p0, p2 - p0));
+    if (inst.w & 1u) n = -n;
+    const float4 *IT = S.inv_transposes + 4 * (size_t)inst.z;
+    float4 it0 = IT[0], it1 = IT[1], it2 = IT[2];
+    return normalize(mk(dot(row3(it0), n), dot(row3(it1), n), dot(row3(it2), n)));
+}
+
+__device__ __forceinline__ void cosine_sample(float ex, float ey, f3 &dir, float &pdf) {
+    float s, c;
+    sincos_2pi(ex, &s, &c);
+    float ct = sqrtf(ey);
+    float st = sqrtf(1.0f - ct * ct);
+    dir = mk(st * c, st * s, ct);
+    pdf = ct * kInvPi;
+}
+
+__device__ __forceinline__ f3 cone_sample(f3 axis, float cos_half, float ux, float uy) {
+    float ct = 1.0f - ux * (1.0f - cos_half);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float s, c;
+    sincos_2pi(uy, &s, &c);
+    f3 local = mk(c * st, s * st, ct);
+    Frame f = make_frame(axis);
+    return to_global(f, local);
+}
+
+// ---------------------------------------------------------------------------
+// Mesh Colors lattice (mesh_colors.slang:37-65, 255-363)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t patch_count(uint32_t r) { return (r + 1) * (r + 2) / 2; }
+
+__device__ __forceinline__ uint32_t lattice_address(uint32_t i, uint32_t j, uint32_t texel_offset, uint2 face) {
+    uint32_t row = i * (face.y + 1) - i * (i - 1) / 2;
+    return texel_offset + face.x + row + j;
+}
+
+__device__ __forceinline__ void address_to_ij(uint32_t address, uint32_t resolution, uint32_t &i, uint32_t &j) {
+    uint32_t left = 0, right = resolution;
+    while (left < right) {
+        uint32_t middle = (left + right + 1) / 2;
+        uint32_t ma = middle * (resolution + 1) - middle * (middle - 1) / 2;
+        if (ma <= address) left = middle; else right = middle - 1;
+    }
+    i = left;
+    j = address - (left * (resolution + 1) - left * (left - 1) / 2);
+}
+
+__device__ __forceinline__ uint32_t nearest_texel(f3 b, uint32_t texel_offset, uint2 face) {
+    uint32_t R = face.y;
+    float fr = (float)R;
+    f3 sc = mk(saturate(b.x) * fr, saturate(b.y) * fr, saturate(b.z) * fr);
+    uint32_t i = (uint32_t)sc.x, j = (uint32_t)sc.y;
+    i = i > R ? R : i;
+    j = j > R - i ? R - i : j;
+    f3 w = mk(fracf(sc.x), fracf(sc.y), fracf(sc.z));
+    bool upper = fabsf(w.x + w.y + w.z - 2.0f) <= 1e-3f;
+    if (upper) w = mk(1.0f - w.x, 1.0f - w.y, 1.0f - w.z);
+    if (w.x + w.y + w.z <= 1e-6f || i >= R || j >= R - i) return lattice_address(i, j, texel_offset, face);
+    float largest = fmaxf(w.x, fmaxf(w.y, w.z));
+    if (upper) {
+        if (fabsf(largest - w.x) < 1e-3f) return lattice_address(i, j + 1, texel_offset, face);
+        if (fabsf(largest - w.y) < 1e-3f) return lattice_address(i + 1, j, texel_offset, face);
+        return lattice_address(i + 1, j + 1, texel_offset, face);
+    }
+    if (fabsf(largest - w.x) < 1e-3f) return lattice_address(i + 1, j, texel_offset, face);
+    if (fabsf(largest - w.y) < 1e-3f) return lattice_address(i, j + 1, texel_offset, face);
+    return lattice_address(i, j, texel_offset, face);
+}
+
+// ---------------------------------------------------------------------------
+// RGB9E5 (mesh_colors.slang:218-253), exact to the float64 restatement of
+// test_mesh_colors_rgb9e5.py:15-50 for every float32 input.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t round_half_up(float x) {
+    float f = floorf(x);
+    return (uint32_t)f + ((x - f) >= 0.5f ? 1u : 0u);
+}
+
+__device__ __forceinline__ uint32_t pack_rgb9e5(float r, float g, float b) {
+    r = r != r ? 0.0f : fminf(fmaxf(r, 0.0f), 65408.0f);
+    g = g != g ? 0.0f : fminf(fmaxf(g, 0.0f), 65408.0f);
+    b = b != b ? 0.0f : fminf(fmaxf(b, 0.0f), 65408.0f);
+    float m = fmaxf(r, fmaxf(g, b));
+    if (m < 0x1p-25f) return 0u;
+    int e = (int)((__float_as_uint(m) >> 23) & 0xFFu) - 127 + 1;
+    e = e < -15 ? -15 : e;
+    uint32_t shared = (uint32_t)(e + 15);
+    float scale = exp2i((int)shared - 24);
+    uint32_t mr = round_half_up(r / scale), mg = round_half_up(g / scale), mbv = round_half_up(b / scale);
+    uint32_t mm = max(mr, max(mg, mbv));
+    if (mm == 512u && shared < 31u) {
+        shared += 1u;
+        scale *= 2.0f;
+        mr = round_half_up(r / scale);
+        mg = round_half_up(g / scale);
+        mbv = round_half_up(b / scale);
+    }
+    mr = min(mr, 511u);
+    mg = min(mg, 511u);
+    mbv = min(mbv, 511u);
+    return mr | (mg << 9) | (mbv << 18) | (shared << 27);
+}
+
+__device__ __forceinline__ f3 unpack_rgb9e5(uint32_t p) {
+    float scale = exp2i((int)(p >> 27) - 24);
+    return mk((float)(p & 0x1ffu) * scale, (float)((p >> 9) & 0x1ffu) * scale, (float)((p >> 18) & 0x1ffu) * scale);
+}
+
+}  // namespace nan

@@ -1,0 +1,313 @@
+"""Mesh Colors layout and lattice math (mesh_colors.py:19-297 of the reference).
+
+Every triangle carries a power-of-two barycentric lattice of
+``(R+1)(R+2)/2`` irradiance texels.  ``MeshColorsLayout.build`` assigns
+resolutions and a flat slot range per instance with the reference's greedy
+budget rule; the tables it produces are bit-identical to the reference's
+``uint2`` face, ``uint4`` instance and ``uint2`` side buffers.
+
+The build is vectorised with numpy.  It reproduces the reference's
+arithmetic: world positions are float32 ``M * (x, y, z, 1)`` products
+(``spy.math.mul``), edge lengths are float64 norms of those float32
+positions, and the budget rule is applied triangle by triangle in instance
+order once the running total can exceed the budget.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .scene_node import SceneNode
+
+MESH_COLORS_PAYLOAD_SIZE = 16
+MESH_COLORS_INVALID_OFFSET = 0xFFFFFFFF
+MESH_COLORS_SIDE_FLAG_DOUBLE_SIDED = 1 << 0
+
+BUDGET_ERROR = (
+    "Mesh Colors texel budget exceeded; lower --texture-space-texels-per-unit "
+    "or --texture-space-min-resolution, or increase --texture-space-max-texels"
+)
+
+
+def colors_per_patch(resolution: int) -> int:
+    """Lattice points of one triangle (mesh_colors.py:24-29)."""
+    resolution = int(resolution)
+    if resolution < 1:
+        raise ValueError("resolution must be positive")
+    return (resolution + 1) * (resolution + 2) // 2
+
+
+def _row_offset(i: int, resolution: int) -> int:
+    return i * (resolution + 1) - i * (i - 1) // 2
+
+
+def address_to_ij(address: int, resolution: int) -> tuple[int, int]:
+    """Inverse of :func:`ij_to_address` (mesh_colors.py:32-52).  Solved here
+    by bisection on the row offsets, the form the shader uses
+    (mesh_colors.slang:48-65); both forms agree on every address."""
+    address = int(address)
+    resolution = int(resolution)
+    if address < 0 or address >= colors_per_patch(resolution):
+        raise ValueError("address is outside the triangle patch")
+    lo, hi = 0, resolution
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        if _row_offset(mid, resolution) <= address:
+            lo = mid
+        else:
+            hi = mid - 1
+    return lo, address - _row_offset(lo, resolution)
+
+
+def ij_to_address(i: int, j: int, resolution: int) -> int:
+    """Row-major lattice address (mesh_colors.py:55-61)."""
+    i, j, resolution = int(i), int(j), int(resolution)
+    if i < 0 or i > resolution or j < 0 or j > resolution - i:
+        raise ValueError("(i, j) is outside the triangle patch")
+    return _row_offset(i, resolution) + j
+
+
+def barycentric_at(i: int, j: int, resolution: int) -> tuple[float, float, float]:
+    """``(i/R, j/R, 1-i/R-j/R)`` (mesh_colors.py:64-69)."""
+    if resolution <= 0:
+        raise ValueError("resolution must be positive")
+    u = float(i) / float(resolution)
+    v = float(j) / float(resolution)
+    return u, v, 1.0 - u - v
+
+
+def next_power_of_two(value: int) -> int:
+    value = max(1, int(value))
+    return 1 << (value - 1).bit_length()
+
+
+def previous_power_of_two(value: int) -> int:
+    value = max(1, int(value))
+    return 1 << (value.bit_length() - 1)
+
+
+def _next_pow2_array(values: np.ndarray) -> np.ndarray:
+    v = np.maximum(values.astype(np.int64), 1)
+    out = np.ones_like(v)
+    # Values here are bounded by max_resolution after clamping; a short loop
+    # over bit positions is exact and fast.
+    while True:
+        mask = out < v
+        if not mask.any():
+            return out
+        out[mask] <<= 1
+
+
+@dataclass(frozen=True)
+class MeshColorsFaceInfo:
+    address: int
+    resolution: int
+
+
+@dataclass(frozen=True)
+class MeshColorsInstanceInfo:
+    face_offset: int
+    face_count: int
+    texel_offset: int
+    texel_count: int
+
+
+@dataclass(frozen=True)
+class MeshColorsSideInfo:
+    back_texel_offset: int
+    flags: int
+
+    @property
+    def is_double_sided(self) -> bool:
+        return bool(self.flags & MESH_COLORS_SIDE_FLAG_DOUBLE_SIDED)
+
+
+def world_positions_f32(transform: np.ndarray, positions: np.ndarray) -> np.ndarray:
+    """float32 ``M * (p, 1)`` with the dot products summed left to right."""
+    m = np.asarray(transform, dtype=np.float32)
+    p = np.asarray(positions, dtype=np.float32)
+    out = np.empty(p.shape, dtype=np.float32)
+    for r in range(3):
+        acc = m[r, 0] * p[..., 0]
+        acc = acc + m[r, 1] * p[..., 1]
+        acc = acc + m[r, 2] * p[..., 2]
+        acc = acc + m[r, 3]
+        out[..., r] = acc
+    return out
+
+
+class MeshColorsLayout:
+    """Face, instance and side tables of one scene (mesh_colors.py:121-297).
+
+    The table arrays are ``face_table`` ``uint32[F,2]`` (address,
+    resolution), ``instance_table`` ``uint32[I,4]`` (face_offset,
+    face_count, texel_offset, texel_count) and ``side_table``
+    ``uint32[I,2]`` (back_texel_offset, flags).
+    """
+
+    def __init__(self, face_table, instance_table, side_table, total_surface_texels,
+                 total_payload_count, texels_per_unit, min_resolution, max_resolution):
+        self.face_table = np.ascontiguousarray(face_table, dtype=np.uint32)
+        self.instance_table = np.ascontiguousarray(instance_table, dtype=np.uint32)
+        self.side_table = np.ascontiguousarray(side_table, dtype=np.uint32)
+        self.total_surface_texels = int(total_surface_texels)
+        self.total_payload_count = int(total_payload_count)
+        self.texels_per_unit = float(texels_per_unit)
+        self.min_resolution = int(min_resolution)
+        self.max_resolution = int(max_resolution)
+
+    @property
+    def total_texel_count(self) -> int:
+        return self.total_surface_texels
+
+    @property
+    def face_infos(self) -> tuple[MeshColorsFaceInfo, ...]:
+        return tuple(MeshColorsFaceInfo(int(a), int(r)) for a, r in self.face_table)
+
+    @property
+    def instance_infos(self) -> tuple[MeshColorsInstanceInfo, ...]:
+        return tuple(MeshColorsInstanceInfo(*(int(x) for x in row)) for row in self.instance_table)
+
+    @property
+    def side_infos(self) -> tuple[MeshColorsSideInfo, ...]:
+        return tuple(MeshColorsSideInfo(int(a), int(f)) for a, f in self.side_table)
+
+    def front_prefix(self) -> np.ndarray:
+        """Exclusive prefix of front texel counts per instance (``uint32[I]``):
+        the flat front-texel index of each instance's first texel."""
+        counts = self.instance_table[:, 3].astype(np.int64)
+        out = np.zeros(len(counts), dtype=np.int64)
+        if len(counts) > 1:
+            np.cumsum(counts[:-1], out=out[1:])
+        return out.astype(np.uint32)
+
+    def texel_face_map(self) -> np.ndarray:
+        """Global face index of every front texel, in flat front-texel order
+        (``uint32[total_surface_texels]``).  Faces tile each instance's front
+        range contiguously, so this equals the result of the shader's
+        per-texel binary search (mesh_colors.slang:272-305)."""
+        res = self.face_table[:, 1].astype(np.int64)
+        counts = (res + 1) * (res + 2) // 2
+        return np.repeat(np.arange(len(res), dtype=np.uint32), counts)
+
+    def face_instance_map(self) -> np.ndarray:
+        """Instance index of every face (``uint32[F]``)."""
+        return np.repeat(
+            np.arange(len(self.instance_table), dtype=np.uint32),
+            self.instance_table[:, 1].astype(np.int64),
+        )
+
+    @classmethod
+    def build(cls, scene_node: SceneNode, *, texels_per_unit: float = 16.0, min_resolution: int = 4,
+              max_resolution: int = 64, max_total_texels: int = 16_777_216) -> "MeshColorsLayout":
+        """Greedy per-triangle resolution assignment (mesh_colors.py:147-297)."""
+        texels_per_unit = max(float(texels_per_unit), 1e-6)
+        min_resolution = next_power_of_two(max(1, int(min_resolution)))
+        max_resolution = previous_power_of_two(max(1, int(max_resolution)))
+        if min_resolution > max_resolution:
+            raise ValueError(
+                "Mesh Colors min_resolution must not exceed max_resolution "
+                "after power-of-two normalization"
+            )
+        max_total_texels = max(1, int(max_total_texels))
+
+        face_chunks: list[np.ndarray] = []
+        instance_rows = []
+        side_rows = []
+        total_surface = 0
+        total_payload = 0
+        face_count_so_far = 0
+        # Resolutions depend only on (mesh, transform); cache them.
+        res_cache: dict[tuple[int, int], np.ndarray] = {}
+
+        for mesh_id, material_id, transform_id in scene_node.instances:
+            mesh = scene_node.meshes[mesh_id]
+            material = scene_node.materials[material_id]
+            double_sided = bool(getattr(material, "double_sided", False))
+            mult = 2 if double_sided else 1
+            key = (mesh_id, transform_id)
+            res = res_cache.get(key)
+            if res is None:
+                res = cls._unclamped_resolutions(
+                    mesh, scene_node.transforms[transform_id].matrix,
+                    texels_per_unit, min_resolution, max_resolution)
+                res_cache[key] = res
+            res = res.copy()
+            patch = (res + 1) * (res + 2) // 2
+            local_end = np.cumsum(patch)  # local address after each triangle
+            over = total_payload + local_end * mult > max_total_texels
+            if over.any():
+                first = int(np.argmax(over))
+                local = int(local_end[first - 1]) if first > 0 else 0
+                for t in range(first, len(res)):
+                    r = int(res[t])
+                    p = (r + 1) * (r + 2) // 2
+                    if total_payload + (local + p) * mult > max_total_texels:
+                        candidate = r
+                        while True:
+                            if candidate <= min_resolution:
+                                raise ValueError(BUDGET_ERROR)
+                            candidate //= 2
+                            p = (candidate + 1) * (candidate + 2) // 2
+                            if total_payload + (local + p) * mult <= max_total_texels:
+                                r = candidate
+                                break
+                        res[t] = r
+                    local += p
+                patch = (res + 1) * (res + 2) // 2
+                local_end = np.cumsum(patch)
+            n = len(res)
+            texel_count = int(local_end[-1]) if n else 0
+            addresses = np.zeros(n, dtype=np.int64)
+            if n > 1:
+                addresses[1:] = local_end[:-1]
+            face_chunks.append(np.stack([addresses, res], axis=1))
+            instance_rows.append((face_count_so_far, n, total_payload, texel_count))
+            back = total_payload + texel_count if double_sided else MESH_COLORS_INVALID_OFFSET
+            side_rows.append((back, MESH_COLORS_SIDE_FLAG_DOUBLE_SIDED if double_sided else 0))
+            face_count_so_far += n
+            total_surface += texel_count
+            total_payload += texel_count * mult
+
+        if total_surface == 0:
+            raise ValueError("Texture-space rendering requires at least one triangle")
+        if total_payload >= 1 << 32:
+            raise ValueError("Mesh Colors payload exceeds 32-bit slot indices")
+
+        faces = np.concatenate(face_chunks, axis=0) if face_chunks else np.zeros((0, 2))
+        return cls(
+            faces.astype(np.uint32),
+            np.array(instance_rows, dtype=np.uint64).astype(np.uint32).reshape(-1, 4),
+            np.array(side_rows, dtype=np.uint64).astype(np.uint32).reshape(-1, 2),
+            total_surface, total_payload, texels_per_unit, min_resolution, max_resolution,
+        )
+
+    @staticmethod
+    def _unclamped_resolutions(mesh, matrix, tpu, min_res, max_res) -> np.ndarray:
+        """``min(max_res, nextpow2(max(min_res, ceil(max_edge * tpu))))`` per
+        triangle (mesh_colors.py:215-228)."""
+        world = world_positions_f32(matrix, mesh.vertices[:, 0:3]).astype(np.float64)
+        tri = world[mesh.indices.astype(np.int64)]  # [T,3,3]
+        e0 = np.sqrt(np.sum((tri[:, 1] - tri[:, 0]) ** 2, axis=1))
+        e1 = np.sqrt(np.sum((tri[:, 2] - tri[:, 1]) ** 2, axis=1))
+        e2 = np.sqrt(np.sum((tri[:, 0] - tri[:, 2]) ** 2, axis=1))
+        max_edge = np.maximum(np.maximum(e0, e1), e2)
+        cells = np.ceil(max_edge * tpu)
+        cells = np.maximum(cells, float(min_res))
+        cells = np.minimum(cells, float(1 << 40))
+        res = _next_pow2_array(np.maximum(cells, 1).astype(np.int64))
+        return np.minimum(res, max_res).astype(np.int64)
+
+    def summary(self) -> str:
+        """The reference's one-line layout report
+        (texture_space_path_tracing_renderer.py:92-100)."""
+        back = self.total_payload_count - self.total_surface_texels
+        return (
+            f"[TextureSpace] {len(self.instance_table)} instances, {len(self.face_table)} faces, "
+            f"{self.total_surface_texels:,} surface texels, {back:,} back texels "
+            f"({self.total_payload_count * MESH_COLORS_PAYLOAD_SIZE / (1024 * 1024):.1f} MiB writable; "
+            f"{self.total_payload_count * 4 / (1024 * 1024):.1f} MiB RGB9E5)"
+        )

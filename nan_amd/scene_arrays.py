@@ -1,0 +1,67 @@
+"""Flattening of a :class:`SceneNode` into the flat arrays the native
+library consumes (the buffers scene.py:330-437 of the reference uploads).
+
+* ``vertices``        float32[V,8]  all meshes concatenated (32-byte stride)
+* ``indices``         uint32[T*3]   mesh-local indices, meshes concatenated
+* ``mesh_descs``      uint32[M,4]   vertex_count, index_count, vertex_offset, index_offset
+* ``instance_descs``  uint32[I,4]   mesh_id, material_id, transform_id, flags
+* ``transforms``      float32[X,16] row-major object-to-world
+* ``inv_transposes``  float32[X,16] row-major transpose(inverse(M))
+* ``materials``       uint32[Mat,8] see ``Material.pack``
+* ``lighting``        float32[16]   see ``Lighting.pack``
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .material import MATERIAL_RECORD_WORDS
+from .scene_node import SceneNode
+from .transform import inverse_transpose, is_odd_negative
+
+INSTANCE_FLAG_ODD_NEGATIVE_TRANSFORM = 1 << 0
+
+
+def flatten_scene(scene_node: SceneNode) -> dict:
+    if not scene_node.instances:
+        raise ValueError("scene has no instances")
+    mesh_descs = np.zeros((len(scene_node.meshes), 4), dtype=np.uint32)
+    vcount = 0
+    icount = 0
+    for k, mesh in enumerate(scene_node.meshes):
+        mesh_descs[k] = (mesh.vertex_count, mesh.index_count, vcount, icount)
+        vcount += mesh.vertex_count
+        icount += mesh.index_count
+    vertices = np.concatenate([m.vertices for m in scene_node.meshes], axis=0).astype(np.float32)
+    indices = np.concatenate([m.indices.reshape(-1) for m in scene_node.meshes]).astype(np.uint32)
+    transforms = np.stack([np.asarray(t.matrix, dtype=np.float32).reshape(16) for t in scene_node.transforms])
+    inv_t = np.stack([inverse_transpose(t.matrix).reshape(16) for t in scene_node.transforms])
+    odd = np.array([is_odd_negative(t.matrix) for t in scene_node.transforms], dtype=bool)
+    inst = np.asarray(scene_node.instances, dtype=np.uint32).reshape(-1, 3)
+    instance_descs = np.zeros((len(inst), 4), dtype=np.uint32)
+    instance_descs[:, 0:3] = inst
+    instance_descs[:, 3] = np.where(odd[inst[:, 2]], INSTANCE_FLAG_ODD_NEGATIVE_TRANSFORM, 0)
+    if scene_node.materials:
+        materials = np.stack([m.pack() for m in scene_node.materials]).astype(np.uint32)
+    else:
+        materials = np.zeros((0, MATERIAL_RECORD_WORDS), dtype=np.uint32)
+    return {
+        "vertices": np.ascontiguousarray(vertices),
+        "indices": np.ascontiguousarray(indices),
+        "mesh_descs": mesh_descs,
+        "instance_descs": instance_descs,
+        "transforms": np.ascontiguousarray(transforms, dtype=np.float32),
+        "inv_transposes": np.ascontiguousarray(inv_t, dtype=np.float32),
+        "materials": materials,
+        "lighting": scene_node.lighting.pack(),
+    }
+
+
+def instance_triangle_offsets(arrays: dict) -> np.ndarray:
+    """Global triangle id of each instance's first triangle (``uint32[I]``);
+    global id = offset[instance] + primitive index."""
+    tri_per_inst = arrays["mesh_descs"][arrays["instance_descs"][:, 0], 1].astype(np.int64) // 3
+    out = np.zeros(len(tri_per_inst), dtype=np.int64)
+    if len(out) > 1:
+        np.cumsum(tri_per_inst[:-1], out=out[1:])
+    return out.astype(np.uint32)

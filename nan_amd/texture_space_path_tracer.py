@@ -1,0 +1,103 @@
+"""Texture-space path tracer pass (texture_space_path_tracer.py:7-59 of the
+reference), backed by the gfx950 kernels of libnan_hip.
+
+The reference dispatches ``compute_main`` once per instance
+(texture_space_path_tracer.py:42-59).  Here one launch covers every front
+texel of every instance through a flat texel->face map; the per-texel work
+is the reference's (texture_space_path_tracer.slang:140-216) and every slot
+update is bit-identical to the CPU oracle.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+from .device import CommandEncoder, Device
+from .mesh_colors import MESH_COLORS_PAYLOAD_SIZE, MeshColorsLayout
+from .scene import Scene
+
+
+def _u32(a):
+    return a.ctypes.data_as(_native.u32p)
+
+
+def as_payload_tensor(buf: torch.Tensor, slots: int) -> torch.Tensor:
+    """Checks a 16-byte-per-slot payload buffer (float32[N,4] or any
+    contiguous tensor of ``16 * N`` bytes) on the device."""
+    if not buf.is_cuda or not buf.is_contiguous():
+        raise ValueError("payload must be a contiguous device tensor")
+    if buf.numel() * buf.element_size() < slots * MESH_COLORS_PAYLOAD_SIZE:
+        raise ValueError(f"payload holds fewer than {slots} 16-byte slots")
+    return buf
+
+
+class TextureSpacePathTracer:
+    """Progressive per-texel irradiance tracer (texture_space_path_tracer.py:7-59)."""
+
+    def __init__(self, device: Device, scene: Scene, layout: MeshColorsLayout, *, samples_per_texel: int = 1,
+                 max_bounces: int = 3):
+        self.device = device
+        self.scene = scene
+        self.layout = layout
+        self.samples_per_texel = max(1, int(samples_per_texel))
+        self.max_bounces = max(1, int(max_bounces))
+        handle = ctypes.c_void_p()
+        err = _native.err_buffer()
+        rc = _native.lib().nan_tracer_create(
+            scene.handle, _u32(layout.face_table), len(layout.face_table), _u32(layout.instance_table),
+            _u32(layout.side_table), len(layout.instance_table), layout.total_surface_texels,
+            layout.total_payload_count, ctypes.byref(handle), err, len(err))
+        if rc != 0:
+            raise _native.NanError(err.value.decode())
+        self.handle = handle
+        dev = device.torch_device
+        # The reference exposes its face/instance/side/adjacency buffers for the
+        # resolve and filter passes (texture_space_path_tracer.py:26-31).
+        self.face_infos_buffer = torch.from_numpy(layout.face_table.view(np.int32)).to(dev)
+        self.instance_infos_buffer = torch.from_numpy(layout.instance_table.view(np.int32)).to(dev)
+        self.side_infos_buffer = torch.from_numpy(layout.side_table.view(np.int32)).to(dev)
+        self.adjacency_infos_buffer = None
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _native.lib().nan_tracer_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, command_encoder: CommandEncoder, output: torch.Tensor, iteration: int, *,
+                reset: bool = False, counters: torch.Tensor | None = None) -> None:
+        """One iteration over all texels into ``output`` (16-byte slots)."""
+        as_payload_tensor(output, self.layout.total_payload_count)
+        stream = command_encoder.stream_handle
+        it = max(0, int(iteration))
+        if counters is None:
+            rc = _native.lib().nan_tst_execute(self.handle, output.data_ptr(), it, self.samples_per_texel,
+                                               self.max_bounces, 1 if reset else 0, stream)
+        else:
+            if counters.dtype != torch.int64 or counters.numel() < _native.COUNTER_COUNT or not counters.is_cuda:
+                raise ValueError("counters must be an int64 device tensor of at least 8 entries")
+            rc = _native.lib().nan_tst_execute_counted(self.handle, output.data_ptr(), it, self.samples_per_texel,
+                                                       self.max_bounces, 1 if reset else 0, counters.data_ptr(),
+                                                       stream)
+        _native.check(rc, "nan_tst_execute")
+
+    def raygen(self, command_encoder: CommandEncoder, iteration: int) -> tuple[torch.Tensor, torch.Tensor]:
+        """Texel ray generation only: int32[front, 4] (instance, primitive,
+        bary.x bits, bary.y bits) and the RNG state after jitter."""
+        n = self.layout.total_surface_texels
+        dev = self.device.torch_device
+        out = torch.empty((n, 4), dtype=torch.int32, device=dev)
+        rng = torch.empty((n,), dtype=torch.int32, device=dev)
+        _native.check(_native.lib().nan_tst_raygen(self.handle, int(iteration), self.samples_per_texel,
+                                                   out.data_ptr(), rng.data_ptr(), command_encoder.stream_handle),
+                      "nan_tst_raygen")
+        return out, rng
