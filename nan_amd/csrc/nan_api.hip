@@ -10,7 +10,9 @@
 //   accumulate / tonemap (accumulator.slang, tone_mapper.slang)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -749,6 +751,9 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     s->dev.inv_transposes = itf;
     s->dev.materials = mats;
     s->dev.root = bvh.root_entry;
+    float max_coord = 1.0f;
+    for (float v : world) max_coord = std::max(max_coord, std::fabs(v));
+    s->dev.slack = max_coord * 0x1p-18f;
     s->dev.light = make_light(d->lighting);
     s->stack_depth = depth;
     s->instance_triangles.resize(d->instance_count);

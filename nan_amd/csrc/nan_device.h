@@ -21,11 +21,15 @@ constexpr float kInvPi = 0x1.45f306p-2f;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kLeafCountShift = 27;
 constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
-// Ray-box slack: the far distance is inflated by 1 + 2^-20 (covers the
-// 2*gamma_3 bound of Ize, "Robust BVH Ray Traversal", JCGT 2013) and the
-// current hit distance by 1 + 2^-12 so near-ties are never culled.
-constexpr float kBoxPad = 0x1.000008p+0f;
-constexpr float kTPad = 0x1.001p+0f;
+// Ray-box slack.  A child box is visited when its entry distance is within
+// far * (1 + 2^-10) + slack and within best_t * (1 + 2^-10) + slack, where
+// slack = 2^-18 * max(1, largest world coordinate) (DevScene::slack).  The
+// relative term covers the slab test's rounding (2*gamma_3 of Ize, "Robust
+// BVH Ray Traversal", JCGT 2013); the absolute term covers the FP32 triangle
+// test reporting a distance below its box's true entry (or a hit just behind
+// the origin) by a few ulps of the coordinates, so culling never changes the
+// closest hit.
+constexpr float kPad = 0x1.004p+0f;
 
 struct f3 {
     float x, y, z;
@@ -112,6 +116,7 @@ struct DevScene {
     const float4 *inv_transposes; // 4 float4 rows
     const float4 *materials;      // 2 float4: base.rgb flags | emissive.rgb alpha
     uint32_t root;
+    float slack;                  // absolute box-test slack (see kPad)
     Light light;
 };
 
@@ -204,7 +209,7 @@ __device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &h
             uint32_t valid = w0.w >> 24;
             uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
             uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
-            float limit = (ANY_HIT ? r.tmax : hit.t) * kTPad;
+            float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack;
             float tn[4];
             uint32_t ent[4];
             int nhit = 0;
@@ -220,7 +225,7 @@ __device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &h
                 float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
                 float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
                 float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-                float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxPad;
+                float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + S.slack;
                 far = fminf(far, limit);
                 bool ok = ((valid >> c) & 1u) && near <= far;
                 // insertion into a sorted list (ascending near), unrolled

@@ -80,6 +80,9 @@ def liboracle():
     lib.orc_accumulate.argtypes = [f32p, f32p, f32p, ctypes.c_uint64, ctypes.c_int]
     lib.orc_tonemap.argtypes = [f32p, f32p, ctypes.c_uint64, ctypes.c_float]
     lib.orc_sincos_2pi.argtypes = [f32p, ctypes.c_uint64, f32p, f32p]
+    lib.orc_set_crosscheck.argtypes = [ctypes.c_int]
+    lib.orc_crosscheck_result.restype = ctypes.c_int
+    lib.orc_crosscheck_result.argtypes = [f32p]
     _LIB = lib
     return lib
 

@@ -327,6 +327,7 @@ typedef struct {
     uint32_t node_count;
     uint32_t *order;     /* leaf triangle order */
     int use_bvh;
+    float slack;         /* absolute box-test slack, see box_hit */
 } orc_scene;
 
 static void build_world_triangles(orc_scene *s) {
@@ -464,6 +465,9 @@ ORC_API orc_scene *orc_scene_create(const orc_scene_desc *desc, int use_bvh) {
     s->d = *desc;
     s->use_bvh = use_bvh;
     build_world_triangles(s);
+    float m = 1.0f;
+    for (size_t i = 0; i < 9 * (size_t)s->tri_count; ++i) m = fmaxf(m, fabsf(s->tri[i]));
+    s->slack = m * 0x1p-18f;
     if (use_bvh) build_bvh(s);
     return s;
 }
@@ -547,15 +551,20 @@ static int tri_intersect(const orc_ray *r, const float *tri, float *t_out, float
     return 1;
 }
 
-#define ORC_BOX_PAD 0x1.000008p+0f  /* 1 + 2^-20 */
-#define ORC_T_PAD 0x1.001p+0f       /* 1 + 2^-12 */
+/* Box-test slack.  A box is visited when its entry distance is within
+ * far * (1 + 2^-10) + slack and within best_t * (1 + 2^-10) + slack, where
+ * slack = 2^-18 * max(1, largest world coordinate).  The relative term covers
+ * the slab test's rounding (2 gamma_3, Ize 2013); the absolute term covers the
+ * FP32 triangle test reporting a distance below its box's true entry (or a
+ * hit just behind the origin) by a few ulps of the coordinates. */
+#define ORC_PAD 0x1.004p+0f  /* 1 + 2^-10 */
 
-static int box_hit(const orc_ray *r, const float *lo, const float *hi, float tfar_limit) {
+static int box_hit(const orc_ray *r, const float *lo, const float *hi, float tfar_limit, float slack) {
     float t0x = (lo[0] - r->o.x) * r->inv.x, t1x = (hi[0] - r->o.x) * r->inv.x;
     float t0y = (lo[1] - r->o.y) * r->inv.y, t1y = (hi[1] - r->o.y) * r->inv.y;
     float t0z = (lo[2] - r->o.z) * r->inv.z, t1z = (hi[2] - r->o.z) * r->inv.z;
     float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r->tmin));
-    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * ORC_BOX_PAD;
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * ORC_PAD + slack;
     tf = fminf(tf, tfar_limit);
     return tn <= tf;
 }
@@ -580,8 +589,8 @@ static orc_hit closest_hit(const orc_scene *s, const orc_ray *r, uint64_t *tri_t
         stack[sp++] = 0;
         while (sp > 0) {
             const orc_bnode *nd = &s->nodes[stack[--sp]];
-            float limit = h.valid ? h.t * ORC_T_PAD : r->tmax * ORC_T_PAD;
-            if (!box_hit(r, nd->lo, nd->hi, limit)) continue;
+            float limit = h.t * ORC_PAD + s->slack;
+            if (!box_hit(r, nd->lo, nd->hi, limit, s->slack)) continue;
             if (nd->b & ORC_LEAF) {
                 uint32_t cnt = nd->b & ~ORC_LEAF;
                 for (uint32_t i = 0; i < cnt; ++i) {
@@ -616,7 +625,7 @@ static int any_hit(const orc_scene *s, const orc_ray *r) {
     stack[sp++] = 0;
     while (sp > 0) {
         const orc_bnode *nd = &s->nodes[stack[--sp]];
-        if (!box_hit(r, nd->lo, nd->hi, r->tmax * ORC_T_PAD)) continue;
+        if (!box_hit(r, nd->lo, nd->hi, r->tmax * ORC_PAD + s->slack, s->slack)) continue;
         if (nd->b & ORC_LEAF) {
             uint32_t cnt = nd->b & ~ORC_LEAF;
             for (uint32_t i = 0; i < cnt; ++i) {
@@ -629,6 +638,26 @@ static int any_hit(const orc_scene *s, const orc_ray *r) {
         }
     }
     return 0;
+}
+
+/* Debug cross-check: with orc_set_crosscheck(1), every BVH query is repeated
+ * by brute force and the first disagreeing ray is kept. */
+static int g_crosscheck = 0;
+static float g_bad_ray[8];
+static int g_bad_kind = 0;
+static pthread_mutex_t g_bad_lock = PTHREAD_MUTEX_INITIALIZER;
+
+ORC_API void orc_set_crosscheck(int on) { g_crosscheck = on; g_bad_kind = 0; }
+ORC_API int orc_crosscheck_result(float *ray8) { memcpy(ray8, g_bad_ray, sizeof(g_bad_ray)); return g_bad_kind; }
+
+static void record_bad(const orc_ray *r, int kind) {
+    pthread_mutex_lock(&g_bad_lock);
+    if (!g_bad_kind) {
+        g_bad_kind = kind;
+        float q[8] = {r->o.x, r->o.y, r->o.z, r->tmin, r->d.x, r->d.y, r->d.z, r->tmax};
+        memcpy(g_bad_ray, q, sizeof(q));
+    }
+    pthread_mutex_unlock(&g_bad_lock);
 }
 
 /* rays: [n][8] = origin.xyz, tmin, dir.xyz, tmax.  Outputs t, (b1,b2), instance, prim;
@@ -869,7 +898,13 @@ static int shadow_visible(const orc_scene *s, f3 origin, f3 dir, orc_counters *c
     r.o = origin; r.d = dir; r.tmin = 0.0f; r.tmax = 1000.0f;
     ray_setup(&r);
     cnt->rays_shadow++;
-    return !any_hit(s, &r);
+    int occ = any_hit(s, &r);
+    if (g_crosscheck && s->use_bvh) {
+        orc_scene tmp = *s;
+        tmp.use_bvh = 0;
+        if (any_hit(&tmp, &r) != occ) record_bad(&r, 2);
+    }
+    return !occ;
 }
 
 /* trace_radiance (texture_space_path_tracer.slang:57-84) */
@@ -882,6 +917,12 @@ static f3 trace_radiance(const orc_scene *s, const orc_light *L, f3 origin, f3 d
         ray_setup(&r);
         cnt->rays_extend++;
         orc_hit h = closest_hit(s, &r, &cnt->tri_tests);
+        if (g_crosscheck && s->use_bvh) {
+            orc_scene tmp = *s;
+            tmp.use_bvh = 0;
+            orc_hit b = closest_hit(&tmp, &r, NULL);
+            if (b.valid != h.valid || b.id != h.id || b.t != h.t) record_bad(&r, 1);
+        }
         if (!h.valid) {
             rad = add(rad, mul(T, L->sky));
             return rad;

@@ -289,3 +289,24 @@ def test_lattice_point_resolve_reads_its_texel(cornell):
                      payload=payload)
     center = img[40, 32, :3]
     np.testing.assert_allclose(center, np.asarray([0.73] * 3) / np.pi * 2.0, rtol=1e-5)
+
+
+def test_oracle_bvh_queries_equal_brute_force_sunlit_scene():
+    # every extend and shadow query of a sunlit batch scene, repeated by brute
+    # force inside the oracle: the box-test slack keeps culling exact even for
+    # the coplanar box bottoms buried in the terrain
+    node = batch_scene(21, 6000)
+    layout = MeshColorsLayout.build(node, texels_per_unit=8.0, min_resolution=1, max_resolution=8,
+                                    max_total_texels=2_000_000)
+    lib = oracle.liboracle()
+    lib.orc_set_crosscheck(1)
+    try:
+        sc = oracle.OracleScene(flatten_scene(node))
+        p = np.zeros((layout.total_payload_count, 4), dtype=np.float32)
+        sc.trace_texels(layout.face_table, layout.instance_table, layout.side_table, p, 0, 1, 3, True,
+                        threads=oracle.cpu_count())
+        ray = np.zeros(8, dtype=np.float32)
+        kind = lib.orc_crosscheck_result(ray.ctypes.data_as(oracle.f32p))
+    finally:
+        lib.orc_set_crosscheck(0)
+    assert kind == 0, f"BVH and brute force disagree on ray {ray.tolist()} (kind {kind})"
