@@ -112,7 +112,21 @@ def main() -> int:
     ap.add_argument("--workload", choices=["synthetic", "cornell"], default="synthetic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = 1 << 28  # 2^28 float32 = 1 GiB
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n, "copies": 3}))
+        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the path shards by texel range with no "
@@ -183,7 +197,6 @@ def main() -> int:
     # shaded hits: every texel surface (front/back) + every extend ray that hit
     algo_bytes = (nodes * BYTES_PER_NODE + tris * BYTES_PER_TRI + slots * BYTES_PER_SLOT_UPDATE
                   + front * BYTES_PER_TEXEL_SETUP + (slots + ext) * BYTES_PER_SHADED_HIT)
-    samples_per_s = slots / (avg_launch_ms * 1e-3)
     value = slots * args.steps / (total_ms * 1e-3)
     mrays = (ext + shd) * args.steps / (total_ms * 1e-3) / 1e6
     achieved = algo_bytes / (avg_launch_ms * 1e-3) / 1e9
