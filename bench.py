@@ -29,6 +29,9 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+ROOFLINE_KERNEL = "tst_megakernel"
+# bumped whenever the tracer kernels change, so stale rocprof summaries are ignored
+KERNEL_VERSION = 1
 METRIC = "texel-path samples/s (Mrays/s) at equal RMSE vs 4096-spp bake; learned vs plain"
 
 # algorithmic bytes (see the module docstring)
@@ -241,6 +244,16 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    # HBM traffic of this kernel on this workload, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); null when the summary
+    # is for another kernel variant or workload
+    prof = ROOT / "profiles" / f"traffic_{ROOFLINE_KERNEL}_{args.workload}.json"
+    if prof.exists():
+        rec = json.loads(prof.read_text())
+        if str(layout.total_payload_count) in rec.get("workload", "") and rec.get("kernel_version") == KERNEL_VERSION:
+            out["roofline"]["traffic"] = rec["traffic_bytes_per_launch"]
+            out["roofline"]["l2_hit_rate"] = rec.get("l2_hit_rate")
+            out["roofline"]["traffic_source"] = rec.get("source")
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)
