@@ -8,11 +8,13 @@ sky 0.3), texels_per_unit 8, min_resolution 2, max_resolution 16, budget
 runs the small Cornell box instead.
 
 Prints one JSON line.  ``value`` is texel-path samples/s (slot updates per
-second) with the scene, layout and cache resident in HBM.  Work counts come
-from a counted replay of the same iterations; the roofline entry prices the
-tracer kernel at its algorithmic bytes (counted BVH node and triangle
-fetches, slot read/write, texel->face map, layout tables and per-hit shading
-records) over its measured launch time.
+second) with the scene, layout and cache resident in HBM.  Every tracer
+kernel launch in the timed region is bracketed by HIP events on its stream
+(``nan_tracer_profile``); work counts come from a counted replay of the
+same iterations.  The roofline entry is for the stage with the largest
+time (the closest-hit traversal kernel ``wf_extend`` in wavefront mode):
+its algorithmic bytes (64 B per counted node visit, 48 B per triangle test,
+the queued ray and the hit record) over its measured launch time.
 """
 
 from __future__ import annotations
@@ -29,9 +31,8 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-ROOFLINE_KERNEL = "tst_megakernel"
 # bumped whenever the tracer kernels change, so stale rocprof summaries are ignored
-KERNEL_VERSION = 1
+KERNEL_VERSION = 2
 METRIC = "texel-path samples/s (Mrays/s) at equal RMSE vs 4096-spp bake; learned vs plain"
 
 # algorithmic bytes (see the module docstring)
@@ -42,6 +43,9 @@ BYTES_PER_TEXEL_SETUP = 4 + 4 + 16 + 8 + 4 + 8   # texel->face, face->instance, 
 # per shaded hit: tri id map 8 + instance 16 + mesh 16 + 3 indices 12 + 3 vertices x 32
 # + transform 48 + inverse transpose 64 + material 32
 BYTES_PER_SHADED_HIT = 8 + 16 + 16 + 12 + 96 + 48 + 64 + 32
+BYTES_PER_RAY = 32                 # queued origin + direction (float4 x 2)
+BYTES_PER_HIT = 16                 # t, b1, b2, triangle id
+BYTES_PER_SHADOW_RAY = 48          # origin, direction, contribution (float4 x 3)
 
 
 def log(msg: str) -> None:
@@ -113,6 +117,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["synthetic", "cornell"], default="synthetic")
+    ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -155,7 +160,7 @@ def main() -> int:
     scene = Scene(device, node)
     log(f"BVH: {scene.info.node_count} nodes, {scene.info.leaf_count} leaves, max stack {scene.info.max_stack}, "
         f"build {scene.info.build_ms:.0f} ms, upload total {time.perf_counter() - t0:.1f} s")
-    tracer = TextureSpacePathTracer(device, scene, layout, samples_per_texel=1, max_bounces=3)
+    tracer = TextureSpacePathTracer(device, scene, layout, samples_per_texel=1, max_bounces=3, mode=args.mode)
     payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=device.torch_device)
     enc = device.create_command_encoder()
     stream = torch.cuda.current_stream(device.torch_device)
@@ -165,11 +170,12 @@ def main() -> int:
         tracer.execute(enc, payload, it, reset=(it == 0))
         it += 1
     torch.cuda.synchronize()
-    log(f"warmup {args.warmup} iterations done")
+    log(f"warmup {args.warmup} iterations done ({args.mode})")
 
     first_timed = it
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    tracer.profile(True)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     for k in range(args.steps):
@@ -179,10 +185,11 @@ def main() -> int:
         it += 1
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    stages = tracer.stage_times()
+    tracer.profile(False)
     total_ms = starts[0].elapsed_time(ends[-1])
-    avg_launch_ms = float(np.mean(launch_ms))
-    log(f"timed {args.steps} iterations: {total_ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)")
+    log(f"timed {args.steps} iterations: {total_ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall); stages "
+        + ", ".join(f"{k} {v[0] / args.steps:.2f} ms" for k, v in stages.items()))
 
     # counted replay of the same iterations on a scratch copy (counts depend
     # only on the iteration, not on the accumulated values)
@@ -193,16 +200,28 @@ def main() -> int:
     torch.cuda.synchronize()
     c = counters.cpu().numpy().astype(np.int64) // args.steps
     del scratch
-    ext, shd, nodes, tris, slots = (int(c[_native.COUNTER_EXTEND_RAYS]), int(c[_native.COUNTER_SHADOW_RAYS]),
-                                    int(c[_native.COUNTER_NODE_VISITS]), int(c[_native.COUNTER_TRI_TESTS]),
-                                    int(c[_native.COUNTER_SLOT_UPDATES]))
+    ext, shd = int(c[_native.COUNTER_EXTEND_RAYS]), int(c[_native.COUNTER_SHADOW_RAYS])
+    nodes, tris = int(c[_native.COUNTER_NODE_VISITS]), int(c[_native.COUNTER_TRI_TESTS])
+    snodes, stris = int(c[_native.COUNTER_SHADOW_NODE_VISITS]), int(c[_native.COUNTER_SHADOW_TRI_TESTS])
+    slots = int(c[_native.COUNTER_SLOT_UPDATES])
     front = layout.total_surface_texels
-    # shaded hits: every texel surface (front/back) + every extend ray that hit
-    algo_bytes = (nodes * BYTES_PER_NODE + tris * BYTES_PER_TRI + slots * BYTES_PER_SLOT_UPDATE
-                  + front * BYTES_PER_TEXEL_SETUP + (slots + ext) * BYTES_PER_SHADED_HIT)
     value = slots * args.steps / (total_ms * 1e-3)
     mrays = (ext + shd) * args.steps / (total_ms * 1e-3) / 1e6
-    achieved = algo_bytes / (avg_launch_ms * 1e-3) / 1e9
+
+    # algorithmic bytes per frame of each stage
+    stage_bytes = {
+        "extend": nodes * BYTES_PER_NODE + tris * BYTES_PER_TRI + ext * (BYTES_PER_RAY + BYTES_PER_HIT),
+        "shadow": snodes * BYTES_PER_NODE + stris * BYTES_PER_TRI + shd * BYTES_PER_SHADOW_RAY,
+        "megakernel": ((nodes + snodes) * BYTES_PER_NODE + (tris + stris) * BYTES_PER_TRI
+                       + slots * BYTES_PER_SLOT_UPDATE + front * BYTES_PER_TEXEL_SETUP
+                       + (slots + ext) * BYTES_PER_SHADED_HIT),
+    }
+    dominant = max((k for k in stages if k in stage_bytes), key=lambda k: stages[k][0])
+    dom_ms, dom_launches = stages[dominant]
+    bytes_per_launch = stage_bytes[dominant] * args.steps / dom_launches
+    avg_launch_ms = dom_ms / dom_launches
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    kernel_name = {"extend": "wf_extend", "shadow": "wf_shadow", "megakernel": "tst_megakernel"}[dominant]
 
     out = {
         "metric": METRIC,
@@ -226,11 +245,14 @@ def main() -> int:
             "layout": layout_kw,
             "samples_per_texel": 1,
             "max_bounces": 3,
+            "tracer": args.mode,
             "parallelism": "single GPU",
         },
         "mrays_per_s": mrays,
         "rays_per_step": {"extend": ext, "shadow": shd},
-        "traversal_per_step": {"node_visits": nodes, "tri_tests": tris},
+        "traversal_per_step": {"extend_node_visits": nodes, "extend_tri_tests": tris,
+                               "shadow_node_visits": snodes, "shadow_tri_tests": stris},
+        "stage_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -238,16 +260,17 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "tst_megakernel",
-            "algorithmic_bytes_per_launch": algo_bytes,
+            "kernel": kernel_name,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_launch_ms,
+            "launches_per_step": dom_launches / args.steps,
         },
         "cpu_baseline": None,
     }
     # HBM traffic of this kernel on this workload, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); null when the summary
     # is for another kernel variant or workload
-    prof = ROOT / "profiles" / f"traffic_{ROOFLINE_KERNEL}_{args.workload}.json"
+    prof = ROOT / "profiles" / f"traffic_{kernel_name}_{args.workload}.json"
     if prof.exists():
         rec = json.loads(prof.read_text())
         if str(layout.total_payload_count) in rec.get("workload", "") and rec.get("kernel_version") == KERNEL_VERSION:

@@ -78,9 +78,11 @@ typedef struct NanTracer NanTracer;
 enum {
     NAN_COUNTER_EXTEND_RAYS = 0,
     NAN_COUNTER_SHADOW_RAYS = 1,
-    NAN_COUNTER_NODE_VISITS = 2,
-    NAN_COUNTER_TRI_TESTS = 3,
+    NAN_COUNTER_NODE_VISITS = 2,   /* extend (closest-hit) rays */
+    NAN_COUNTER_TRI_TESTS = 3,     /* extend (closest-hit) rays */
     NAN_COUNTER_SLOT_UPDATES = 4,
+    NAN_COUNTER_SHADOW_NODE_VISITS = 5,
+    NAN_COUNTER_SHADOW_TRI_TESTS = 6,
     NAN_COUNTER_COUNT = 8
 };
 
@@ -115,6 +117,23 @@ NAN_API void nan_tracer_destroy(NanTracer *tracer);
  * (texture_space_path_tracer.py:26-31 face/instance/side buffers). */
 NAN_API int nan_tracer_tables(const NanTracer *tracer, const void **face_table, const void **instance_table,
                               const void **side_table);
+
+/* Kernel organisation of nan_tst_execute; both give bit-identical payloads.
+ * NAN_TRACER_WAVEFRONT (default): staged ray-gen / extend / shade / shadow /
+ * finish kernels with persistent traversal; NAN_TRACER_MEGAKERNEL: one thread
+ * per texel runs the whole path (the reference's dispatch shape). */
+enum { NAN_TRACER_MEGAKERNEL = 0, NAN_TRACER_WAVEFRONT = 1 };
+NAN_API int nan_tracer_set_mode(NanTracer *tracer, int mode);
+
+/* Per-stage kernel timing with HIP events on the launch stream.  While enabled,
+ * every tracer kernel launch is bracketed by events; nan_tracer_stage_times
+ * waits for them and returns milliseconds summed per stage (double[NAN_STAGE_COUNT])
+ * and launch counts (uint32[NAN_STAGE_COUNT], may be NULL), then clears them.
+ * No reference counterpart (the reference has no GPU timers on this path). */
+enum { NAN_STAGE_GENERATE = 0, NAN_STAGE_EXTEND, NAN_STAGE_SHADE, NAN_STAGE_SHADOW, NAN_STAGE_FINISH,
+       NAN_STAGE_MEGAKERNEL, NAN_STAGE_COUNT };
+NAN_API int nan_tracer_profile(NanTracer *tracer, int enable);
+NAN_API int nan_tracer_stage_times(NanTracer *tracer, double *ms, uint32_t *launches);
 
 /* One progressive iteration over every texel of every instance
  * (TextureSpacePathTracer.execute, texture_space_path_tracer.py:33-59; kernel

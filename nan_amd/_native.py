@@ -57,13 +57,20 @@ COUNTER_SHADOW_RAYS = 1
 COUNTER_NODE_VISITS = 2
 COUNTER_TRI_TESTS = 3
 COUNTER_SLOT_UPDATES = 4
+COUNTER_SHADOW_NODE_VISITS = 5
+COUNTER_SHADOW_TRI_TESTS = 6
 COUNTER_COUNT = 8
+
+STAGES = ("generate", "extend", "shade", "shadow", "finish", "megakernel")
+
+TRACER_MEGAKERNEL = 0
+TRACER_WAVEFRONT = 1
 
 # every symbol include/nan_hip.h declares
 EXPORTED_SYMBOLS = (
     "nan_version", "nan_last_error", "nan_scene_create", "nan_scene_destroy", "nan_scene_info",
     "nan_scene_set_lighting", "nan_trace_rays", "nan_tracer_create", "nan_tracer_destroy", "nan_tracer_tables",
-    "nan_tst_execute", "nan_tst_execute_counted", "nan_tst_raygen", "nan_rgb9e5_pack", "nan_rgb9e5_unpack",
+    "nan_tracer_set_mode", "nan_tracer_profile", "nan_tracer_stage_times", "nan_tst_execute", "nan_tst_execute_counted", "nan_tst_raygen", "nan_rgb9e5_pack", "nan_rgb9e5_unpack",
     "nan_resolve", "nan_accumulate", "nan_tonemap",
 )
 
@@ -87,6 +94,9 @@ def _declare(lib) -> None:
                                   ctypes.c_size_t]),
         "nan_tracer_destroy": (None, [vp]),
         "nan_tracer_tables": (I, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "nan_tracer_set_mode": (I, [vp, I]),
+        "nan_tracer_profile": (I, [vp, I]),
+        "nan_tracer_stage_times": (I, [vp, ctypes.POINTER(ctypes.c_double), u32p]),
         "nan_tst_execute": (I, [vp, vp, U32, U32, U32, U32, vp]),
         "nan_tst_execute_counted": (I, [vp, vp, U32, U32, U32, U32, vp, vp]),
         "nan_tst_raygen": (I, [vp, U32, U32, vp, vp, vp]),

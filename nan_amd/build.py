@@ -37,10 +37,13 @@ def build(verbose: bool = False) -> Path:
     bvh_obj = build_dir / "nan_bvh.o"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden", "-c",
           str(CSRC / "nan_bvh.cpp"), "-o", str(bvh_obj)])
-    api_obj = build_dir / "nan_api.o"
-    _run([HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / "nan_api.hip"), "-o", str(api_obj)])
+    objs = []
+    for src in ("nan_api.hip", "nan_wavefront.hip"):
+        obj = build_dir / (Path(src).stem + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(api_obj), str(bvh_obj), "-o", str(tmp)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, str(bvh_obj), "-o", str(tmp)])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -22,275 +22,10 @@
 #include <vector>
 
 #include "nan_bvh.h"
-#include "nan_device.h"
-#include "../../include/nan_hip.h"
+#include "nan_internal.h"
+#include "nan_tracer.h"
 
-using namespace nan;
-
-namespace {
-
-thread_local std::string g_last_error;
-
-int fail(char *err, size_t cap, const char *fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    if (err && cap) {
-        strncpy(err, buf, cap - 1);
-        err[cap - 1] = '\0';
-    }
-    return -1;
-}
-
-#define HIP_TRY(expr, err, cap)                                                                  \
-    do {                                                                                         \
-        hipError_t _e = (expr);                                                                  \
-        if (_e != hipSuccess) return fail(err, cap, "%s: %s", #expr, hipGetErrorString(_e));     \
-    } while (0)
-
-template <typename T>
-int upload(T **dst, const void *src, size_t bytes, char *err, size_t cap) {
-    *dst = nullptr;
-    if (bytes == 0) bytes = 16;
-    HIP_TRY(hipMalloc((void **)dst, bytes), err, cap);
-    if (src) HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), err, cap);
-    return 0;
-}
-
-inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
-
-constexpr int kTraceBlock = 128;
-constexpr int kFlatBlock = 256;
-
-}  // namespace
-
-struct NanScene {
-    DevScene dev{};
-    std::vector<void *> allocations;
-    NanSceneInfo info{};
-    uint32_t stack_depth = 0;
-    std::vector<uint32_t> instance_triangles;  // triangles of each instance's mesh
-    ~NanScene() {
-        for (void *p : allocations) hipFree(p);
-    }
-};
-
-struct NanTracer {
-    NanScene *scene = nullptr;
-    uint2 *faces = nullptr;
-    uint4 *instances = nullptr;
-    uint2 *sides = nullptr;
-    uint32_t *texel_face = nullptr;
-    uint32_t *face_inst = nullptr;
-    uint32_t *front_prefix = nullptr;
-    uint32_t face_count = 0, instance_count = 0;
-    uint64_t total_front = 0, total_payload = 0;
-    ~NanTracer() {
-        hipFree(faces);
-        hipFree(instances);
-        hipFree(sides);
-        hipFree(texel_face);
-        hipFree(face_inst);
-        hipFree(front_prefix);
-    }
-};
-
-struct DevLayout {
-    const uint2 *faces;
-    const uint4 *instances;
-    const uint2 *sides;
-    const uint32_t *texel_face;
-    const uint32_t *face_inst;
-    const uint32_t *front_prefix;
-    uint64_t total_front;
-};
-
-// ---------------------------------------------------------------------------
-// path tracing device functions
-// ---------------------------------------------------------------------------
-struct PathCounters {
-    uint32_t extend, shadow, nodes, tris, slots;
-};
-
-template <int STACK, int BLOCK, bool COUNT>
-__device__ __forceinline__ bool shadow_visible(const DevScene &S, f3 origin, f3 dir, uint32_t *col,
-                                               PathCounters &pc) {
-    Ray r;
-    r.o = origin;
-    r.d = dir;
-    r.tmin = 0.0f;
-    r.tmax = 1000.0f;
-    ray_setup(r);
-    Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
-    TraversalStats st{0, 0};
-    bool occluded = traverse<STACK, BLOCK, true, COUNT>(S, r, h, col, st);
-    if (COUNT) {
-        pc.shadow++;
-        pc.nodes += st.nodes;
-        pc.tris += st.tris;
-    }
-    return !occluded;
-}
-
-// trace_radiance (texture_space_path_tracer.slang:57-84)
-template <int STACK, int BLOCK, bool COUNT>
-__device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_bounces, Rng &rng, uint32_t *col,
-                             PathCounters &pc) {
-    const Light &L = S.light;
-    f3 T = mk(1.0f, 1.0f, 1.0f), rad = mk(0.0f, 0.0f, 0.0f);
-    for (uint32_t bounce = 0; bounce < max_bounces; ++bounce) {
-        Ray r;
-        r.o = origin;
-        r.d = dir;
-        r.tmin = 0.0f;
-        r.tmax = 1000.0f;
-        ray_setup(r);
-        Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
-        TraversalStats st{0, 0};
-        bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
-        if (COUNT) {
-            pc.extend++;
-            pc.nodes += st.nodes;
-            pc.tris += st.tris;
-        }
-        if (!hit) return rad + T * L.sky;
-        uint2 ids = S.tri_ids[h.id];
-        Shading sd;
-        shading_data(S, ids.x, ids.y, h.b1, h.b2, -dir, sd);
-        rad = rad + T * sd.emissive;
-        float ux = rng.next_1d(), uy = rng.next_1d();
-        f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
-        float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
-        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc)) {
-            f3 f = sd.diffuse * kInvPi;
-            f3 li = (f * L.sun_illum) * ndl;
-            rad = rad + T * li;
-        }
-        float ex = rng.next_1d(), ey = rng.next_1d();
-        f3 sdir;
-        float pdf;
-        cosine_sample(ex, ey, sdir, pdf);
-        if (pdf <= 1e-6f) return rad;
-        f3 f = sd.diffuse * kInvPi;
-        float az = fabsf(sdir.z);
-        T = T * mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf);
-        origin = sd.origin_front;
-        dir = to_global(sd.frame, sdir);
-    }
-    return rad;
-}
-
-// estimate_surface_irradiance (texture_space_path_tracer.slang:86-117)
-template <int STACK, int BLOCK, bool COUNT>
-__device__ f3 estimate_irradiance(const DevScene &S, uint32_t instance, uint32_t prim, float b1, float b2, f3 view,
-                                  uint32_t spp, uint32_t max_bounces, Rng &rng, uint32_t *col, PathCounters &pc) {
-    const Light &L = S.light;
-    Shading sd;
-    shading_data(S, instance, prim, b1, b2, view, sd);
-    f3 acc = mk(0.0f, 0.0f, 0.0f);
-    for (uint32_t k = 0; k < spp; ++k) {
-        f3 sample = mk(0.0f, 0.0f, 0.0f);
-        float ux = rng.next_1d(), uy = rng.next_1d();
-        f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
-        float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
-        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
-            sample = L.sun_illum * ndl;
-        float ex = rng.next_1d(), ey = rng.next_1d();
-        f3 sdir;
-        float pdf;
-        cosine_sample(ex, ey, sdir, pdf);
-        if (pdf > 1e-6f) {
-            f3 rad = trace_radiance<STACK, BLOCK, COUNT>(S, sd.origin_front, to_global(sd.frame, sdir), max_bounces,
-                                                         rng, col, pc);
-            float az = fabsf(sdir.z);
-            sample = sample + mk(rad.x * az / pdf, rad.y * az / pdf, rad.z * az / pdf);
-        }
-        acc = acc + sample;
-    }
-    float fs = (float)spp;
-    return mk(acc.x / fs, acc.y / fs, acc.z / fs);
-}
-
-// store_accumulated_irradiance (texture_space_path_tracer.slang:119-138)
-__device__ __forceinline__ void store_accumulated(float4 *payload, uint32_t slot, f3 e, uint32_t reset) {
-    float4 p = payload[slot];
-    uint32_t prev_count = reset ? 0u : __float_as_uint(p.w);
-    f3 prev = prev_count == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(p.x, p.y, p.z);
-    uint32_t next = prev_count + 1 < kMaxTexelSamples ? prev_count + 1 : kMaxTexelSamples;
-    float w = 1.0f / (float)(prev_count + 1);
-    float4 o;
-    o.x = prev.x + w * (e.x - prev.x);
-    o.y = prev.y + w * (e.y - prev.y);
-    o.z = prev.z + w * (e.z - prev.z);
-    o.w = __uint_as_float(next);
-    payload[slot] = o;
-}
-
-struct TexelSurface {
-    uint32_t instance, prim, global_texel, local;
-    float b1, b2;
-    uint2 side;
-};
-
-// compute_main prologue (texture_space_path_tracer.slang:142-199) for flat
-// front texel w; leaves rng after the jitter draws.
-__device__ __forceinline__ TexelSurface texel_surface(const DevLayout &Ly, uint64_t w, uint32_t iteration,
-                                                      uint32_t spp, Rng &rng) {
-    TexelSurface ts;
-    uint32_t f = Ly.texel_face[w];
-    uint32_t inst = Ly.face_inst[f];
-    uint4 ii = Ly.instances[inst];
-    uint2 face = Ly.faces[f];
-    ts.side = Ly.sides[inst];
-    ts.instance = inst;
-    ts.prim = f - ii.x;
-    ts.local = (uint32_t)(w - Ly.front_prefix[inst]);
-    ts.global_texel = ii.z + ts.local;
-    uint32_t i, j;
-    address_to_ij(ts.local - face.x, face.y, i, j);
-    float fr = (float)face.y;
-    float base_x = (float)i / fr, base_y = (float)j / fr;
-    rng.seed(ts.global_texel, inst, iteration * (spp > 1u ? spp : 1u));
-    float bx = base_x, by = base_y;
-    for (int attempt = 0; attempt < 32; ++attempt) {
-        float ux = rng.next_1d();
-        float uy = rng.next_1d();
-        float cx = base_x + (ux - 0.5f) / fr;
-        float cy = base_y + (uy - 0.5f) / fr;
-        if (cx < 0.0f || cy < 0.0f || cx + cy > 1.0f) continue;
-        f3 cb = mk(cx, cy, 1.0f - cx - cy);
-        if (nearest_texel(cb, ii.z, face) == ts.global_texel) {
-            bx = cx;
-            by = cy;
-            break;
-        }
-    }
-    ts.b1 = by;
-    ts.b2 = fmaxf(0.0f, 1.0f - bx - by);
-    return ts;
-}
-
-__device__ __forceinline__ void flush_counters(const PathCounters &pc, unsigned long long *counters) {
-    __shared__ unsigned int block_sum[5];
-    if (threadIdx.x < 5) block_sum[threadIdx.x] = 0;
-    __syncthreads();
-    atomicAdd(&block_sum[0], pc.extend);
-    atomicAdd(&block_sum[1], pc.shadow);
-    atomicAdd(&block_sum[2], pc.nodes);
-    atomicAdd(&block_sum[3], pc.tris);
-    atomicAdd(&block_sum[4], pc.slots);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(&counters[NAN_COUNTER_EXTEND_RAYS], (unsigned long long)block_sum[0]);
-        atomicAdd(&counters[NAN_COUNTER_SHADOW_RAYS], (unsigned long long)block_sum[1]);
-        atomicAdd(&counters[NAN_COUNTER_NODE_VISITS], (unsigned long long)block_sum[2]);
-        atomicAdd(&counters[NAN_COUNTER_TRI_TESTS], (unsigned long long)block_sum[3]);
-        atomicAdd(&counters[NAN_COUNTER_SLOT_UPDATES], (unsigned long long)block_sum[4]);
-    }
-}
+using namespace nanx;
 
 template <int STACK, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void tst_megakernel(DevScene S, DevLayout Ly, float4 *payload,
@@ -298,7 +33,7 @@ __global__ __launch_bounds__(BLOCK) void tst_megakernel(DevScene S, DevLayout Ly
                                                         uint32_t reset, unsigned long long *counters) {
     __shared__ uint32_t stack[STACK * BLOCK];
     uint32_t *col = stack + threadIdx.x;
-    PathCounters pc{0, 0, 0, 0, 0};
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
     uint64_t w = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (w < Ly.total_front) {
         Rng rng;
@@ -560,19 +295,7 @@ __global__ void tonemap_kernel(const float4 *in, float4 *out, uint64_t n, float 
 // ---------------------------------------------------------------------------
 namespace {
 
-DevLayout dev_layout(const NanTracer *t) {
-    DevLayout L;
-    L.faces = t->faces;
-    L.instances = t->instances;
-    L.sides = t->sides;
-    L.texel_face = t->texel_face;
-    L.face_inst = t->face_inst;
-    L.front_prefix = t->front_prefix;
-    L.total_front = t->total_front;
-    return L;
-}
 
-inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
 
 Light make_light(const float *L) {
     Light l;
@@ -619,11 +342,6 @@ void launch_resolve(const NanTracer *t, const void *cache, int frozen, const Dev
                            dim3(kTraceBlock), 0, stream, t->scene->dev, L, cache, cam, out, w, h);
 }
 
-int check_launch(const char *what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(nullptr, 0, "%s launch failed: %s", what, hipGetErrorString(e));
-    return 0;
-}
 
 // Kernel stack depths compiled in; a scene whose worst-case traversal needs
 // more than the largest is rejected at creation.
@@ -859,6 +577,12 @@ NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint3
     rc |= upload(&t->texel_face, texel_face.data(), texel_face.size() * 4, err, cap);
     rc |= upload(&t->face_inst, face_inst.data(), face_inst.size() * 4, err, cap);
     rc |= upload(&t->front_prefix, prefix.data(), prefix.size() * 4, err, cap);
+    std::vector<uint32_t> ds;
+    for (uint32_t i = 0; i < instance_count; ++i)
+        if (side_table[2 * (size_t)i + 1] & 1u)
+            for (uint32_t k = 0; k < instance_table[4 * (size_t)i + 3]; ++k) ds.push_back(prefix[i] + k);
+    t->ds_count = (uint32_t)ds.size();
+    rc |= upload(&t->ds_list, ds.data(), ds.size() * 4, err, cap);
     if (rc) {
         delete t;
         return -1;
@@ -883,11 +607,50 @@ static int tst_execute(NanTracer *t, void *payload, uint32_t iteration, uint32_t
     if (!t || !payload) return fail(nullptr, 0, "nan_tst_execute: null argument");
     if (t->total_front == 0) return 0;
     unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
+    if (t->mode == NAN_TRACER_WAVEFRONT)
+        return wavefront_execute(t, (float4 *)payload, iteration, spp, bounces, reset, c, as_stream(stream));
+    StageTimer tm(t, as_stream(stream), kStageMegakernel);
     if (t->scene->stack_depth <= 32)
         launch_megakernel<32>(t, (float4 *)payload, iteration, spp, bounces, reset, c, as_stream(stream));
     else
         launch_megakernel<64>(t, (float4 *)payload, iteration, spp, bounces, reset, c, as_stream(stream));
     return check_launch("tst_megakernel");
+}
+
+NAN_API int nan_tracer_profile(NanTracer *tracer, int enable) {
+    if (!tracer) return fail(nullptr, 0, "nan_tracer_profile: null tracer");
+    tracer->profile = enable != 0;
+    tracer->stage_events.clear();
+    tracer->events_used = 0;
+    return 0;
+}
+
+NAN_API int nan_tracer_stage_times(NanTracer *tracer, double *ms, uint32_t *launches) {
+    if (!tracer || !ms) return fail(nullptr, 0, "nan_tracer_stage_times: null argument");
+    for (int k = 0; k < kStageCount; ++k) {
+        ms[k] = 0.0;
+        if (launches) launches[k] = 0;
+    }
+    for (auto &se : tracer->stage_events) {
+        HIP_TRY(hipEventSynchronize(se.second.second), nullptr, 0);
+        float dt = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&dt, se.second.first, se.second.second), nullptr, 0);
+        ms[se.first] += dt;
+        if (launches) launches[se.first]++;
+    }
+    tracer->stage_events.clear();
+    tracer->events_used = 0;
+    return 0;
+}
+
+NAN_API int nan_tracer_set_mode(NanTracer *tracer, int mode) {
+    if (!tracer) return fail(nullptr, 0, "nan_tracer_set_mode: null tracer");
+    if (mode != NAN_TRACER_MEGAKERNEL && mode != NAN_TRACER_WAVEFRONT)
+        return fail(nullptr, 0, "nan_tracer_set_mode: unknown mode %d", mode);
+    if (mode == NAN_TRACER_MEGAKERNEL && tracer->scene->stack_depth == 0)
+        return fail(nullptr, 0, "nan_tracer_set_mode: BVH too deep for the megakernel");
+    tracer->mode = mode;
+    return 0;
 }
 
 NAN_API int nan_tst_execute(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,

@@ -1,0 +1,194 @@
+// Device-side per-texel path functions shared by the megakernel and the
+// wavefront pipeline (texture_space_path_tracer.slang:57-216).
+#pragma once
+
+#include "nan_internal.h"
+
+// ---------------------------------------------------------------------------
+// path tracing device functions
+// ---------------------------------------------------------------------------
+struct PathCounters {
+    uint32_t extend, shadow, nodes, tris, slots, shadow_nodes, shadow_tris;
+};
+
+template <int STACK, int BLOCK, bool COUNT>
+__device__ __forceinline__ bool shadow_visible(const DevScene &S, f3 origin, f3 dir, uint32_t *col,
+                                               PathCounters &pc) {
+    Ray r;
+    r.o = origin;
+    r.d = dir;
+    r.tmin = 0.0f;
+    r.tmax = 1000.0f;
+    ray_setup(r);
+    Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+    TraversalStats st{0, 0};
+    bool occluded = traverse<STACK, BLOCK, true, COUNT>(S, r, h, col, st);
+    if (COUNT) {
+        pc.shadow++;
+        pc.shadow_nodes += st.nodes;
+        pc.shadow_tris += st.tris;
+    }
+    return !occluded;
+}
+
+// trace_radiance (texture_space_path_tracer.slang:57-84)
+template <int STACK, int BLOCK, bool COUNT>
+__device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_bounces, Rng &rng, uint32_t *col,
+                             PathCounters &pc) {
+    const Light &L = S.light;
+    f3 T = mk(1.0f, 1.0f, 1.0f), rad = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t bounce = 0; bounce < max_bounces; ++bounce) {
+        Ray r;
+        r.o = origin;
+        r.d = dir;
+        r.tmin = 0.0f;
+        r.tmax = 1000.0f;
+        ray_setup(r);
+        Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+        TraversalStats st{0, 0};
+        bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
+        if (COUNT) {
+            pc.extend++;
+            pc.nodes += st.nodes;
+            pc.tris += st.tris;
+        }
+        if (!hit) return rad + T * L.sky;
+        uint2 ids = S.tri_ids[h.id];
+        Shading sd;
+        shading_data(S, ids.x, ids.y, h.b1, h.b2, -dir, sd);
+        rad = rad + T * sd.emissive;
+        float ux = rng.next_1d(), uy = rng.next_1d();
+        f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+        float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc)) {
+            f3 f = sd.diffuse * kInvPi;
+            f3 li = (f * L.sun_illum) * ndl;
+            rad = rad + T * li;
+        }
+        float ex = rng.next_1d(), ey = rng.next_1d();
+        f3 sdir;
+        float pdf;
+        cosine_sample(ex, ey, sdir, pdf);
+        if (pdf <= 1e-6f) return rad;
+        f3 f = sd.diffuse * kInvPi;
+        float az = fabsf(sdir.z);
+        T = T * mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf);
+        origin = sd.origin_front;
+        dir = to_global(sd.frame, sdir);
+    }
+    return rad;
+}
+
+// estimate_surface_irradiance (texture_space_path_tracer.slang:86-117)
+template <int STACK, int BLOCK, bool COUNT>
+__device__ f3 estimate_irradiance(const DevScene &S, uint32_t instance, uint32_t prim, float b1, float b2, f3 view,
+                                  uint32_t spp, uint32_t max_bounces, Rng &rng, uint32_t *col, PathCounters &pc) {
+    const Light &L = S.light;
+    Shading sd;
+    shading_data(S, instance, prim, b1, b2, view, sd);
+    f3 acc = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = 0; k < spp; ++k) {
+        f3 sample = mk(0.0f, 0.0f, 0.0f);
+        float ux = rng.next_1d(), uy = rng.next_1d();
+        f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+        float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+            sample = L.sun_illum * ndl;
+        float ex = rng.next_1d(), ey = rng.next_1d();
+        f3 sdir;
+        float pdf;
+        cosine_sample(ex, ey, sdir, pdf);
+        if (pdf > 1e-6f) {
+            f3 rad = trace_radiance<STACK, BLOCK, COUNT>(S, sd.origin_front, to_global(sd.frame, sdir), max_bounces,
+                                                         rng, col, pc);
+            float az = fabsf(sdir.z);
+            sample = sample + mk(rad.x * az / pdf, rad.y * az / pdf, rad.z * az / pdf);
+        }
+        acc = acc + sample;
+    }
+    float fs = (float)spp;
+    return mk(acc.x / fs, acc.y / fs, acc.z / fs);
+}
+
+// store_accumulated_irradiance (texture_space_path_tracer.slang:119-138)
+__device__ __forceinline__ void store_accumulated(float4 *payload, uint32_t slot, f3 e, uint32_t reset) {
+    float4 p = payload[slot];
+    uint32_t prev_count = reset ? 0u : __float_as_uint(p.w);
+    f3 prev = prev_count == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(p.x, p.y, p.z);
+    uint32_t next = prev_count + 1 < kMaxTexelSamples ? prev_count + 1 : kMaxTexelSamples;
+    float w = 1.0f / (float)(prev_count + 1);
+    float4 o;
+    o.x = prev.x + w * (e.x - prev.x);
+    o.y = prev.y + w * (e.y - prev.y);
+    o.z = prev.z + w * (e.z - prev.z);
+    o.w = __uint_as_float(next);
+    payload[slot] = o;
+}
+
+struct TexelSurface {
+    uint32_t instance, prim, global_texel, local;
+    float b1, b2;
+    uint2 side;
+};
+
+// compute_main prologue (texture_space_path_tracer.slang:142-199) for flat
+// front texel w; leaves rng after the jitter draws.
+__device__ __forceinline__ TexelSurface texel_surface(const DevLayout &Ly, uint64_t w, uint32_t iteration,
+                                                      uint32_t spp, Rng &rng) {
+    TexelSurface ts;
+    uint32_t f = Ly.texel_face[w];
+    uint32_t inst = Ly.face_inst[f];
+    uint4 ii = Ly.instances[inst];
+    uint2 face = Ly.faces[f];
+    ts.side = Ly.sides[inst];
+    ts.instance = inst;
+    ts.prim = f - ii.x;
+    ts.local = (uint32_t)(w - Ly.front_prefix[inst]);
+    ts.global_texel = ii.z + ts.local;
+    uint32_t i, j;
+    address_to_ij(ts.local - face.x, face.y, i, j);
+    float fr = (float)face.y;
+    float base_x = (float)i / fr, base_y = (float)j / fr;
+    rng.seed(ts.global_texel, inst, iteration * (spp > 1u ? spp : 1u));
+    float bx = base_x, by = base_y;
+    for (int attempt = 0; attempt < 32; ++attempt) {
+        float ux = rng.next_1d();
+        float uy = rng.next_1d();
+        float cx = base_x + (ux - 0.5f) / fr;
+        float cy = base_y + (uy - 0.5f) / fr;
+        if (cx < 0.0f || cy < 0.0f || cx + cy > 1.0f) continue;
+        f3 cb = mk(cx, cy, 1.0f - cx - cy);
+        if (nearest_texel(cb, ii.z, face) == ts.global_texel) {
+            bx = cx;
+            by = cy;
+            break;
+        }
+    }
+    ts.b1 = by;
+    ts.b2 = fmaxf(0.0f, 1.0f - bx - by);
+    return ts;
+}
+
+__device__ __forceinline__ void flush_counters(const PathCounters &pc, unsigned long long *counters) {
+    __shared__ unsigned int block_sum[7];
+    if (threadIdx.x < 7) block_sum[threadIdx.x] = 0;
+    __syncthreads();
+    atomicAdd(&block_sum[0], pc.extend);
+    atomicAdd(&block_sum[1], pc.shadow);
+    atomicAdd(&block_sum[2], pc.nodes);
+    atomicAdd(&block_sum[3], pc.tris);
+    atomicAdd(&block_sum[4], pc.slots);
+    atomicAdd(&block_sum[5], pc.shadow_nodes);
+    atomicAdd(&block_sum[6], pc.shadow_tris);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&counters[NAN_COUNTER_EXTEND_RAYS], (unsigned long long)block_sum[0]);
+        atomicAdd(&counters[NAN_COUNTER_SHADOW_RAYS], (unsigned long long)block_sum[1]);
+        atomicAdd(&counters[NAN_COUNTER_NODE_VISITS], (unsigned long long)block_sum[2]);
+        atomicAdd(&counters[NAN_COUNTER_TRI_TESTS], (unsigned long long)block_sum[3]);
+        atomicAdd(&counters[NAN_COUNTER_SLOT_UPDATES], (unsigned long long)block_sum[4]);
+        atomicAdd(&counters[NAN_COUNTER_SHADOW_NODE_VISITS], (unsigned long long)block_sum[5]);
+        atomicAdd(&counters[NAN_COUNTER_SHADOW_TRI_TESTS], (unsigned long long)block_sum[6]);
+    }
+}
+

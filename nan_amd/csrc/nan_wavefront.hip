@@ -1,0 +1,611 @@
+// Wavefront texture-space path tracer for gfx950.
+//
+// The per-texel work of texture_space_path_tracer.slang:140-216 split into
+// stages so that the traversal kernels run alone, with few registers and
+// many waves per SIMD:
+//
+//   generate  ray-gen over the Mesh Colors lattice, surface shading, the
+//             surface sun shadow ray and the first bounce ray
+//   extend    closest hit for the live bounce rays          (traversal)
+//   shade     emission, sun next-event ray, Lambert bounce, compaction of
+//             the surviving paths into the next ray queue by wave ballot
+//   shadow    any-hit for the queued sun rays             (traversal)
+//   finish    running-mean store into the front/back 16-byte slots
+//
+// repeated `max_bounces` times per side.  Back sides of double-sided
+// instances run after every front side has finished, continuing each
+// texel's RNG where its front path left it, so the draw order and every
+// floating-point operation equal the single-kernel form (and the CPU
+// oracle): the payload is bit-identical.
+//
+// Traversal kernels are persistent: a fixed grid sized from the occupancy
+// calculator, each wave fetching 64 rays at a time from an atomic counter.
+// Their stack is a 16-entry ring per lane in LDS that spills its oldest
+// entries to a per-lane column in HBM when a traversal goes deeper.
+#include "nan_tracer.h"
+
+using namespace nanx;
+
+namespace {
+
+constexpr int kWfBlock = 256;       // stage kernels
+constexpr int kTravBlock = 256;     // traversal kernels
+constexpr int kLdsStack = 16;       // ring entries per lane in LDS
+constexpr int kMaxStack = 128;      // deepest traversal the spill area holds
+constexpr uint32_t kTargetSun = 0u;
+constexpr uint32_t kTargetRad = 1u;
+constexpr uint32_t kCounterSlots = 1024;
+
+}  // namespace
+
+struct WfBuffers {
+    uint32_t capacity = 0;      // paths (front texels)
+    uint32_t *rng = nullptr;    // [cap]
+    float4 *T = nullptr;        // [cap] throughput.xyz, |cos| of the first bounce
+    float4 *rad = nullptr;      // [cap] path radiance.xyz, pdf of the first bounce
+    float4 *sun = nullptr;      // [cap] surface sun irradiance of the current sample
+    float4 *acc = nullptr;      // [cap] sum of samples over spp (spp > 1 only)
+    float4 *ray_o[2] = {nullptr, nullptr};  // [cap] origin.xyz, path id bits
+    float4 *ray_d[2] = {nullptr, nullptr};  // [cap] direction.xyz
+    float4 *hits = nullptr;     // [cap] t, b1, b2, triangle id bits
+    float4 *sh_o = nullptr;     // [2 cap] origin.xyz, path id bits
+    float4 *sh_d = nullptr;     // [2 cap] direction.xyz, target bits
+    float4 *sh_c = nullptr;     // [2 cap] contribution.xyz
+    uint32_t *counters = nullptr;   // [kCounterSlots]
+    uint32_t *spill = nullptr;      // [(kMaxStack - kLdsStack) * spill_lanes]
+    uint32_t spill_lanes = 0;
+    uint32_t trav_blocks = 0;       // persistent grid of the traversal kernels
+    uint32_t stage_blocks = 0;      // grid of the stage kernels
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Appends `want` lanes of the wave to a queue with one atomic per wave;
+// returns the lane's slot (valid when want).
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
+    unsigned long long mask = __ballot(want);
+    uint32_t n = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (n) {
+        int leader = __ffsll((long long)mask) - 1;
+        if ((int)lane_id() == leader) base = atomicAdd(count, n);
+        base = __shfl(base, leader);
+    }
+    unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+// Traversal with a kLdsStack-entry LDS ring per lane that spills to HBM.
+// Identical visiting rule and result to traverse() in nan_device.h.
+template <bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ bool traverse_ring(const DevScene &S, const Ray &r, Hit &hit, uint32_t *lds_col,
+                                              uint32_t *spill_col, uint32_t spill_stride, TraversalStats &st) {
+    uint32_t cur = S.root;
+    int sp = 0;
+    auto push = [&](uint32_t e) {
+        uint32_t *slot = lds_col + (sp & (kLdsStack - 1)) * kTravBlock;
+        if (sp >= kLdsStack) spill_col[(size_t)(sp - kLdsStack) * spill_stride] = *slot;
+        *slot = e;
+        ++sp;
+    };
+    auto pop = [&]() -> uint32_t {
+        --sp;
+        uint32_t *slot = lds_col + (sp & (kLdsStack - 1)) * kTravBlock;
+        uint32_t e = *slot;
+        if (sp >= kLdsStack) *slot = spill_col[(size_t)(sp - kLdsStack) * spill_stride];
+        return e;
+    };
+    while (true) {
+        if (!(cur & kLeafBit)) {
+            const uint4 *np = S.nodes + 4 * (size_t)cur;
+            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+            if (COUNT) st.nodes++;
+            f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
+            float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
+            float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
+            float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
+            uint32_t valid = w0.w >> 24;
+            uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
+            uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
+            float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack;
+            float tn[4];
+            uint32_t ent[4];
+            int nhit = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float lox = origin.x + (float)((qlx >> (8 * c)) & 0xFFu) * sx;
+                float loy = origin.y + (float)((qly >> (8 * c)) & 0xFFu) * sy;
+                float loz = origin.z + (float)((qlz >> (8 * c)) & 0xFFu) * sz;
+                float hix = origin.x + (float)((qhx >> (8 * c)) & 0xFFu) * sx;
+                float hiy = origin.y + (float)((qhy >> (8 * c)) & 0xFFu) * sy;
+                float hiz = origin.z + (float)((qhz >> (8 * c)) & 0xFFu) * sz;
+                float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
+                float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
+                float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
+                float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+                float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + S.slack;
+                far = fminf(far, limit);
+                bool ok = ((valid >> c) & 1u) && near <= far;
+                if (ok) {
+                    float key = near;
+                    uint32_t e = child[c];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (k < nhit && key < tn[k]) {
+                            float tk = tn[k];
+                            uint32_t ek = ent[k];
+                            tn[k] = key;
+                            ent[k] = e;
+                            key = tk;
+                            e = ek;
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (k == nhit) {
+                            tn[k] = key;
+                            ent[k] = e;
+                        }
+                    nhit++;
+                }
+            }
+            if (nhit == 0) {
+                if (sp == 0) break;
+                cur = pop();
+                continue;
+            }
+#pragma unroll
+            for (int k = 3; k >= 1; --k)
+                if (k < nhit) push(ent[k]);
+            cur = ent[0];
+            continue;
+        }
+        uint32_t first = cur & kLeafFirstMask;
+        uint32_t count = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+        for (uint32_t i = 0; i < count; ++i) {
+            const float4 *tp = S.tris + 3 * (size_t)(first + i);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            if (COUNT) st.tris++;
+            float t, b1, b2;
+            if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
+                uint32_t id = __float_as_uint(c.y);
+                if (ANY_HIT) {
+                    hit.t = t;
+                    hit.id = id;
+                    return true;
+                }
+                if (t < hit.t || (t == hit.t && id < hit.id)) {
+                    hit.t = t;
+                    hit.b1 = b1;
+                    hit.b2 = b2;
+                    hit.id = id;
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = pop();
+    }
+    return hit.id != kInvalid;
+}
+
+__device__ __forceinline__ void add_counter(unsigned long long *counters, int slot, uint32_t v) {
+    // wave-reduce, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (lane_id() == 0 && v) atomicAdd(&counters[slot], (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void wf_zero_counters(uint32_t *c, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0u;
+}
+
+struct GenParams {
+    const uint32_t *list;  // null: path i is front texel i
+    uint32_t count;
+    uint32_t side;         // 0 front, 1 back
+    uint32_t sample;       // 0 .. spp-1
+    uint32_t iteration, spp;
+    uint32_t ext_slot, sh_slot;  // counter slots of the queues written
+};
+
+__global__ __launch_bounds__(kWfBlock) void wf_generate(DevScene S, DevLayout Ly, WfBuffers W, GenParams P) {
+    const Light &L = S.light;
+    uint32_t stride = gridDim.x * kWfBlock;
+    uint32_t *ext_count = W.counters + P.ext_slot;
+    uint32_t *sh_count = W.counters + P.sh_slot;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < P.count; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < P.count;
+        bool want_shadow = false, want_ray = false;
+        f3 origin = mk(0.0f, 0.0f, 0.0f), ld = origin, dir = origin, contrib = origin;
+        uint32_t w = 0;
+        if (active) {
+            w = P.list ? P.list[i] : i;
+            Rng rng;
+            TexelSurface ts = texel_surface(Ly, w, P.iteration, P.spp, rng);
+            if (P.side != 0 || P.sample != 0) rng.state = W.rng[w];
+            f3 n = world_flat_normal(S, ts.instance, ts.prim);
+            Shading sd;
+            shading_data(S, ts.instance, ts.prim, ts.b1, ts.b2, P.side ? -n : n, sd);
+            float ux = rng.next_1d(), uy = rng.next_1d();
+            ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+            float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+            want_shadow = L.sun_on && ndl > 0.0f;
+            contrib = L.sun_illum * ndl;
+            float ex = rng.next_1d(), ey = rng.next_1d();
+            f3 sdir;
+            float pdf;
+            cosine_sample(ex, ey, sdir, pdf);
+            float az = fabsf(sdir.z);
+            want_ray = pdf > 1e-6f;
+            dir = to_global(sd.frame, sdir);
+            origin = sd.origin_front;
+            W.T[w] = make_float4(1.0f, 1.0f, 1.0f, az);
+            W.rad[w] = make_float4(0.0f, 0.0f, 0.0f, pdf);
+            W.sun[w] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            W.rng[w] = rng.state;
+        }
+        uint32_t qs = wave_append(active && want_shadow, sh_count);
+        if (active && want_shadow) {
+            W.sh_o[qs] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
+            W.sh_d[qs] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(kTargetSun));
+            W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+        }
+        uint32_t qr = wave_append(active && want_ray, ext_count);
+        if (active && want_ray) {
+            W.ray_o[0][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
+            W.ray_d[0][qr] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+        }
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTravBlock) void wf_extend(DevScene S, WfBuffers W, uint32_t q, uint32_t count_slot,
+                                                       uint32_t fetch_slot, unsigned long long *counters) {
+    __shared__ uint32_t stack[kLdsStack * kTravBlock];
+    uint32_t *col = stack + threadIdx.x;
+    uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
+    uint32_t *spill_col = W.spill + gid;
+    const uint32_t count = W.counters[count_slot];
+    uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
+    while (true) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&W.counters[fetch_slot], 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        uint32_t k = base + lane_id();
+        if (k < count) {
+            float4 o = W.ray_o[q][k], d = W.ray_d[q][k];
+            Ray r;
+            r.o = mk(o.x, o.y, o.z);
+            r.d = mk(d.x, d.y, d.z);
+            r.tmin = 0.0f;
+            r.tmax = 1000.0f;
+            ray_setup(r);
+            Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+            TraversalStats st{0, 0};
+            traverse_ring<false, COUNT>(S, r, h, col, spill_col, W.spill_lanes, st);
+            W.hits[k] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.id));
+            if (COUNT) {
+                n_rays++;
+                n_nodes += st.nodes;
+                n_tris += st.tris;
+            }
+        }
+    }
+    if (COUNT) {
+        add_counter(counters, NAN_COUNTER_EXTEND_RAYS, n_rays);
+        add_counter(counters, NAN_COUNTER_NODE_VISITS, n_nodes);
+        add_counter(counters, NAN_COUNTER_TRI_TESTS, n_tris);
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTravBlock) void wf_shadow(DevScene S, WfBuffers W, uint32_t count_slot,
+                                                       uint32_t fetch_slot, unsigned long long *counters) {
+    __shared__ uint32_t stack[kLdsStack * kTravBlock];
+    uint32_t *col = stack + threadIdx.x;
+    uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
+    uint32_t *spill_col = W.spill + gid;
+    const uint32_t count = W.counters[count_slot];
+    uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
+    while (true) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&W.counters[fetch_slot], 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        uint32_t k = base + lane_id();
+        if (k < count) {
+            float4 o = W.sh_o[k], d = W.sh_d[k];
+            Ray r;
+            r.o = mk(o.x, o.y, o.z);
+            r.d = mk(d.x, d.y, d.z);
+            r.tmin = 0.0f;
+            r.tmax = 1000.0f;
+            ray_setup(r);
+            Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+            TraversalStats st{0, 0};
+            bool occluded = traverse_ring<true, COUNT>(S, r, h, col, spill_col, W.spill_lanes, st);
+            if (!occluded) {
+                uint32_t p = __float_as_uint(o.w);
+                float4 c = W.sh_c[k];
+                if (__float_as_uint(d.w) == kTargetSun) {
+                    W.sun[p] = make_float4(c.x, c.y, c.z, 0.0f);
+                } else {
+                    float4 v = W.rad[p];
+                    v.x = v.x + c.x;
+                    v.y = v.y + c.y;
+                    v.z = v.z + c.z;
+                    W.rad[p] = v;
+                }
+            }
+            if (COUNT) {
+                n_rays++;
+                n_nodes += st.nodes;
+                n_tris += st.tris;
+            }
+        }
+    }
+    if (COUNT) {
+        add_counter(counters, NAN_COUNTER_SHADOW_RAYS, n_rays);
+        add_counter(counters, NAN_COUNTER_SHADOW_NODE_VISITS, n_nodes);
+        add_counter(counters, NAN_COUNTER_SHADOW_TRI_TESTS, n_tris);
+    }
+}
+
+struct ShadeParams {
+    uint32_t q_in, q_out;
+    uint32_t in_slot, out_slot, sh_slot;
+    uint32_t bounce, max_bounces;
+};
+
+// one iteration of trace_radiance's loop body (texture_space_path_tracer.slang:60-83)
+__global__ __launch_bounds__(kWfBlock) void wf_shade(DevScene S, WfBuffers W, ShadeParams P) {
+    const Light &L = S.light;
+    const uint32_t count = W.counters[P.in_slot];
+    uint32_t stride = gridDim.x * kWfBlock;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += stride) {
+        uint32_t k = base + threadIdx.x;
+        bool active = k < count;
+        bool want_shadow = false, want_ray = false;
+        f3 origin = mk(0.0f, 0.0f, 0.0f), ld = origin, dir = origin, contrib = origin;
+        uint32_t p = 0;
+        if (active) {
+            float4 ro = W.ray_o[P.q_in][k], rd = W.ray_d[P.q_in][k];
+            float4 hv = W.hits[k];
+            p = __float_as_uint(ro.w);
+            float4 Tv = W.T[p], Rv = W.rad[p];
+            f3 T = mk(Tv.x, Tv.y, Tv.z), rad = mk(Rv.x, Rv.y, Rv.z);
+            uint32_t id = __float_as_uint(hv.w);
+            if (id == kInvalid) {
+                rad = rad + T * L.sky;
+            } else {
+                Rng rng;
+                rng.state = W.rng[p];
+                uint2 ids = S.tri_ids[id];
+                f3 d = mk(rd.x, rd.y, rd.z);
+                Shading sd;
+                shading_data(S, ids.x, ids.y, hv.y, hv.z, -d, sd);
+                rad = rad + T * sd.emissive;
+                float ux = rng.next_1d(), uy = rng.next_1d();
+                ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+                float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+                if (L.sun_on && ndl > 0.0f) {
+                    f3 f = sd.diffuse * kInvPi;
+                    f3 li = (f * L.sun_illum) * ndl;
+                    contrib = T * li;
+                    want_shadow = true;
+                }
+                float ex = rng.next_1d(), ey = rng.next_1d();
+                f3 sdir;
+                float pdf;
+                cosine_sample(ex, ey, sdir, pdf);
+                if (pdf > 1e-6f) {
+                    f3 f = sd.diffuse * kInvPi;
+                    float az = fabsf(sdir.z);
+                    T = T * mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf);
+                    origin = sd.origin_front;
+                    dir = to_global(sd.frame, sdir);
+                    want_ray = P.bounce < P.max_bounces;
+                }
+                W.rng[p] = rng.state;
+                W.T[p] = make_float4(T.x, T.y, T.z, Tv.w);
+            }
+            W.rad[p] = make_float4(rad.x, rad.y, rad.z, Rv.w);
+        }
+        uint32_t qs = wave_append(active && want_shadow, W.counters + P.sh_slot);
+        if (active && want_shadow) {
+            W.sh_o[qs] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(p));
+            W.sh_d[qs] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(kTargetRad));
+            W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+        }
+        uint32_t qr = wave_append(active && want_ray, W.counters + P.out_slot);
+        if (active && want_ray) {
+            W.ray_o[P.q_out][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(p));
+            W.ray_d[P.q_out][qr] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+        }
+    }
+}
+
+struct FinishParams {
+    const uint32_t *list;
+    uint32_t count, side, sample, spp, reset;
+};
+
+template <bool COUNT>
+__global__ __launch_bounds__(kWfBlock) void wf_finish(DevLayout Ly, WfBuffers W, float4 *payload, FinishParams P,
+                                                     unsigned long long *counters) {
+    uint32_t stride = gridDim.x * kWfBlock;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < P.count; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t slots = 0;
+        if (i < P.count) {
+            uint32_t w = P.list ? P.list[i] : i;
+            float4 sv = W.sun[w], Rv = W.rad[w], Tv = W.T[w];
+            f3 sample = mk(sv.x, sv.y, sv.z);
+            float az = Tv.w, pdf = Rv.w;
+            if (pdf > 1e-6f) sample = sample + mk(Rv.x * az / pdf, Rv.y * az / pdf, Rv.z * az / pdf);
+            f3 acc;
+            if (P.spp > 1) {
+                float4 a = P.sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : W.acc[w];
+                acc = mk(a.x, a.y, a.z) + sample;
+                W.acc[w] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+            } else {
+                acc = mk(0.0f, 0.0f, 0.0f) + sample;
+            }
+            if (P.sample + 1 == P.spp) {
+                float fs = (float)P.spp;
+                f3 e = mk(acc.x / fs, acc.y / fs, acc.z / fs);
+                uint32_t f = Ly.texel_face[w];
+                uint32_t inst = Ly.face_inst[f];
+                uint32_t local = w - Ly.front_prefix[inst];
+                uint32_t slot = P.side ? Ly.sides[inst].x + local : Ly.instances[inst].z + local;
+                store_accumulated(payload, slot, e, P.reset);
+                slots = 1;
+            }
+        }
+        if (COUNT) add_counter(counters, NAN_COUNTER_SLOT_UPDATES, slots);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace nanx {
+
+void wavefront_release(NanTracer *t) {
+    WfBuffers *W = t->wf;
+    if (!W) return;
+    hipFree(W->rng);
+    hipFree(W->T);
+    hipFree(W->rad);
+    hipFree(W->sun);
+    hipFree(W->acc);
+    hipFree(W->ray_o[0]);
+    hipFree(W->ray_o[1]);
+    hipFree(W->ray_d[0]);
+    hipFree(W->ray_d[1]);
+    hipFree(W->hits);
+    hipFree(W->sh_o);
+    hipFree(W->sh_d);
+    hipFree(W->sh_c);
+    hipFree(W->counters);
+    hipFree(W->spill);
+    delete W;
+    t->wf = nullptr;
+}
+
+static int wavefront_alloc(NanTracer *t, uint32_t spp) {
+    if (t->wf && (spp <= 1 || t->wf->acc)) return 0;
+    if (t->wf && spp > 1 && !t->wf->acc) {
+        HIP_TRY(hipMalloc((void **)&t->wf->acc, (size_t)t->wf->capacity * 16), nullptr, 0);
+        return 0;
+    }
+    WfBuffers *W = new WfBuffers();
+    t->wf = W;
+    uint32_t cap = (uint32_t)std::max<uint64_t>(t->total_front, 1);
+    W->capacity = cap;
+    size_t c4 = (size_t)cap * 16;
+    HIP_TRY(hipMalloc((void **)&W->rng, (size_t)cap * 4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->T, c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->rad, c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->sun, c4), nullptr, 0);
+    if (spp > 1) HIP_TRY(hipMalloc((void **)&W->acc, c4), nullptr, 0);
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(hipMalloc((void **)&W->ray_o[k], c4), nullptr, 0);
+        HIP_TRY(hipMalloc((void **)&W->ray_d[k], c4), nullptr, 0);
+    }
+    HIP_TRY(hipMalloc((void **)&W->hits, c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->sh_o, 2 * c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->sh_d, 2 * c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->sh_c, 2 * c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->counters, kCounterSlots * 4), nullptr, 0);
+    int dev = 0, cus = 0, per_cu_ext = 0, per_cu_sh = 0;
+    HIP_TRY(hipGetDevice(&dev), nullptr, 0);
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_ext, wf_extend<false>, kTravBlock, 0), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_sh, wf_shadow<false>, kTravBlock, 0), nullptr, 0);
+    int per_cu = std::max(1, std::max(per_cu_ext, per_cu_sh));
+    W->trav_blocks = (uint32_t)(cus * per_cu);
+    W->stage_blocks = (uint32_t)(cus * 8);
+    W->spill_lanes = W->trav_blocks * kTravBlock;
+    HIP_TRY(hipMalloc((void **)&W->spill, (size_t)(kMaxStack - kLdsStack) * W->spill_lanes * 4), nullptr, 0);
+    return 0;
+}
+
+int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
+                      uint32_t reset, unsigned long long *counters, hipStream_t stream) {
+    if (t->scene->info.max_stack > (uint32_t)kMaxStack)
+        return fail(nullptr, 0, "wavefront: BVH needs a %u-entry stack (max %d)", t->scene->info.max_stack,
+                    kMaxStack);
+    if (wavefront_alloc(t, spp)) return -1;
+    WfBuffers &W = *t->wf;
+    const DevScene &S = t->scene->dev;
+    DevLayout Ly = dev_layout(t);
+    uint32_t nspp = spp > 1 ? spp : 1;
+    uint32_t slot = 0;  // next free counter slot
+    hipLaunchKernelGGL(wf_zero_counters, dim3(1), dim3(1024), 0, stream, W.counters, kCounterSlots);
+    for (uint32_t side = 0; side < 2; ++side) {
+        uint32_t count = side ? t->ds_count : (uint32_t)t->total_front;
+        const uint32_t *list = side ? t->ds_list : nullptr;
+        if (count == 0) continue;
+        uint32_t gen_blocks = std::min<uint32_t>(W.stage_blocks, grid_for(count, kWfBlock));
+        for (uint32_t k = 0; k < nspp; ++k) {
+            if (slot + 4 * bounces + 8 > kCounterSlots) return fail(nullptr, 0, "wavefront: too many stages");
+            uint32_t ext_slot = slot++, sh_slot = slot++;
+            GenParams gp{list, count, side, k, iteration, spp, ext_slot, sh_slot};
+            {
+                StageTimer tm(t, stream, kStageGenerate);
+                hipLaunchKernelGGL(wf_generate, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, Ly, W, gp);
+            }
+            uint32_t q = 0;
+            for (uint32_t b = 1; b <= bounces; ++b) {
+                uint32_t fetch = slot++;
+                {
+                StageTimer tm(t, stream, kStageExtend);
+                if (counters)
+                    hipLaunchKernelGGL(wf_extend<true>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
+                                       ext_slot, fetch, counters);
+                else
+                    hipLaunchKernelGGL(wf_extend<false>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
+                                       ext_slot, fetch, counters);
+                }
+                uint32_t next_ext = slot++;
+                ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces};
+                {
+                    StageTimer tm(t, stream, kStageShade);
+                    hipLaunchKernelGGL(wf_shade, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
+                }
+                uint32_t sfetch = slot++;
+                {
+                StageTimer tm(t, stream, kStageShadow);
+                if (counters)
+                    hipLaunchKernelGGL(wf_shadow<true>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
+                                       sh_slot, sfetch, counters);
+                else
+                    hipLaunchKernelGGL(wf_shadow<false>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
+                                       sh_slot, sfetch, counters);
+                }
+                sh_slot = slot++;
+                ext_slot = next_ext;
+                q = 1u - q;
+            }
+            FinishParams fp{list, count, side, k, nspp, reset};
+            StageTimer tm(t, stream, kStageFinish);
+            if (counters)
+                hipLaunchKernelGGL(wf_finish<true>, dim3(gen_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
+                                   counters);
+            else
+                hipLaunchKernelGGL(wf_finish<false>, dim3(gen_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
+                                   counters);
+        }
+    }
+    return check_launch("wavefront");
+}
+
+}  // namespace nanx

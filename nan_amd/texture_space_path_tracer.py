@@ -38,8 +38,10 @@ def as_payload_tensor(buf: torch.Tensor, slots: int) -> torch.Tensor:
 class TextureSpacePathTracer:
     """Progressive per-texel irradiance tracer (texture_space_path_tracer.py:7-59)."""
 
+    MODES = {"megakernel": _native.TRACER_MEGAKERNEL, "wavefront": _native.TRACER_WAVEFRONT}
+
     def __init__(self, device: Device, scene: Scene, layout: MeshColorsLayout, *, samples_per_texel: int = 1,
-                 max_bounces: int = 3):
+                 max_bounces: int = 3, mode: str = "wavefront"):
         self.device = device
         self.scene = scene
         self.layout = layout
@@ -54,6 +56,7 @@ class TextureSpacePathTracer:
         if rc != 0:
             raise _native.NanError(err.value.decode())
         self.handle = handle
+        self.set_mode(mode)
         dev = device.torch_device
         # The reference exposes its face/instance/side/adjacency buffers for the
         # resolve and filter passes (texture_space_path_tracer.py:26-31).
@@ -61,6 +64,14 @@ class TextureSpacePathTracer:
         self.instance_infos_buffer = torch.from_numpy(layout.instance_table.view(np.int32)).to(dev)
         self.side_infos_buffer = torch.from_numpy(layout.side_table.view(np.int32)).to(dev)
         self.adjacency_infos_buffer = None
+
+    def set_mode(self, mode: str) -> None:
+        """``"wavefront"`` (staged kernels, default) or ``"megakernel"`` (one
+        thread per texel); both produce bit-identical payloads."""
+        if mode not in self.MODES:
+            raise ValueError(f"unknown tracer mode {mode!r}")
+        _native.check(_native.lib().nan_tracer_set_mode(self.handle, self.MODES[mode]), "nan_tracer_set_mode")
+        self.mode = mode
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -89,6 +100,18 @@ class TextureSpacePathTracer:
                                                        self.max_bounces, 1 if reset else 0, counters.data_ptr(),
                                                        stream)
         _native.check(rc, "nan_tst_execute")
+
+    def profile(self, enable: bool) -> None:
+        """Bracket every kernel launch with HIP events (see stage_times)."""
+        _native.check(_native.lib().nan_tracer_profile(self.handle, 1 if enable else 0), "nan_tracer_profile")
+
+    def stage_times(self) -> dict:
+        """Milliseconds and launches per stage since profiling was enabled
+        (waits for the recorded events)."""
+        ms = (ctypes.c_double * 8)()
+        n = (ctypes.c_uint32 * 8)()
+        _native.check(_native.lib().nan_tracer_stage_times(self.handle, ms, n), "nan_tracer_stage_times")
+        return {name: (ms[k], n[k]) for k, name in enumerate(_native.STAGES) if n[k]}
 
     def raygen(self, command_encoder: CommandEncoder, iteration: int) -> tuple[torch.Tensor, torch.Tensor]:
         """Texel ray generation only: int32[front, 4] (instance, primitive,

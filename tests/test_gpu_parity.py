@@ -33,11 +33,11 @@ def nan(device):
     return scene, texture_space_path_tracer, passes
 
 
-def _make(device, node, nan, **layout_kw):
+def _make(device, node, nan, mode="wavefront", spp=1, **layout_kw):
     scene_mod, tst_mod, _ = nan
     layout = MeshColorsLayout.build(node, **layout_kw)
     sc = scene_mod.Scene(device, node)
-    tracer = tst_mod.TextureSpacePathTracer(device, sc, layout)
+    tracer = tst_mod.TextureSpacePathTracer(device, sc, layout, mode=mode, samples_per_texel=spp)
     return layout, sc, tracer
 
 
@@ -166,20 +166,21 @@ def test_trace_rays_bit_exact_vs_oracle(device, nan, which):
 
 
 # ------------------------------------------------------------------ tracer
-def _oracle_payload(node, layout, iterations, bounces=3, reset_first=True):
+def _oracle_payload(node, layout, iterations, bounces=3, reset_first=True, spp=1):
     orc = oracle.OracleScene(flatten_scene(node))
     p = np.zeros((layout.total_payload_count, 4), dtype=np.float32)
     counts = np.zeros(4, dtype=np.uint64)
     for it in range(iterations):
-        counts += orc.trace_texels(layout.face_table, layout.instance_table, layout.side_table, p, it, 1, bounces,
+        counts += orc.trace_texels(layout.face_table, layout.instance_table, layout.side_table, p, it, spp, bounces,
                                    reset_first and it == 0, threads=oracle.cpu_count())
     return p, counts
 
 
-def test_tracer_cornell_bit_exact(device, nan):
-    _say("tracer cornell 3 iterations")
+@pytest.mark.parametrize("mode", ["wavefront", "megakernel"])
+def test_tracer_cornell_bit_exact(device, nan, mode):
+    _say(f"tracer cornell 3 iterations ({mode})")
     node = cornell_box()
-    layout, sc, tracer = _make(device, node, nan)
+    layout, sc, tracer = _make(device, node, nan, mode=mode)
     dev = device.torch_device
     payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=dev)
     counters = torch.zeros(8, dtype=torch.int64, device=dev)
@@ -201,25 +202,62 @@ def test_tracer_cornell_bit_exact(device, nan):
     assert c[0] == cnt[0] and c[1] == cnt[1] and c[4] == cnt[3]
 
 
+@pytest.mark.parametrize("mode", ["wavefront", "megakernel"])
 @pytest.mark.parametrize("which", ["batch", "synthetic_small"])
-def test_tracer_sunlit_scene_bit_exact(device, nan, which):
-    _say(f"tracer {which}")
+def test_tracer_sunlit_scene_bit_exact(device, nan, which, mode):
+    _say(f"tracer {which} ({mode})")
     if which == "batch":
         node = batch_scene(21, 6000)
         kw = dict(texels_per_unit=8.0, min_resolution=1, max_resolution=8, max_total_texels=2_000_000)
     else:
         node = synthetic_scene(2, terrain_quads=64, box_grid=8, spheres=20, extent=20.0)
         kw = dict(texels_per_unit=8.0, min_resolution=2, max_resolution=8, max_total_texels=4_000_000)
-    layout, sc, tracer = _make(device, node, nan, **kw)
+    layout, sc, tracer = _make(device, node, nan, mode=mode, **kw)
     dev = device.torch_device
     payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=dev)
     enc = device.create_command_encoder()
+    counters = torch.zeros(8, dtype=torch.int64, device=dev)
     for it in range(2):
-        tracer.execute(enc, payload, it, reset=(it == 0))
+        tracer.execute(enc, payload, it, reset=(it == 0), counters=counters if it == 1 else None)
     got = payload.cpu().numpy()
-    ref, _ = _oracle_payload(node, layout, 2)
-    diff = np.any(got.view(np.uint32) != ref.view(np.uint32), axis=1)
+    ref, _ = _oracle_payload(node, layout, 1)
+    p = ref.copy()
+    cnt = oracle.OracleScene(flatten_scene(node)).trace_texels(
+        layout.face_table, layout.instance_table, layout.side_table, p, 1, 1, 3, False, threads=oracle.cpu_count())
+    diff = np.any(got.view(np.uint32) != p.view(np.uint32), axis=1)
     assert diff.sum() == 0, f"{diff.sum()} of {len(diff)} slots differ"
+    c = counters.cpu().numpy()
+    assert (c[0], c[1], c[4]) == (cnt[0], cnt[1], cnt[3])
+
+
+def test_wavefront_two_samples_per_texel_bit_exact(device, nan):
+    _say("tracer spp=2 wavefront vs megakernel vs oracle")
+    node = batch_scene(22, 4000)
+    kw = dict(texels_per_unit=6.0, min_resolution=1, max_resolution=8, max_total_texels=2_000_000)
+    outs = []
+    for mode in ("wavefront", "megakernel"):
+        layout, sc, tracer = _make(device, node, nan, mode=mode, spp=2, **kw)
+        payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=device.torch_device)
+        enc = device.create_command_encoder()
+        for it in range(2):
+            tracer.execute(enc, payload, it, reset=(it == 0))
+        outs.append(payload.cpu().numpy())
+    np.testing.assert_array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    ref, _ = _oracle_payload(node, layout, 2, spp=2)
+    np.testing.assert_array_equal(outs[0].view(np.uint32), ref.view(np.uint32))
+
+
+def test_stage_timing_reports_every_stage(device, nan):
+    node = cornell_box()
+    layout, sc, tracer = _make(device, node, nan)
+    payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=device.torch_device)
+    tracer.profile(True)
+    tracer.execute(device.create_command_encoder(), payload, 0, reset=True)
+    st = tracer.stage_times()
+    tracer.profile(False)
+    # front + back phases: 2 generate, 6 extend, 6 shade, 6 shadow, 2 finish
+    assert {k: v[1] for k, v in st.items()} == {"generate": 2, "extend": 6, "shade": 6, "shadow": 6, "finish": 2}
+    assert all(v[0] > 0 for v in st.values())
 
 
 # ------------------------------------------------------------------ resolve / accumulate / tonemap
