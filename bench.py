@@ -40,9 +40,9 @@ BYTES_PER_NODE = 64
 BYTES_PER_TRI = 48
 BYTES_PER_SLOT_UPDATE = 32         # 16-byte slot read + write
 BYTES_PER_TEXEL_SETUP = 4 + 4 + 16 + 8 + 4 + 8   # texel->face, face->instance, instance, side, prefix, face
-# per shaded hit: tri id map 8 + instance 16 + mesh 16 + 3 indices 12 + 3 vertices x 32
-# + transform 48 + inverse transpose 64 + material 32
-BYTES_PER_SHADED_HIT = 8 + 16 + 16 + 12 + 96 + 48 + 64 + 32
+# per shaded hit: 80-byte triangle shading record + transform rows 48 + inverse
+# transpose 64 + material 32
+BYTES_PER_SHADED_HIT = 80 + 48 + 64 + 32
 BYTES_PER_RAY = 32                 # queued origin + direction (float4 x 2)
 BYTES_PER_HIT = 16                 # t, b1, b2, triangle id
 BYTES_PER_SHADOW_RAY = 48          # origin, direction, contribution (float4 x 3)

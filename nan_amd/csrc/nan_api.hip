@@ -39,14 +39,13 @@ __global__ __launch_bounds__(BLOCK) void tst_megakernel(DevScene S, DevLayout Ly
         Rng rng;
         TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
         uint32_t nspp = spp > 1u ? spp : 1u;
-        f3 n = world_flat_normal(S, ts.instance, ts.prim);
-        f3 e = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.instance, ts.prim, ts.b1, ts.b2, n, nspp, max_bounces,
-                                                        rng, col, pc);
+        f3 n = world_flat_normal(S, ts.tri);
+        f3 e = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.tri, ts.b1, ts.b2, n, nspp, max_bounces, rng, col, pc);
         store_accumulated(payload, ts.global_texel, e, reset);
         if (COUNT) pc.slots++;
         if (ts.side.y & 1u) {
-            f3 eb = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.instance, ts.prim, ts.b1, ts.b2, -n, nspp,
-                                                             max_bounces, rng, col, pc);
+            f3 eb = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.tri, ts.b1, ts.b2, -n, nspp, max_bounces, rng, col,
+                                                             pc);
             store_accumulated(payload, ts.side.x + ts.local, eb, reset);
             if (COUNT) pc.slots++;
         }
@@ -208,7 +207,7 @@ __global__ __launch_bounds__(BLOCK) void resolve_kernel(DevScene S, DevLayout Ly
         color = sky_with_sun_disk(S.light, d);
     } else {
         uint2 ids = S.tri_ids[h.id];
-        f3 fn = world_flat_normal(S, ids.x, ids.y);
+        f3 fn = world_flat_normal(S, h.id);
         bool back = dot(fn, -d) <= 0.0f;
         uint4 ii = Ly.instances[ids.x];
         uint2 side = Ly.sides[ids.x];
@@ -219,7 +218,7 @@ __global__ __launch_bounds__(BLOCK) void resolve_kernel(DevScene S, DevLayout Ly
             f3 bc = mk(1.0f - h.b1 - h.b2, h.b1, h.b2);
             f3 E = sample_irradiance<FROZEN>(cache, bc, ii.z, face, side, back);
             Shading sd;
-            shading_data(S, ids.x, ids.y, h.b1, h.b2, -d, sd);
+            shading_data(S, h.id, h.b1, h.b2, -d, sd);
             color = sd.emissive + (sd.diffuse * kInvPi) * E;
         }
     }
@@ -411,6 +410,34 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
             }
         }
     }
+    // per-triangle shading records (load_triangle in nan_device.h)
+    std::vector<float> shade_rec((size_t)tri_total * 20);
+    g = 0;
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const uint32_t *id = d->instance_descs + 4 * (size_t)i;
+        const uint32_t *md = d->mesh_descs + 4 * (size_t)id[0];
+        uint32_t ntri = md[1] / 3;
+        uint32_t matw = id[1] | ((id[3] & 1u) << 31);
+        for (uint32_t p = 0; p < ntri; ++p, ++g) {
+            const float *v[3];
+            for (int k = 0; k < 3; ++k) v[k] = d->vertices + 8 * (size_t)(md[2] + d->indices[md[3] + 3 * p + k]);
+            float *r = &shade_rec[20 * g];
+            for (int k = 0; k < 3; ++k) {
+                r[4 * k + 0] = v[k][0];
+                r[4 * k + 1] = v[k][1];
+                r[4 * k + 2] = v[k][2];
+                r[4 * k + 3] = v[k][3];
+            }
+            r[12] = v[0][4];
+            r[13] = v[0][5];
+            r[14] = v[1][4];
+            r[15] = v[1][5];
+            r[16] = v[2][4];
+            r[17] = v[2][5];
+            std::memcpy(&r[18], &id[2], 4);
+            std::memcpy(&r[19], &matw, 4);
+        }
+    }
     nanbvh::BuildOptions bo;
     if (options) {
         if (options->sah_bins) bo.bins = options->sah_bins;
@@ -435,7 +462,7 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     };
     // vertices: 32-byte stride copied as-is (pos.xyz n.x | n.yz uv)
     uint4 *nodes;
-    float4 *tris, *verts, *xf, *itf, *mats;
+    float4 *tris, *verts, *xf, *itf, *mats, *shrec;
     uint2 *ids;
     uint32_t *indices;
     uint4 *mesh_descs, *inst_descs;
@@ -451,13 +478,14 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     rc |= up(&xf, d->transforms, (size_t)d->transform_count * 64);
     rc |= up(&itf, d->inv_transposes, (size_t)d->transform_count * 64);
     rc |= up(&mats, d->materials, (size_t)d->material_count * 32);
+    rc |= up(&shrec, shade_rec.data(), shade_rec.size() * 4);
     if (rc) {
         delete s;
         return -1;
     }
     bytes_total = bvh.nodes.size() * 4 + bvh.tris.size() * 4 + tri_ids.size() * 4 + (size_t)d->vertex_count * 32 +
                   (size_t)d->index_count * 4 + (size_t)d->mesh_count * 16 + (size_t)d->instance_count * 16 +
-                  (size_t)d->transform_count * 128 + (size_t)d->material_count * 32;
+                  (size_t)d->transform_count * 128 + (size_t)d->material_count * 32 + shade_rec.size() * 4;
     s->dev.nodes = nodes;
     s->dev.tris = tris;
     s->dev.tri_ids = ids;
@@ -468,6 +496,7 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     s->dev.transforms = xf;
     s->dev.inv_transposes = itf;
     s->dev.materials = mats;
+    s->dev.tri_shading = shrec;
     s->dev.root = bvh.root_entry;
     float max_coord = 1.0f;
     for (float v : world) max_coord = std::max(max_coord, std::fabs(v));
@@ -532,6 +561,7 @@ NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint3
     std::vector<uint32_t> face_inst(face_count);
     std::vector<uint32_t> prefix(instance_count);
     uint64_t front = 0, payload_end = 0;
+    uint64_t faces_before = 0;
     for (uint32_t i = 0; i < instance_count; ++i) {
         const uint32_t *ii = instance_table + 4 * (size_t)i;
         const uint32_t *sd = side_table + 2 * (size_t)i;
@@ -539,6 +569,9 @@ NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint3
             return fail(err, cap, "nan_tracer_create: instance %u face range out of bounds", i);
         if (ii[1] != scene->instance_triangles[i])
             return fail(err, cap, "nan_tracer_create: instance %u face_count differs from its mesh", i);
+        if (ii[0] != faces_before)
+            return fail(err, cap, "nan_tracer_create: instance %u faces are not in instance order", i);
+        faces_before += ii[1];
         prefix[i] = (uint32_t)front;
         uint64_t local = 0;
         for (uint32_t f = ii[0]; f < ii[0] + ii[1]; ++f) {

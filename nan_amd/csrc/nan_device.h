@@ -115,6 +115,7 @@ struct DevScene {
     const float4 *transforms;     // 4 float4 rows
     const float4 *inv_transposes; // 4 float4 rows
     const float4 *materials;      // 2 float4: base.rgb flags | emissive.rgb alpha
+    const float4 *tri_shading;    // 5 float4 per global triangle (load_triangle)
     uint32_t root;
     float slack;                  // absolute box-test slack (see kPad)
     Light light;
@@ -316,33 +317,34 @@ __device__ __forceinline__ f3 mul33(const float4 *m, f3 v) {
     return mk(dot(row3(m[0]), v), dot(row3(m[1]), v), dot(row3(m[2]), v));
 }
 
-__device__ __forceinline__ void load_triangle(const DevScene &S, uint32_t instance, uint32_t prim, uint4 &inst,
-                                              f3 &v0, f3 &v1, f3 &v2, float4 (&vb)[3]) {
-    inst = S.instance_descs[instance];
-    uint4 mesh = S.mesh_descs[inst.x];
-    const uint32_t *idx = S.indices + mesh.w + 3 * (size_t)prim;
-    uint32_t i0 = mesh.z + idx[0], i1 = mesh.z + idx[1], i2 = mesh.z + idx[2];
-    float4 a0 = S.vertices[2 * (size_t)i0], a1 = S.vertices[2 * (size_t)i1], a2 = S.vertices[2 * (size_t)i2];
-    vb[0] = S.vertices[2 * (size_t)i0 + 1];
-    vb[1] = S.vertices[2 * (size_t)i1 + 1];
-    vb[2] = S.vertices[2 * (size_t)i2 + 1];
-    v0 = mk(a0.x, a0.y, a0.z);
-    v1 = mk(a1.x, a1.y, a1.z);
-    v2 = mk(a2.x, a2.y, a2.z);
-    // normals: (a.w, vb.x, vb.y)
-    vb[0] = make_float4(a0.w, vb[0].x, vb[0].y, 0.0f);
-    vb[1] = make_float4(a1.w, vb[1].x, vb[1].y, 0.0f);
-    vb[2] = make_float4(a2.w, vb[2].x, vb[2].y, 0.0f);
+// Per-triangle shading record (5 float4 = 80 B, global triangle order):
+//   v0.xyz n0.x | v1.xyz n1.x | v2.xyz n2.x | n0.yz n1.yz | n2.yz transform_id material_id|odd<<31
+// holding the object-space vertices and normals the reference gathers through
+// instance -> mesh -> index -> vertex (scene.slang:271-300), copied bit for bit.
+__device__ __forceinline__ void load_triangle(const DevScene &S, uint32_t tri, f3 &v0, f3 &v1, f3 &v2, f3 (&n)[3],
+                                              uint32_t &xf, uint32_t &mat, bool &odd) {
+    const float4 *rec = S.tri_shading + 5 * (size_t)tri;
+    float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3], r4 = rec[4];
+    v0 = mk(r0.x, r0.y, r0.z);
+    v1 = mk(r1.x, r1.y, r1.z);
+    v2 = mk(r2.x, r2.y, r2.z);
+    n[0] = mk(r0.w, r3.x, r3.y);
+    n[1] = mk(r1.w, r3.z, r3.w);
+    n[2] = mk(r2.w, r4.x, r4.y);
+    xf = __float_as_uint(r4.z);
+    uint32_t m = __float_as_uint(r4.w);
+    mat = m & 0x7FFFFFFFu;
+    odd = (m >> 31) != 0u;
 }
 
-__device__ __forceinline__ void shading_data(const DevScene &S, uint32_t instance, uint32_t prim, float b1, float b2,
-                                             f3 view, Shading &sd) {
-    uint4 inst;
-    f3 v0, v1, v2;
-    float4 nrm[3];
-    load_triangle(S, instance, prim, inst, v0, v1, v2, nrm);
-    const float4 *M = S.transforms + 4 * (size_t)inst.z;
-    const float4 *IT = S.inv_transposes + 4 * (size_t)inst.z;
+__device__ __forceinline__ void shading_data(const DevScene &S, uint32_t tri, float b1, float b2, f3 view,
+                                             Shading &sd) {
+    f3 v0, v1, v2, nrm[3];
+    uint32_t xf, mat;
+    bool odd;
+    load_triangle(S, tri, v0, v1, v2, nrm, xf, mat, odd);
+    const float4 *M = S.transforms + 4 * (size_t)xf;
+    const float4 *IT = S.inv_transposes + 4 * (size_t)xf;
     float4 m0 = M[0], m1 = M[1], m2 = M[2];
     float4 it0 = IT[0], it1 = IT[1], it2 = IT[2], it3 = IT[3];
     f3 bc = mk(1.0f - b1 - b2, b1, b2);
@@ -375,12 +377,10 @@ __device__ __forceinline__ void shading_data(const DevScene &S, uint32_t instanc
     float world_offset = dot(world_error, fabs3(world_normal));
     world_offset = __builtin_fmaf(world_scale, object_offset, world_offset);
 
-    f3 n0 = mk(nrm[0].x, nrm[0].y, nrm[0].z), n1 = mk(nrm[1].x, nrm[1].y, nrm[1].z),
-       n2 = mk(nrm[2].x, nrm[2].y, nrm[2].z);
-    f3 ni = n0 * bc.x + n1 * bc.y + n2 * bc.z;
+    f3 ni = nrm[0] * bc.x + nrm[1] * bc.y + nrm[2] * bc.z;
     f3 geometry_normal = normalize(mk(dot(row3(it0), ni), dot(row3(it1), ni), dot(row3(it2), ni)));
     f3 face_normal = world_normal;
-    float4 ma = S.materials[2 * (size_t)inst.y], mb = S.materials[2 * (size_t)inst.y + 1];
+    float4 ma = S.materials[2 * (size_t)mat], mb = S.materials[2 * (size_t)mat + 1];
     bool double_sided = (__float_as_uint(ma.w) & 1u) != 0;
     bool front = dot(face_normal, view) > 0.0f;
     float sign = (double_sided && !front) ? -1.0f : 1.0f;
@@ -399,14 +399,14 @@ __device__ __forceinline__ void shading_data(const DevScene &S, uint32_t instanc
 }
 
 // texture_space_path_tracer.slang:39-55
-__device__ __forceinline__ f3 world_flat_normal(const DevScene &S, uint32_t instance, uint32_t prim) {
-    uint4 inst;
-    f3 p0, p1, p2;
-    float4 nrm[3];
-    load_triangle(S, instance, prim, inst, p0, p1, p2, nrm);
+__device__ __forceinline__ f3 world_flat_normal(const DevScene &S, uint32_t tri) {
+    f3 p0, p1, p2, nrm[3];
+    uint32_t xf, mat;
+    bool odd;
+    load_triangle(S, tri, p0, p1, p2, nrm, xf, mat, odd);
     f3 n = normalize(cross(p1 - p0, p2 - p0));
-    if (inst.w & 1u) n = -n;
-    const float4 *IT = S.inv_transposes + 4 * (size_t)inst.z;
+    if (odd) n = -n;
+    const float4 *IT = S.inv_transposes + 4 * (size_t)xf;
     float4 it0 = IT[0], it1 = IT[1], it2 = IT[2];
     return normalize(mk(dot(row3(it0), n), dot(row3(it1), n), dot(row3(it2), n)));
 }

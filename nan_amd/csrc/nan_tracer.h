@@ -53,9 +53,8 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
             pc.tris += st.tris;
         }
         if (!hit) return rad + T * L.sky;
-        uint2 ids = S.tri_ids[h.id];
         Shading sd;
-        shading_data(S, ids.x, ids.y, h.b1, h.b2, -dir, sd);
+        shading_data(S, h.id, h.b1, h.b2, -dir, sd);
         rad = rad + T * sd.emissive;
         float ux = rng.next_1d(), uy = rng.next_1d();
         f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
@@ -81,11 +80,11 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
 
 // estimate_surface_irradiance (texture_space_path_tracer.slang:86-117)
 template <int STACK, int BLOCK, bool COUNT>
-__device__ f3 estimate_irradiance(const DevScene &S, uint32_t instance, uint32_t prim, float b1, float b2, f3 view,
+__device__ f3 estimate_irradiance(const DevScene &S, uint32_t tri, float b1, float b2, f3 view,
                                   uint32_t spp, uint32_t max_bounces, Rng &rng, uint32_t *col, PathCounters &pc) {
     const Light &L = S.light;
     Shading sd;
-    shading_data(S, instance, prim, b1, b2, view, sd);
+    shading_data(S, tri, b1, b2, view, sd);
     f3 acc = mk(0.0f, 0.0f, 0.0f);
     for (uint32_t k = 0; k < spp; ++k) {
         f3 sample = mk(0.0f, 0.0f, 0.0f);
@@ -126,7 +125,7 @@ __device__ __forceinline__ void store_accumulated(float4 *payload, uint32_t slot
 }
 
 struct TexelSurface {
-    uint32_t instance, prim, global_texel, local;
+    uint32_t instance, prim, tri, global_texel, local;
     float b1, b2;
     uint2 side;
 };
@@ -143,6 +142,7 @@ __device__ __forceinline__ TexelSurface texel_surface(const DevLayout &Ly, uint6
     ts.side = Ly.sides[inst];
     ts.instance = inst;
     ts.prim = f - ii.x;
+    ts.tri = f;  // layout faces follow the scene's global triangle order
     ts.local = (uint32_t)(w - Ly.front_prefix[inst]);
     ts.global_texel = ii.z + ts.local;
     uint32_t i, j;

@@ -19,7 +19,8 @@
 // oracle): the payload is bit-identical.
 //
 // Traversal kernels are persistent: a fixed grid sized from the occupancy
-// calculator, each wave fetching 64 rays at a time from an atomic counter.
+// calculator; a lane whose ray is done takes the next queued ray (refills of a
+// wave share one atomic).
 // Their stack is a 16-entry ring per lane in LDS that spills its oldest
 // entries to a per-lane column in HBM when a traversal goes deeper.
 #include "nan_tracer.h"
@@ -78,60 +79,66 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
     return base + (uint32_t)__popcll(below);
 }
 
-// Traversal with a kLdsStack-entry LDS ring per lane that spills to HBM.
-// Identical visiting rule and result to traverse() in nan_device.h.
+// Resumable traversal with a kLdsStack-entry LDS ring per lane that spills
+// its oldest entries to HBM.  One call of trav_step() processes one node or
+// one leaf; the visiting rule and the result equal traverse() in
+// nan_device.h (closest hit = smallest (t, triangle id); any-hit stops at
+// the first hit in [tmin, tmax], in whatever order children are visited).
+struct TravState {
+    uint32_t cur;
+    int sp;
+};
+
 template <bool ANY_HIT, bool COUNT>
-__device__ __forceinline__ bool traverse_ring(const DevScene &S, const Ray &r, Hit &hit, uint32_t *lds_col,
-                                              uint32_t *spill_col, uint32_t spill_stride, TraversalStats &st) {
-    uint32_t cur = S.root;
-    int sp = 0;
+__device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, Hit &hit, TravState &ts, uint32_t *lds_col,
+                                          uint32_t *spill_col, uint32_t spill_stride, TraversalStats &st) {
     auto push = [&](uint32_t e) {
-        uint32_t *slot = lds_col + (sp & (kLdsStack - 1)) * kTravBlock;
-        if (sp >= kLdsStack) spill_col[(size_t)(sp - kLdsStack) * spill_stride] = *slot;
+        uint32_t *slot = lds_col + (ts.sp & (kLdsStack - 1)) * kTravBlock;
+        if (ts.sp >= kLdsStack) spill_col[(size_t)(ts.sp - kLdsStack) * spill_stride] = *slot;
         *slot = e;
-        ++sp;
+        ++ts.sp;
     };
     auto pop = [&]() -> uint32_t {
-        --sp;
-        uint32_t *slot = lds_col + (sp & (kLdsStack - 1)) * kTravBlock;
+        --ts.sp;
+        uint32_t *slot = lds_col + (ts.sp & (kLdsStack - 1)) * kTravBlock;
         uint32_t e = *slot;
-        if (sp >= kLdsStack) *slot = spill_col[(size_t)(sp - kLdsStack) * spill_stride];
+        if (ts.sp >= kLdsStack) *slot = spill_col[(size_t)(ts.sp - kLdsStack) * spill_stride];
         return e;
     };
-    while (true) {
-        if (!(cur & kLeafBit)) {
-            const uint4 *np = S.nodes + 4 * (size_t)cur;
-            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
-            if (COUNT) st.nodes++;
-            f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
-            float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
-            float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
-            float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
-            uint32_t valid = w0.w >> 24;
-            uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
-            uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
-            float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack;
-            float tn[4];
-            uint32_t ent[4];
-            int nhit = 0;
+    if (!(ts.cur & kLeafBit)) {
+        const uint4 *np = S.nodes + 4 * (size_t)ts.cur;
+        uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+        if (COUNT) st.nodes++;
+        f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
+        float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
+        float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
+        float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
+        uint32_t valid = w0.w >> 24;
+        uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
+        uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
+        float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack;
+        float tn[4];
+        uint32_t ent[4];
+        int nhit = 0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float lox = origin.x + (float)((qlx >> (8 * c)) & 0xFFu) * sx;
-                float loy = origin.y + (float)((qly >> (8 * c)) & 0xFFu) * sy;
-                float loz = origin.z + (float)((qlz >> (8 * c)) & 0xFFu) * sz;
-                float hix = origin.x + (float)((qhx >> (8 * c)) & 0xFFu) * sx;
-                float hiy = origin.y + (float)((qhy >> (8 * c)) & 0xFFu) * sy;
-                float hiz = origin.z + (float)((qhz >> (8 * c)) & 0xFFu) * sz;
-                float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
-                float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
-                float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
-                float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-                float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + S.slack;
-                far = fminf(far, limit);
-                bool ok = ((valid >> c) & 1u) && near <= far;
-                if (ok) {
-                    float key = near;
-                    uint32_t e = child[c];
+        for (int c = 0; c < 4; ++c) {
+            float lox = origin.x + (float)((qlx >> (8 * c)) & 0xFFu) * sx;
+            float loy = origin.y + (float)((qly >> (8 * c)) & 0xFFu) * sy;
+            float loz = origin.z + (float)((qlz >> (8 * c)) & 0xFFu) * sz;
+            float hix = origin.x + (float)((qhx >> (8 * c)) & 0xFFu) * sx;
+            float hiy = origin.y + (float)((qhy >> (8 * c)) & 0xFFu) * sy;
+            float hiz = origin.z + (float)((qhz >> (8 * c)) & 0xFFu) * sz;
+            float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
+            float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
+            float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
+            float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+            float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + S.slack;
+            far = fminf(far, limit);
+            bool ok = ((valid >> c) & 1u) && near <= far;
+            if (ok) {
+                float key = near;
+                uint32_t e = child[c];
+                if (!ANY_HIT) {
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         if (k < nhit && key < tn[k]) {
@@ -143,52 +150,108 @@ __device__ __forceinline__ bool traverse_ring(const DevScene &S, const Ray &r, H
                             e = ek;
                         }
                     }
+                }
 #pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        if (k == nhit) {
-                            tn[k] = key;
-                            ent[k] = e;
-                        }
-                    nhit++;
-                }
-            }
-            if (nhit == 0) {
-                if (sp == 0) break;
-                cur = pop();
-                continue;
-            }
-#pragma unroll
-            for (int k = 3; k >= 1; --k)
-                if (k < nhit) push(ent[k]);
-            cur = ent[0];
-            continue;
-        }
-        uint32_t first = cur & kLeafFirstMask;
-        uint32_t count = ((cur >> kLeafCountShift) & 0xFu) + 1u;
-        for (uint32_t i = 0; i < count; ++i) {
-            const float4 *tp = S.tris + 3 * (size_t)(first + i);
-            float4 a = tp[0], b = tp[1], c = tp[2];
-            if (COUNT) st.tris++;
-            float t, b1, b2;
-            if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
-                uint32_t id = __float_as_uint(c.y);
-                if (ANY_HIT) {
-                    hit.t = t;
-                    hit.id = id;
-                    return true;
-                }
-                if (t < hit.t || (t == hit.t && id < hit.id)) {
-                    hit.t = t;
-                    hit.b1 = b1;
-                    hit.b2 = b2;
-                    hit.id = id;
-                }
+                for (int k = 0; k < 4; ++k)
+                    if (k == nhit) {
+                        tn[k] = key;
+                        ent[k] = e;
+                    }
+                nhit++;
             }
         }
-        if (sp == 0) break;
-        cur = pop();
+        if (nhit == 0) {
+            if (ts.sp == 0) return true;
+            ts.cur = pop();
+            return false;
+        }
+#pragma unroll
+        for (int k = 3; k >= 1; --k)
+            if (k < nhit) push(ent[k]);
+        ts.cur = ent[0];
+        return false;
     }
-    return hit.id != kInvalid;
+    uint32_t first = ts.cur & kLeafFirstMask;
+    uint32_t count = ((ts.cur >> kLeafCountShift) & 0xFu) + 1u;
+    for (uint32_t i = 0; i < count; ++i) {
+        const float4 *tp = S.tris + 3 * (size_t)(first + i);
+        float4 a = tp[0], b = tp[1], c = tp[2];
+        if (COUNT) st.tris++;
+        float t, b1, b2;
+        if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
+            uint32_t id = __float_as_uint(c.y);
+            if (ANY_HIT) {
+                hit.t = t;
+                hit.id = id;
+                return true;
+            }
+            if (t < hit.t || (t == hit.t && id < hit.id)) {
+                hit.t = t;
+                hit.b1 = b1;
+                hit.b2 = b2;
+                hit.id = id;
+            }
+        }
+    }
+    if (ts.sp == 0) return true;
+    ts.cur = pop();
+    return false;
+}
+
+// Persistent traversal over a ray queue with per-lane refill: a lane whose
+// ray has finished takes the next queued ray (one atomic per wave for all
+// refilling lanes), so a wave keeps tracing while any rays remain instead
+// of waiting for its slowest lane.
+template <bool ANY_HIT, bool COUNT, typename Load, typename Store>
+__device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t count, uint32_t *fetch, uint32_t *lds_col,
+                                            uint32_t *spill_col, uint32_t spill_stride, Load load, Store store,
+                                            uint32_t &n_rays, uint32_t &n_nodes, uint32_t &n_tris) {
+    constexpr int kSteps = 16;
+    Ray r;
+    Hit h;
+    TravState ts{0, 0};
+    TraversalStats st{0, 0};
+    uint32_t k = 0;
+    bool has = false;
+    bool drained = false;
+    while (true) {
+        if (!drained) {
+            bool need = !has;
+            unsigned long long mask = __ballot(need);
+            uint32_t n = (uint32_t)__popcll(mask);
+            if (n) {
+                uint32_t base = 0;
+                int leader = __ffsll((long long)mask) - 1;
+                if ((int)lane_id() == leader) base = atomicAdd(fetch, n);
+                base = __shfl(base, leader);
+                if (need) {
+                    uint32_t kk = base + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+                    if (kk < count) {
+                        k = kk;
+                        load(k, r);
+                        ray_setup(r);
+                        h = Hit{r.tmax, 0.0f, 0.0f, kInvalid};
+                        ts = TravState{S.root, 0};
+                        st = TraversalStats{0, 0};
+                        has = true;
+                    }
+                }
+                if (base + n >= count) drained = true;
+            }
+        }
+        if (__ballot(has) == 0ull) break;
+        for (int s = 0; s < kSteps && has; ++s) {
+            if (trav_step<ANY_HIT, COUNT>(S, r, h, ts, lds_col, spill_col, spill_stride, st)) {
+                store(k, h);
+                has = false;
+                if (COUNT) {
+                    n_rays++;
+                    n_nodes += st.nodes;
+                    n_tris += st.tris;
+                }
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ void add_counter(unsigned long long *counters, int slot, uint32_t v) {
@@ -229,9 +292,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_generate(DevScene S, DevLayout Ly
             Rng rng;
             TexelSurface ts = texel_surface(Ly, w, P.iteration, P.spp, rng);
             if (P.side != 0 || P.sample != 0) rng.state = W.rng[w];
-            f3 n = world_flat_normal(S, ts.instance, ts.prim);
+            f3 n = world_flat_normal(S, ts.tri);
             Shading sd;
-            shading_data(S, ts.instance, ts.prim, ts.b1, ts.b2, P.side ? -n : n, sd);
+            shading_data(S, ts.tri, ts.b1, ts.b2, P.side ? -n : n, sd);
             float ux = rng.next_1d(), uy = rng.next_1d();
             ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
             float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
@@ -265,39 +328,25 @@ __global__ __launch_bounds__(kWfBlock) void wf_generate(DevScene S, DevLayout Ly
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTravBlock) void wf_extend(DevScene S, WfBuffers W, uint32_t q, uint32_t count_slot,
+__global__ __launch_bounds__(kTravBlock, 8) void wf_extend(DevScene S, WfBuffers W, uint32_t q, uint32_t count_slot,
                                                        uint32_t fetch_slot, unsigned long long *counters) {
     __shared__ uint32_t stack[kLdsStack * kTravBlock];
-    uint32_t *col = stack + threadIdx.x;
     uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
-    uint32_t *spill_col = W.spill + gid;
     const uint32_t count = W.counters[count_slot];
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
-    while (true) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&W.counters[fetch_slot], 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        uint32_t k = base + lane_id();
-        if (k < count) {
-            float4 o = W.ray_o[q][k], d = W.ray_d[q][k];
-            Ray r;
+    const float4 *ro = W.ray_o[q], *rd = W.ray_d[q];
+    float4 *hits = W.hits;
+    trace_queue<false, COUNT>(
+        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes,
+        [&](uint32_t k, Ray &r) {
+            float4 o = ro[k], d = rd[k];
             r.o = mk(o.x, o.y, o.z);
             r.d = mk(d.x, d.y, d.z);
             r.tmin = 0.0f;
             r.tmax = 1000.0f;
-            ray_setup(r);
-            Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
-            TraversalStats st{0, 0};
-            traverse_ring<false, COUNT>(S, r, h, col, spill_col, W.spill_lanes, st);
-            W.hits[k] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.id));
-            if (COUNT) {
-                n_rays++;
-                n_nodes += st.nodes;
-                n_tris += st.tris;
-            }
-        }
-    }
+        },
+        [&](uint32_t k, const Hit &h) { hits[k] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.id)); },
+        n_rays, n_nodes, n_tris);
     if (COUNT) {
         add_counter(counters, NAN_COUNTER_EXTEND_RAYS, n_rays);
         add_counter(counters, NAN_COUNTER_NODE_VISITS, n_nodes);
@@ -306,51 +355,36 @@ __global__ __launch_bounds__(kTravBlock) void wf_extend(DevScene S, WfBuffers W,
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTravBlock) void wf_shadow(DevScene S, WfBuffers W, uint32_t count_slot,
+__global__ __launch_bounds__(kTravBlock, 8) void wf_shadow(DevScene S, WfBuffers W, uint32_t count_slot,
                                                        uint32_t fetch_slot, unsigned long long *counters) {
     __shared__ uint32_t stack[kLdsStack * kTravBlock];
-    uint32_t *col = stack + threadIdx.x;
     uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
-    uint32_t *spill_col = W.spill + gid;
     const uint32_t count = W.counters[count_slot];
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
-    while (true) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&W.counters[fetch_slot], 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        uint32_t k = base + lane_id();
-        if (k < count) {
+    trace_queue<true, COUNT>(
+        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes,
+        [&](uint32_t k, Ray &r) {
             float4 o = W.sh_o[k], d = W.sh_d[k];
-            Ray r;
             r.o = mk(o.x, o.y, o.z);
             r.d = mk(d.x, d.y, d.z);
             r.tmin = 0.0f;
             r.tmax = 1000.0f;
-            ray_setup(r);
-            Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
-            TraversalStats st{0, 0};
-            bool occluded = traverse_ring<true, COUNT>(S, r, h, col, spill_col, W.spill_lanes, st);
-            if (!occluded) {
-                uint32_t p = __float_as_uint(o.w);
-                float4 c = W.sh_c[k];
-                if (__float_as_uint(d.w) == kTargetSun) {
-                    W.sun[p] = make_float4(c.x, c.y, c.z, 0.0f);
-                } else {
-                    float4 v = W.rad[p];
-                    v.x = v.x + c.x;
-                    v.y = v.y + c.y;
-                    v.z = v.z + c.z;
-                    W.rad[p] = v;
-                }
+        },
+        [&](uint32_t k, const Hit &h) {
+            if (h.id != kInvalid) return;  // occluded
+            float4 o = W.sh_o[k], d = W.sh_d[k], c = W.sh_c[k];
+            uint32_t p = __float_as_uint(o.w);
+            if (__float_as_uint(d.w) == kTargetSun) {
+                W.sun[p] = make_float4(c.x, c.y, c.z, 0.0f);
+            } else {
+                float4 v = W.rad[p];
+                v.x = v.x + c.x;
+                v.y = v.y + c.y;
+                v.z = v.z + c.z;
+                W.rad[p] = v;
             }
-            if (COUNT) {
-                n_rays++;
-                n_nodes += st.nodes;
-                n_tris += st.tris;
-            }
-        }
-    }
+        },
+        n_rays, n_nodes, n_tris);
     if (COUNT) {
         add_counter(counters, NAN_COUNTER_SHADOW_RAYS, n_rays);
         add_counter(counters, NAN_COUNTER_SHADOW_NODE_VISITS, n_nodes);
@@ -387,10 +421,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_shade(DevScene S, WfBuffers W, Sh
             } else {
                 Rng rng;
                 rng.state = W.rng[p];
-                uint2 ids = S.tri_ids[id];
                 f3 d = mk(rd.x, rd.y, rd.z);
                 Shading sd;
-                shading_data(S, ids.x, ids.y, hv.y, hv.z, -d, sd);
+                shading_data(S, id, hv.y, hv.z, -d, sd);
                 rad = rad + T * sd.emissive;
                 float ux = rng.next_1d(), uy = rng.next_1d();
                 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
