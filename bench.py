@@ -111,12 +111,77 @@ def cpu_baseline(node, layout, threads: int, target_seconds: float = 12.0) -> di
     }
 
 
+def bench_batch(args) -> int:
+    """64 seeded scenes (1k-200k triangles, 16k-2M slot budgets, 1-8 bounces)
+    traced by one set of wavefront launches captured in a hipGraph, after the
+    autotuner picked the traversal launch shape."""
+    import numpy as np
+    import torch
+
+    from nan_amd.autotune import autotune
+    from nan_amd.batch import SceneBatch, batch_workload
+    from nan_amd.device import Device
+
+    t0 = time.perf_counter()
+    nodes, kws, bounces = batch_workload(0, 64)
+    device = Device(0)
+    batch = SceneBatch(device, nodes, kws, bounces)
+    log(f"batch of {len(nodes)} scenes, {sum(batch.triangle_counts)} tris, {batch.total_slots} slots "
+        f"({time.perf_counter() - t0:.1f} s)")
+    enc = device.create_command_encoder()
+    tuned = autotune(batch, enc, frames=1, log=log)
+    # frame 0 eagerly, then replay a captured frame
+    batch.execute(enc, 0, reset=True)
+    for it in range(1, 1 + args.warmup):
+        batch.execute(enc, it)
+    torch.cuda.synchronize()
+    graph = batch.capture(enc, 1 + args.warmup)
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        graph.replay()
+    end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(end)
+    batch.clear_device_frame()
+    slots = batch.total_slots
+    out = {
+        "metric": METRIC,
+        "value": slots * args.steps / (total_ms * 1e-3),
+        "unit": "texel-path samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "64-scene batch (seed 0): 1k-200k triangles and 16k-2M slot budgets log-uniform, "
+                        "1-8 bounces, one hipGraph per frame",
+            "scenes": len(nodes),
+            "triangles": int(sum(batch.triangle_counts)),
+            "payload_slots": int(slots),
+            "bounces": bounces,
+            "tuning": tuned["best"],
+        },
+        "autotune": tuned["results"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["synthetic", "cornell"], default="synthetic")
+    ap.add_argument("--workload", choices=["synthetic", "cornell", "batch"], default="synthetic")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -149,6 +214,9 @@ def main() -> int:
     from nan_amd.device import Device
     from nan_amd.scene import Scene
     from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
+    if args.workload == "batch":
+        return bench_batch(args)
 
     t0 = time.perf_counter()
     node, layout, layout_kw, desc = build_workload(args.workload)

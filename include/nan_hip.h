@@ -135,6 +135,33 @@ enum { NAN_STAGE_GENERATE = 0, NAN_STAGE_EXTEND, NAN_STAGE_SHADE, NAN_STAGE_SHAD
 NAN_API int nan_tracer_profile(NanTracer *tracer, int enable);
 NAN_API int nan_tracer_stage_times(NanTracer *tracer, double *ms, uint32_t *launches);
 
+/* Launch shape of the wavefront traversal kernels: persistent blocks per CU
+ * (0 = occupancy limit), LDS stack ring entries per lane (16, 24 or 32; deeper
+ * entries spill to HBM) and traversal steps between per-lane ray refills.
+ * Results do not depend on it.  No reference counterpart (autotuner hook). */
+NAN_API int nan_tracer_tune(NanTracer *tracer, uint32_t blocks_per_cu, uint32_t lds_stack, uint32_t refill_steps);
+
+/* Many independent scenes traced by one set of launches: concatenates the
+ * scenes' BVHs, shading records and layouts (each tracer must be over the
+ * scene at the same index) into a batched tracer (wavefront mode).  Each
+ * scene keeps its own cache, RNG streams and lighting, so every slot equals a
+ * one-at-a-time run of that scene.  No reference counterpart (the reference
+ * dispatches per instance of one scene, texture_space_path_tracer.py:42-59). */
+NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers, uint32_t count, NanTracer **out,
+                             char *err, size_t cap);
+/* Per-scene payload buffers (device pointers, one per scene); required before
+ * nan_tst_execute on a batch, whose payload argument is then ignored. */
+NAN_API int nan_batch_set_payloads(NanTracer *batch, void *const *payloads);
+/* Per-scene path length (0 = the launch's max_bounces); the launch's
+ * max_bounces must be at least the largest. */
+NAN_API int nan_batch_set_bounces(NanTracer *batch, const uint32_t *max_bounces);
+
+/* Device-frame mode for graph replay: while enabled, nan_tst_execute takes the
+ * iteration and reset flag from a device counter (set here to iteration/reset)
+ * and advances it at the end of each call, so one captured hipGraph of a call
+ * replays successive iterations.  No reference counterpart. */
+NAN_API int nan_tracer_device_frame(NanTracer *tracer, int enable, uint32_t iteration, uint32_t reset);
+
 /* One progressive iteration over every texel of every instance
  * (TextureSpacePathTracer.execute, texture_space_path_tracer.py:33-59; kernel
  * texture_space_path_tracer.slang:140-216).  payload: 16-byte slots

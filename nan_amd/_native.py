@@ -71,7 +71,8 @@ EXPORTED_SYMBOLS = (
     "nan_version", "nan_last_error", "nan_scene_create", "nan_scene_destroy", "nan_scene_info",
     "nan_scene_set_lighting", "nan_trace_rays", "nan_tracer_create", "nan_tracer_destroy", "nan_tracer_tables",
     "nan_tracer_set_mode", "nan_tracer_profile", "nan_tracer_stage_times", "nan_tst_execute", "nan_tst_execute_counted", "nan_tst_raygen", "nan_rgb9e5_pack", "nan_rgb9e5_unpack",
-    "nan_resolve", "nan_accumulate", "nan_tonemap",
+    "nan_resolve", "nan_accumulate", "nan_tonemap", "nan_tracer_tune", "nan_batch_create",
+    "nan_batch_set_payloads", "nan_batch_set_bounces", "nan_tracer_device_frame",
 )
 
 _LIB = None
@@ -105,6 +106,12 @@ def _declare(lib) -> None:
         "nan_resolve": (I, [vp, vp, I, f32p, vp, U32, U32, vp]),
         "nan_accumulate": (I, [vp, vp, vp, U64, I, vp]),
         "nan_tonemap": (I, [vp, vp, U64, ctypes.c_float, vp]),
+        "nan_tracer_tune": (I, [vp, U32, U32, U32]),
+        "nan_batch_create": (I, [ctypes.POINTER(vp), ctypes.POINTER(vp), U32, ctypes.POINTER(vp), ctypes.c_char_p,
+                                 ctypes.c_size_t]),
+        "nan_batch_set_payloads": (I, [vp, ctypes.POINTER(vp)]),
+        "nan_batch_set_bounces": (I, [vp, u32p]),
+        "nan_tracer_device_frame": (I, [vp, I, U32, U32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -497,11 +497,30 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     s->dev.inv_transposes = itf;
     s->dev.materials = mats;
     s->dev.tri_shading = shrec;
+    s->h_nodes = std::move(bvh.nodes);
+    s->h_tris = std::move(bvh.tris);
+    s->h_shade = std::move(shade_rec);
+    s->h_xf.assign(d->transforms, d->transforms + (size_t)d->transform_count * 16);
+    s->h_itf.assign(d->inv_transposes, d->inv_transposes + (size_t)d->transform_count * 16);
+    s->h_mats.assign(d->materials, d->materials + (size_t)d->material_count * 8);
     s->dev.root = bvh.root_entry;
     float max_coord = 1.0f;
     for (float v : world) max_coord = std::max(max_coord, std::fabs(v));
     s->dev.slack = max_coord * 0x1p-18f;
     s->dev.light = make_light(d->lighting);
+    {
+        SceneRec rec{};
+        rec.root = s->dev.root;
+        rec.slack = s->dev.slack;
+        rec.max_bounces = 0;
+        rec.light = s->dev.light;
+        SceneRec *drec;
+        if (up(&drec, &rec, sizeof(rec))) {
+            delete s;
+            return -1;
+        }
+        s->dev.scenes = drec;
+    }
     s->stack_depth = depth;
     s->instance_triangles.resize(d->instance_count);
     for (uint32_t i = 0; i < d->instance_count; ++i)
@@ -529,6 +548,11 @@ NAN_API int nan_scene_info(const NanScene *scene, NanSceneInfo *out) {
 NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16) {
     if (!scene || !lighting16) return fail(nullptr, 0, "nan_scene_set_lighting: null argument");
     scene->dev.light = make_light(lighting16);
+    SceneRec rec{};
+    rec.root = scene->dev.root;
+    rec.slack = scene->dev.slack;
+    rec.light = scene->dev.light;
+    HIP_TRY(hipMemcpy((void *)scene->dev.scenes, &rec, sizeof(rec), hipMemcpyHostToDevice), nullptr, 0);
     return 0;
 }
 
@@ -616,6 +640,13 @@ NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint3
             for (uint32_t k = 0; k < instance_table[4 * (size_t)i + 3]; ++k) ds.push_back(prefix[i] + k);
     t->ds_count = (uint32_t)ds.size();
     rc |= upload(&t->ds_list, ds.data(), ds.size() * 4, err, cap);
+    t->h_faces.assign(face_table, face_table + (size_t)face_count * 2);
+    t->h_instances.assign(instance_table, instance_table + (size_t)instance_count * 4);
+    t->h_sides.assign(side_table, side_table + (size_t)instance_count * 2);
+    t->h_texel_face = std::move(texel_face);
+    t->h_face_inst = std::move(face_inst);
+    t->h_prefix = std::move(prefix);
+    t->h_ds = std::move(ds);
     if (rc) {
         delete t;
         return -1;
@@ -637,8 +668,10 @@ NAN_API int nan_tracer_tables(const NanTracer *tracer, const void **face_table, 
 
 static int tst_execute(NanTracer *t, void *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                        uint32_t reset, uint64_t *counters, void *stream) {
-    if (!t || !payload) return fail(nullptr, 0, "nan_tst_execute: null argument");
+    if (!t || (!payload && t->scene_count == 1)) return fail(nullptr, 0, "nan_tst_execute: null argument");
     if (t->total_front == 0) return 0;
+    if (t->scene->batch && t->mode != NAN_TRACER_WAVEFRONT)
+        return fail(nullptr, 0, "nan_tst_execute: scene batches run in wavefront mode only");
     unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
     if (t->mode == NAN_TRACER_WAVEFRONT)
         return wavefront_execute(t, (float4 *)payload, iteration, spp, bounces, reset, c, as_stream(stream));
@@ -756,6 +789,210 @@ NAN_API int nan_tonemap(const float *input, float *output, uint64_t n, float exp
     hipLaunchKernelGGL(tonemap_kernel, dim3(grid_for(n, kFlatBlock)), dim3(kFlatBlock), 0, as_stream(stream),
                        (const float4 *)input, (float4 *)output, n, exposure);
     return check_launch("tonemap");
+}
+
+
+NAN_API int nan_tracer_tune(NanTracer *tracer, uint32_t blocks_per_cu, uint32_t lds_stack, uint32_t refill_steps) {
+    if (!tracer) return fail(nullptr, 0, "nan_tracer_tune: null tracer");
+    if (lds_stack != 16 && lds_stack != 24 && lds_stack != 32)
+        return fail(nullptr, 0, "nan_tracer_tune: lds_stack must be 16, 24 or 32");
+    if (refill_steps == 0 || refill_steps > 4096) return fail(nullptr, 0, "nan_tracer_tune: refill_steps out of range");
+    tracer->tuning.blocks_per_cu = blocks_per_cu;
+    tracer->tuning.ring = lds_stack;
+    tracer->tuning.refill_steps = refill_steps;
+    return 0;
+}
+
+NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers, uint32_t count, NanTracer **out,
+                             char *err, size_t cap) {
+    if (!scenes || !tracers || !out || count == 0) return fail(err, cap, "nan_batch_create: null argument");
+    *out = nullptr;
+    uint64_t nodes = 0, tris = 0, xfs = 0, mats = 0, faces = 0, insts = 0, front = 0, ds = 0;
+    uint32_t max_stack = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+        const NanScene *sc = scenes[k];
+        const NanTracer *tr = tracers[k];
+        if (!sc || !tr || tr->scene != sc) return fail(err, cap, "nan_batch_create: tracer %u is not over scene %u", k, k);
+        if (sc->batch || tr->scene_count != 1) return fail(err, cap, "nan_batch_create: batches cannot be nested");
+        nodes += sc->h_nodes.size() / nanbvh::kNodeWords;
+        tris += sc->h_tris.size() / nanbvh::kTriWords;
+        xfs += sc->h_xf.size() / 16;
+        mats += sc->h_mats.size() / 8;
+        faces += tr->face_count;
+        insts += tr->instance_count;
+        front += tr->total_front;
+        ds += tr->ds_count;
+        max_stack = std::max(max_stack, sc->info.max_stack);
+    }
+    if (tris > nanbvh::kLeafFirstMask || nodes >= (1ull << 31) || front >= (1ull << 32) || faces >= (1ull << 32))
+        return fail(err, cap, "nan_batch_create: batch too large for 32-bit indices");
+    std::vector<uint32_t> h_nodes;
+    std::vector<float> h_tris, h_shade, h_xf, h_itf;
+    std::vector<uint32_t> h_mats, h_faces, h_inst, h_sides, h_tf, h_fi, h_prefix, h_ds, h_face_scene, h_inst_base;
+    std::vector<SceneRec> recs(count);
+    h_nodes.reserve(nodes * nanbvh::kNodeWords);
+    h_tris.reserve(tris * nanbvh::kTriWords);
+    h_shade.reserve(tris * 20);
+    uint32_t node_off = 0, tri_off = 0, xf_off = 0, mat_off = 0, face_off = 0, inst_off = 0, front_off = 0;
+    auto rebase_entry = [&](uint32_t e) -> uint32_t {
+        if (e == 0xFFFFFFFFu) return e;
+        if (e & nanbvh::kLeafBit) return (e & ~nanbvh::kLeafFirstMask) | ((e & nanbvh::kLeafFirstMask) + tri_off);
+        return e + node_off;
+    };
+    for (uint32_t k = 0; k < count; ++k) {
+        const NanScene *sc = scenes[k];
+        const NanTracer *tr = tracers[k];
+        size_t n0 = h_nodes.size();
+        h_nodes.insert(h_nodes.end(), sc->h_nodes.begin(), sc->h_nodes.end());
+        for (size_t i = n0; i < h_nodes.size(); i += nanbvh::kNodeWords)
+            for (int c = 0; c < 4; ++c) h_nodes[i + 10 + c] = rebase_entry(h_nodes[i + 10 + c]);
+        size_t t0 = h_tris.size();
+        h_tris.insert(h_tris.end(), sc->h_tris.begin(), sc->h_tris.end());
+        for (size_t i = t0; i < h_tris.size(); i += nanbvh::kTriWords) {
+            uint32_t id;
+            std::memcpy(&id, &h_tris[i + 9], 4);
+            id += tri_off;
+            std::memcpy(&h_tris[i + 9], &id, 4);
+        }
+        size_t s0 = h_shade.size();
+        h_shade.insert(h_shade.end(), sc->h_shade.begin(), sc->h_shade.end());
+        for (size_t i = s0; i < h_shade.size(); i += 20) {
+            uint32_t xf, mw;
+            std::memcpy(&xf, &h_shade[i + 18], 4);
+            std::memcpy(&mw, &h_shade[i + 19], 4);
+            xf += xf_off;
+            mw = (mw & 0x80000000u) | ((mw & 0x7FFFFFFFu) + mat_off);
+            std::memcpy(&h_shade[i + 18], &xf, 4);
+            std::memcpy(&h_shade[i + 19], &mw, 4);
+        }
+        h_xf.insert(h_xf.end(), sc->h_xf.begin(), sc->h_xf.end());
+        h_itf.insert(h_itf.end(), sc->h_itf.begin(), sc->h_itf.end());
+        h_mats.insert(h_mats.end(), sc->h_mats.begin(), sc->h_mats.end());
+        recs[k] = SceneRec{};
+        recs[k].root = rebase_entry(sc->dev.root);
+        recs[k].slack = sc->dev.slack;
+        recs[k].max_bounces = 0;
+        recs[k].light = sc->dev.light;
+        // layout
+        h_faces.insert(h_faces.end(), tr->h_faces.begin(), tr->h_faces.end());
+        for (uint32_t i = 0; i < tr->instance_count; ++i) {
+            const uint32_t *ii = &tr->h_instances[4 * (size_t)i];
+            h_inst.push_back(ii[0] + face_off);
+            h_inst.push_back(ii[1]);
+            h_inst.push_back(ii[2]);
+            h_inst.push_back(ii[3]);
+            h_sides.push_back(tr->h_sides[2 * (size_t)i]);
+            h_sides.push_back(tr->h_sides[2 * (size_t)i + 1]);
+            h_prefix.push_back(tr->h_prefix[i] + front_off);
+        }
+        for (uint32_t f : tr->h_texel_face) h_tf.push_back(f + face_off);
+        for (uint32_t i : tr->h_face_inst) h_fi.push_back(i + inst_off);
+        for (uint32_t w : tr->h_ds) h_ds.push_back(w + front_off);
+        h_face_scene.insert(h_face_scene.end(), tr->face_count, k);
+        h_inst_base.push_back(inst_off);
+        node_off += (uint32_t)(sc->h_nodes.size() / nanbvh::kNodeWords);
+        tri_off += (uint32_t)(sc->h_tris.size() / nanbvh::kTriWords);
+        xf_off += (uint32_t)(sc->h_xf.size() / 16);
+        mat_off += (uint32_t)(sc->h_mats.size() / 8);
+        face_off += tr->face_count;
+        inst_off += tr->instance_count;
+        front_off += (uint32_t)tr->total_front;
+    }
+    NanScene *bs = new NanScene();
+    bs->batch = true;
+    auto up = [&](auto **dst, const void *src, size_t bytes) -> int {
+        int rc = upload(dst, src, bytes, err, cap);
+        if (rc == 0) bs->allocations.push_back((void *)*dst);
+        return rc;
+    };
+    uint4 *d_nodes;
+    float4 *d_tris, *d_shade, *d_xf, *d_itf, *d_mats;
+    SceneRec *d_recs;
+    int rc = 0;
+    rc |= up(&d_nodes, h_nodes.data(), h_nodes.size() * 4);
+    rc |= up(&d_tris, h_tris.data(), h_tris.size() * 4);
+    rc |= up(&d_shade, h_shade.data(), h_shade.size() * 4);
+    rc |= up(&d_xf, h_xf.data(), h_xf.size() * 4);
+    rc |= up(&d_itf, h_itf.data(), h_itf.size() * 4);
+    rc |= up(&d_mats, h_mats.data(), h_mats.size() * 4);
+    rc |= up(&d_recs, recs.data(), recs.size() * sizeof(SceneRec));
+    if (rc) {
+        delete bs;
+        return -1;
+    }
+    bs->dev.nodes = d_nodes;
+    bs->dev.tris = d_tris;
+    bs->dev.tri_shading = d_shade;
+    bs->dev.transforms = d_xf;
+    bs->dev.inv_transposes = d_itf;
+    bs->dev.materials = d_mats;
+    bs->dev.scenes = d_recs;
+    bs->dev.root = recs[0].root;
+    bs->dev.slack = recs[0].slack;
+    bs->dev.light = recs[0].light;
+    bs->info.triangle_count = tris;
+    bs->info.node_count = nodes;
+    bs->info.max_stack = max_stack;
+    bs->stack_depth = 0;
+
+    NanTracer *t = new NanTracer();
+    t->scene = bs;
+    t->owned_scene = bs;
+    t->scene_count = count;
+    t->face_count = (uint32_t)faces;
+    t->instance_count = (uint32_t)insts;
+    t->total_front = front;
+    t->ds_count = (uint32_t)ds;
+    t->mode = NAN_TRACER_WAVEFRONT;
+    rc = 0;
+    rc |= upload(&t->faces, h_faces.data(), h_faces.size() * 4, err, cap);
+    rc |= upload(&t->instances, h_inst.data(), h_inst.size() * 4, err, cap);
+    rc |= upload(&t->sides, h_sides.data(), h_sides.size() * 4, err, cap);
+    rc |= upload(&t->texel_face, h_tf.data(), h_tf.size() * 4, err, cap);
+    rc |= upload(&t->face_inst, h_fi.data(), h_fi.size() * 4, err, cap);
+    rc |= upload(&t->front_prefix, h_prefix.data(), h_prefix.size() * 4, err, cap);
+    rc |= upload(&t->ds_list, h_ds.data(), h_ds.size() * 4, err, cap);
+    rc |= upload(&t->face_scene, h_face_scene.data(), h_face_scene.size() * 4, err, cap);
+    rc |= upload(&t->scene_inst_base, h_inst_base.data(), h_inst_base.size() * 4, err, cap);
+    rc |= upload(&t->payload_table, nullptr, (size_t)count * sizeof(float4 *), err, cap);
+    if (rc) {
+        delete t;
+        return -1;
+    }
+    t->payload_table_set = false;
+    *out = t;
+    return 0;
+}
+
+NAN_API int nan_batch_set_payloads(NanTracer *batch, void *const *payloads) {
+    if (!batch || !payloads || batch->scene_count < 2 || !batch->payload_table)
+        return fail(nullptr, 0, "nan_batch_set_payloads: not a batch or null payloads");
+    for (uint32_t k = 0; k < batch->scene_count; ++k)
+        if (!payloads[k]) return fail(nullptr, 0, "nan_batch_set_payloads: payload %u is null", k);
+    HIP_TRY(hipMemcpy(batch->payload_table, payloads, (size_t)batch->scene_count * sizeof(void *),
+                      hipMemcpyHostToDevice), nullptr, 0);
+    batch->payload_table_set = true;
+    return 0;
+}
+
+NAN_API int nan_batch_set_bounces(NanTracer *batch, const uint32_t *max_bounces) {
+    if (!batch || !max_bounces || !batch->scene->batch) return fail(nullptr, 0, "nan_batch_set_bounces: not a batch");
+    std::vector<SceneRec> recs(batch->scene_count);
+    HIP_TRY(hipMemcpy(recs.data(), batch->scene->dev.scenes, recs.size() * sizeof(SceneRec), hipMemcpyDeviceToHost),
+            nullptr, 0);
+    for (uint32_t k = 0; k < batch->scene_count; ++k) recs[k].max_bounces = max_bounces[k];
+    HIP_TRY(hipMemcpy((void *)batch->scene->dev.scenes, recs.data(), recs.size() * sizeof(SceneRec),
+                      hipMemcpyHostToDevice), nullptr, 0);
+    return 0;
+}
+
+NAN_API int nan_tracer_device_frame(NanTracer *tracer, int enable, uint32_t iteration, uint32_t reset) {
+    if (!tracer) return fail(nullptr, 0, "nan_tracer_device_frame: null tracer");
+    if (enable && tracer->mode != NAN_TRACER_WAVEFRONT)
+        return fail(nullptr, 0, "nan_tracer_device_frame: wavefront mode only");
+    tracer->device_frame = enable != 0;
+    if (enable) return wavefront_set_frame(tracer, iteration, reset);
+    return 0;
 }
 
 }  // extern "C"

@@ -104,6 +104,15 @@ struct Light {
     int sun_on;
 };
 
+// Per-scene record for kernels that trace several scenes in one launch.
+struct SceneRec {
+    uint32_t root;         // BVH root entry (rebased into the shared node array)
+    float slack;           // absolute box-test slack of the scene
+    uint32_t max_bounces;  // 0: use the launch's value
+    uint32_t pad;
+    Light light;
+};
+
 struct DevScene {
     const uint4 *nodes;           // 64-byte BVH4 nodes
     const float4 *tris;           // 48-byte triangle records, 3 float4 each
@@ -119,6 +128,7 @@ struct DevScene {
     uint32_t root;
     float slack;                  // absolute box-test slack (see kPad)
     Light light;
+    const SceneRec *scenes;       // one record per scene traced together (>= 1)
 };
 
 struct Ray {

@@ -53,6 +53,15 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 constexpr int kTraceBlock = 128;
 constexpr int kFlatBlock = 256;
 
+int wavefront_set_frame(NanTracer *t, uint32_t iteration, uint32_t reset);
+
+// Launch shape of the wavefront traversal kernels (nan_tracer_tune).
+struct WfTuning {
+    uint32_t blocks_per_cu = 0;  // persistent grid: 0 = occupancy limit
+    uint32_t ring = 16;          // LDS stack ring entries per lane: 16, 24 or 32
+    uint32_t refill_steps = 16;  // traversal steps between lane refills
+};
+
 }  // namespace nanx
 
 struct NanScene {
@@ -61,6 +70,11 @@ struct NanScene {
     NanSceneInfo info{};
     uint32_t stack_depth = 0;
     std::vector<uint32_t> instance_triangles;  // triangles of each instance's mesh
+    // host copies used to assemble batches of scenes
+    std::vector<uint32_t> h_nodes;
+    std::vector<float> h_tris, h_shade, h_xf, h_itf;
+    std::vector<uint32_t> h_mats;
+    bool batch = false;  // assembled from several scenes (wavefront only)
     ~NanScene() {
         for (void *p : allocations) hipFree(p);
     }
@@ -76,10 +90,21 @@ struct NanTracer {
     uint32_t *front_prefix = nullptr;
     uint32_t *ds_list = nullptr;      // front texels of double-sided instances
     uint32_t ds_count = 0;
+    // batches of scenes (nan_batch_create)
+    uint32_t scene_count = 1;
+    uint32_t *face_scene = nullptr;
+    uint32_t *scene_inst_base = nullptr;
+    bool device_frame = false;         // iteration/reset come from a device counter (graph replay)
+    float4 **payload_table = nullptr;  // device array of per-scene payload pointers
+    bool payload_table_set = false;
+    NanScene *owned_scene = nullptr;   // the batch's concatenated scene
+    // host copies of the layout tables (to assemble batches)
+    std::vector<uint32_t> h_faces, h_instances, h_sides, h_texel_face, h_face_inst, h_prefix, h_ds;
     uint32_t face_count = 0, instance_count = 0;
     uint64_t total_front = 0, total_payload = 0;
     int mode = 1;                     // 0 megakernel, 1 wavefront
     struct WfBuffers *wf = nullptr;   // wavefront working set, allocated on first use
+    nanx::WfTuning tuning;
     // stage timing: (stage, start, end) event triples recorded while profiling
     bool profile = false;
     std::vector<hipEvent_t> event_pool;
@@ -95,6 +120,10 @@ void wavefront_release(NanTracer *t);
 inline NanTracer::~NanTracer() {
     nanx::wavefront_release(this);
     for (hipEvent_t e : event_pool) hipEventDestroy(e);
+    hipFree(face_scene);
+    hipFree(scene_inst_base);
+    hipFree(payload_table);
+    delete owned_scene;
     {
         hipFree(ds_list);
         hipFree(faces);
@@ -108,12 +137,14 @@ inline NanTracer::~NanTracer() {
 
 struct DevLayout {
     const uint2 *faces;
-    const uint4 *instances;
-    const uint2 *sides;
-    const uint32_t *texel_face;
-    const uint32_t *face_inst;
-    const uint32_t *front_prefix;
+    const uint4 *instances;        // face_offset global; texel_offset scene-local
+    const uint2 *sides;            // back_texel_offset scene-local
+    const uint32_t *texel_face;    // global face of each flat front texel
+    const uint32_t *face_inst;     // global instance of each face
+    const uint32_t *front_prefix;  // flat index of each instance's first front texel
     uint64_t total_front;
+    const uint32_t *face_scene;       // scene of each face (null: one scene)
+    const uint32_t *scene_inst_base;  // first global instance of each scene (null: one scene)
 };
 
 
@@ -128,6 +159,8 @@ inline DevLayout dev_layout(const NanTracer *t) {
     L.face_inst = t->face_inst;
     L.front_prefix = t->front_prefix;
     L.total_front = t->total_front;
+    L.face_scene = t->face_scene;
+    L.scene_inst_base = t->scene_inst_base;
     return L;
 }
 
@@ -171,5 +204,6 @@ inline int check_launch(const char *what) {
 // Wavefront tracer (nan_wavefront.hip).
 int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                       uint32_t reset, unsigned long long *counters, hipStream_t stream);
+
 
 }  // namespace nanx

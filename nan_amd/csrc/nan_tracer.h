@@ -125,7 +125,7 @@ __device__ __forceinline__ void store_accumulated(float4 *payload, uint32_t slot
 }
 
 struct TexelSurface {
-    uint32_t instance, prim, tri, global_texel, local;
+    uint32_t instance, prim, tri, global_texel, local, scene;
     float b1, b2;
     uint2 side;
 };
@@ -140,7 +140,11 @@ __device__ __forceinline__ TexelSurface texel_surface(const DevLayout &Ly, uint6
     uint4 ii = Ly.instances[inst];
     uint2 face = Ly.faces[f];
     ts.side = Ly.sides[inst];
-    ts.instance = inst;
+    ts.scene = Ly.face_scene ? Ly.face_scene[f] : 0u;
+    // the RNG and the reported instance are scene-local (batched scenes keep
+    // their single-scene streams)
+    uint32_t inst_local = Ly.scene_inst_base ? inst - Ly.scene_inst_base[ts.scene] : inst;
+    ts.instance = inst_local;
     ts.prim = f - ii.x;
     ts.tri = f;  // layout faces follow the scene's global triangle order
     ts.local = (uint32_t)(w - Ly.front_prefix[inst]);
@@ -149,7 +153,7 @@ __device__ __forceinline__ TexelSurface texel_surface(const DevLayout &Ly, uint6
     address_to_ij(ts.local - face.x, face.y, i, j);
     float fr = (float)face.y;
     float base_x = (float)i / fr, base_y = (float)j / fr;
-    rng.seed(ts.global_texel, inst, iteration * (spp > 1u ? spp : 1u));
+    rng.seed(ts.global_texel, inst_local, iteration * (spp > 1u ? spp : 1u));
     float bx = base_x, by = base_y;
     for (int attempt = 0; attempt < 32; ++attempt) {
         float ux = rng.next_1d();

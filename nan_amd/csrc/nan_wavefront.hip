@@ -31,7 +31,7 @@ namespace {
 
 constexpr int kWfBlock = 256;       // stage kernels
 constexpr int kTravBlock = 256;     // traversal kernels
-constexpr int kLdsStack = 16;       // ring entries per lane in LDS
+constexpr int kLdsStackMax = 32;    // largest LDS ring (entries per lane)
 constexpr int kMaxStack = 128;      // deepest traversal the spill area holds
 constexpr uint32_t kTargetSun = 0u;
 constexpr uint32_t kTargetRad = 1u;
@@ -53,9 +53,12 @@ struct WfBuffers {
     float4 *sh_d = nullptr;     // [2 cap] direction.xyz, target bits
     float4 *sh_c = nullptr;     // [2 cap] contribution.xyz
     uint32_t *counters = nullptr;   // [kCounterSlots]
-    uint32_t *spill = nullptr;      // [(kMaxStack - kLdsStack) * spill_lanes]
+    uint32_t *frame = nullptr;      // [iteration, reset] in device-frame mode
+    uint32_t *spill = nullptr;      // [(kMaxStack - ring) * spill_lanes]
     uint32_t spill_lanes = 0;
     uint32_t trav_blocks = 0;       // persistent grid of the traversal kernels
+    uint32_t ring = 16;             // LDS ring entries per lane
+    uint32_t blocks_per_cu_req = 0; // requested persistent blocks per CU (0: occupancy limit)
     uint32_t stage_blocks = 0;      // grid of the stage kernels
 };
 
@@ -79,7 +82,7 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
     return base + (uint32_t)__popcll(below);
 }
 
-// Resumable traversal with a kLdsStack-entry LDS ring per lane that spills
+// Resumable traversal with a RING-entry LDS ring per lane that spills
 // its oldest entries to HBM.  One call of trav_step() processes one node or
 // one leaf; the visiting rule and the result equal traverse() in
 // nan_device.h (closest hit = smallest (t, triangle id); any-hit stops at
@@ -89,20 +92,21 @@ struct TravState {
     int sp;
 };
 
-template <bool ANY_HIT, bool COUNT>
-__device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, Hit &hit, TravState &ts, uint32_t *lds_col,
-                                          uint32_t *spill_col, uint32_t spill_stride, TraversalStats &st) {
+template <int RING, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, float slack, Hit &hit, TravState &ts,
+                                          uint32_t *lds_col, uint32_t *spill_col, uint32_t spill_stride,
+                                          TraversalStats &st) {
     auto push = [&](uint32_t e) {
-        uint32_t *slot = lds_col + (ts.sp & (kLdsStack - 1)) * kTravBlock;
-        if (ts.sp >= kLdsStack) spill_col[(size_t)(ts.sp - kLdsStack) * spill_stride] = *slot;
+        uint32_t *slot = lds_col + (ts.sp % RING) * kTravBlock;
+        if (ts.sp >= RING) spill_col[(size_t)(ts.sp - RING) * spill_stride] = *slot;
         *slot = e;
         ++ts.sp;
     };
     auto pop = [&]() -> uint32_t {
         --ts.sp;
-        uint32_t *slot = lds_col + (ts.sp & (kLdsStack - 1)) * kTravBlock;
+        uint32_t *slot = lds_col + (ts.sp % RING) * kTravBlock;
         uint32_t e = *slot;
-        if (ts.sp >= kLdsStack) *slot = spill_col[(size_t)(ts.sp - kLdsStack) * spill_stride];
+        if (ts.sp >= RING) *slot = spill_col[(size_t)(ts.sp - RING) * spill_stride];
         return e;
     };
     if (!(ts.cur & kLeafBit)) {
@@ -116,7 +120,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, Hit &
         uint32_t valid = w0.w >> 24;
         uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
         uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
-        float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack;
+        float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + slack;
         float tn[4];
         uint32_t ent[4];
         int nhit = 0;
@@ -132,7 +136,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, Hit &
             float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
             float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
             float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-            float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + S.slack;
+            float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + slack;
             far = fminf(far, limit);
             bool ok = ((valid >> c) & 1u) && near <= far;
             if (ok) {
@@ -202,13 +206,13 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, Hit &
 // ray has finished takes the next queued ray (one atomic per wave for all
 // refilling lanes), so a wave keeps tracing while any rays remain instead
 // of waiting for its slowest lane.
-template <bool ANY_HIT, bool COUNT, typename Load, typename Store>
+template <int RING, bool ANY_HIT, bool COUNT, typename Load, typename Store>
 __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t count, uint32_t *fetch, uint32_t *lds_col,
-                                            uint32_t *spill_col, uint32_t spill_stride, Load load, Store store,
-                                            uint32_t &n_rays, uint32_t &n_nodes, uint32_t &n_tris) {
-    constexpr int kSteps = 16;
+                                            uint32_t *spill_col, uint32_t spill_stride, uint32_t steps, Load load,
+                                            Store store, uint32_t &n_rays, uint32_t &n_nodes, uint32_t &n_tris) {
     Ray r;
     Hit h;
+    float slack = 0.0f;
     TravState ts{0, 0};
     TraversalStats st{0, 0};
     uint32_t k = 0;
@@ -228,10 +232,11 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t count, u
                     uint32_t kk = base + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
                     if (kk < count) {
                         k = kk;
-                        load(k, r);
+                        uint32_t root;
+                        load(k, r, root, slack);
                         ray_setup(r);
                         h = Hit{r.tmax, 0.0f, 0.0f, kInvalid};
-                        ts = TravState{S.root, 0};
+                        ts = TravState{root, 0};
                         st = TraversalStats{0, 0};
                         has = true;
                     }
@@ -240,8 +245,8 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t count, u
             }
         }
         if (__ballot(has) == 0ull) break;
-        for (int s = 0; s < kSteps && has; ++s) {
-            if (trav_step<ANY_HIT, COUNT>(S, r, h, ts, lds_col, spill_col, spill_stride, st)) {
+        for (uint32_t s = 0; s < steps && has; ++s) {
+            if (trav_step<RING, ANY_HIT, COUNT>(S, r, slack, h, ts, lds_col, spill_col, spill_stride, st)) {
                 store(k, h);
                 has = false;
                 if (COUNT) {
@@ -267,7 +272,16 @@ __global__ void wf_zero_counters(uint32_t *c, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0u;
 }
 
+// End of a frame in device-frame mode: next iteration, no reset.
+__global__ void wf_advance_frame(uint32_t *frame) {
+    if (threadIdx.x == 0) {
+        frame[0] += 1u;
+        frame[1] = 0u;
+    }
+}
+
 struct GenParams {
+    const uint32_t *frame; // device [iteration, reset] (null: use iteration below)
     const uint32_t *list;  // null: path i is front texel i
     uint32_t count;
     uint32_t side;         // 0 front, 1 back
@@ -276,8 +290,7 @@ struct GenParams {
     uint32_t ext_slot, sh_slot;  // counter slots of the queues written
 };
 
-__global__ __launch_bounds__(kWfBlock) void wf_generate(DevScene S, DevLayout Ly, WfBuffers W, GenParams P) {
-    const Light &L = S.light;
+__global__ __launch_bounds__(kWfBlock, 6) void wf_generate(DevScene S, DevLayout Ly, WfBuffers W, GenParams P) {
     uint32_t stride = gridDim.x * kWfBlock;
     uint32_t *ext_count = W.counters + P.ext_slot;
     uint32_t *sh_count = W.counters + P.sh_slot;
@@ -286,11 +299,13 @@ __global__ __launch_bounds__(kWfBlock) void wf_generate(DevScene S, DevLayout Ly
         bool active = i < P.count;
         bool want_shadow = false, want_ray = false;
         f3 origin = mk(0.0f, 0.0f, 0.0f), ld = origin, dir = origin, contrib = origin;
-        uint32_t w = 0;
+        uint32_t w = 0, scene = 0;
         if (active) {
             w = P.list ? P.list[i] : i;
             Rng rng;
-            TexelSurface ts = texel_surface(Ly, w, P.iteration, P.spp, rng);
+            TexelSurface ts = texel_surface(Ly, w, P.frame ? P.frame[0] : P.iteration, P.spp, rng);
+            scene = ts.scene;
+            const Light &L = S.scenes[scene].light;
             if (P.side != 0 || P.sample != 0) rng.state = W.rng[w];
             f3 n = world_flat_normal(S, ts.tri);
             Shading sd;
@@ -317,33 +332,37 @@ __global__ __launch_bounds__(kWfBlock) void wf_generate(DevScene S, DevLayout Ly
         if (active && want_shadow) {
             W.sh_o[qs] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
             W.sh_d[qs] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(kTargetSun));
-            W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+            W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(scene));
         }
         uint32_t qr = wave_append(active && want_ray, ext_count);
         if (active && want_ray) {
             W.ray_o[0][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
-            W.ray_d[0][qr] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+            W.ray_d[0][qr] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scene));
         }
     }
 }
 
-template <bool COUNT>
+template <int RING, bool COUNT>
 __global__ __launch_bounds__(kTravBlock, 8) void wf_extend(DevScene S, WfBuffers W, uint32_t q, uint32_t count_slot,
-                                                       uint32_t fetch_slot, unsigned long long *counters) {
-    __shared__ uint32_t stack[kLdsStack * kTravBlock];
+                                                          uint32_t fetch_slot, uint32_t steps,
+                                                          unsigned long long *counters) {
+    __shared__ uint32_t stack[RING * kTravBlock];
     uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
     const uint32_t count = W.counters[count_slot];
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
     const float4 *ro = W.ray_o[q], *rd = W.ray_d[q];
     float4 *hits = W.hits;
-    trace_queue<false, COUNT>(
-        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes,
-        [&](uint32_t k, Ray &r) {
+    trace_queue<RING, false, COUNT>(
+        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
+        [&](uint32_t k, Ray &r, uint32_t &root, float &slack) {
             float4 o = ro[k], d = rd[k];
             r.o = mk(o.x, o.y, o.z);
             r.d = mk(d.x, d.y, d.z);
             r.tmin = 0.0f;
             r.tmax = 1000.0f;
+            const SceneRec &sc = S.scenes[__float_as_uint(d.w)];
+            root = sc.root;
+            slack = sc.slack;
         },
         [&](uint32_t k, const Hit &h) { hits[k] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.id)); },
         n_rays, n_nodes, n_tris);
@@ -354,21 +373,25 @@ __global__ __launch_bounds__(kTravBlock, 8) void wf_extend(DevScene S, WfBuffers
     }
 }
 
-template <bool COUNT>
+template <int RING, bool COUNT>
 __global__ __launch_bounds__(kTravBlock, 8) void wf_shadow(DevScene S, WfBuffers W, uint32_t count_slot,
-                                                       uint32_t fetch_slot, unsigned long long *counters) {
-    __shared__ uint32_t stack[kLdsStack * kTravBlock];
+                                                          uint32_t fetch_slot, uint32_t steps,
+                                                          unsigned long long *counters) {
+    __shared__ uint32_t stack[RING * kTravBlock];
     uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
     const uint32_t count = W.counters[count_slot];
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
-    trace_queue<true, COUNT>(
-        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes,
-        [&](uint32_t k, Ray &r) {
+    trace_queue<RING, true, COUNT>(
+        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
+        [&](uint32_t k, Ray &r, uint32_t &root, float &slack) {
             float4 o = W.sh_o[k], d = W.sh_d[k];
             r.o = mk(o.x, o.y, o.z);
             r.d = mk(d.x, d.y, d.z);
             r.tmin = 0.0f;
             r.tmax = 1000.0f;
+            const SceneRec &sc = S.scenes[__float_as_uint(W.sh_c[k].w)];
+            root = sc.root;
+            slack = sc.slack;
         },
         [&](uint32_t k, const Hit &h) {
             if (h.id != kInvalid) return;  // occluded
@@ -399,8 +422,7 @@ struct ShadeParams {
 };
 
 // one iteration of trace_radiance's loop body (texture_space_path_tracer.slang:60-83)
-__global__ __launch_bounds__(kWfBlock) void wf_shade(DevScene S, WfBuffers W, ShadeParams P) {
-    const Light &L = S.light;
+__global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W, ShadeParams P) {
     const uint32_t count = W.counters[P.in_slot];
     uint32_t stride = gridDim.x * kWfBlock;
     for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += stride) {
@@ -408,11 +430,15 @@ __global__ __launch_bounds__(kWfBlock) void wf_shade(DevScene S, WfBuffers W, Sh
         bool active = k < count;
         bool want_shadow = false, want_ray = false;
         f3 origin = mk(0.0f, 0.0f, 0.0f), ld = origin, dir = origin, contrib = origin;
-        uint32_t p = 0;
+        uint32_t p = 0, scene = 0;
         if (active) {
             float4 ro = W.ray_o[P.q_in][k], rd = W.ray_d[P.q_in][k];
             float4 hv = W.hits[k];
             p = __float_as_uint(ro.w);
+            scene = __float_as_uint(rd.w);
+            const SceneRec &sc = S.scenes[scene];
+            const Light &L = sc.light;
+            uint32_t max_bounces = sc.max_bounces ? sc.max_bounces : P.max_bounces;
             float4 Tv = W.T[p], Rv = W.rad[p];
             f3 T = mk(Tv.x, Tv.y, Tv.z), rad = mk(Rv.x, Rv.y, Rv.z);
             uint32_t id = __float_as_uint(hv.w);
@@ -444,7 +470,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_shade(DevScene S, WfBuffers W, Sh
                     T = T * mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf);
                     origin = sd.origin_front;
                     dir = to_global(sd.frame, sdir);
-                    want_ray = P.bounce < P.max_bounces;
+                    want_ray = P.bounce < max_bounces;
                 }
                 W.rng[p] = rng.state;
                 W.T[p] = make_float4(T.x, T.y, T.z, Tv.w);
@@ -455,19 +481,21 @@ __global__ __launch_bounds__(kWfBlock) void wf_shade(DevScene S, WfBuffers W, Sh
         if (active && want_shadow) {
             W.sh_o[qs] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(p));
             W.sh_d[qs] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(kTargetRad));
-            W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+            W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(scene));
         }
         uint32_t qr = wave_append(active && want_ray, W.counters + P.out_slot);
         if (active && want_ray) {
             W.ray_o[P.q_out][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(p));
-            W.ray_d[P.q_out][qr] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+            W.ray_d[P.q_out][qr] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scene));
         }
     }
 }
 
 struct FinishParams {
+    const uint32_t *frame; // device [iteration, reset] (null: use reset below)
     const uint32_t *list;
     uint32_t count, side, sample, spp, reset;
+    float4 *const *payloads;  // per-scene payload (null: the launch's payload)
 };
 
 template <bool COUNT>
@@ -498,7 +526,8 @@ __global__ __launch_bounds__(kWfBlock) void wf_finish(DevLayout Ly, WfBuffers W,
                 uint32_t inst = Ly.face_inst[f];
                 uint32_t local = w - Ly.front_prefix[inst];
                 uint32_t slot = P.side ? Ly.sides[inst].x + local : Ly.instances[inst].z + local;
-                store_accumulated(payload, slot, e, P.reset);
+                float4 *dst = P.payloads ? P.payloads[Ly.face_scene ? Ly.face_scene[f] : 0u] : payload;
+                store_accumulated(dst, slot, e, P.frame ? P.frame[1] : P.reset);
                 slots = 1;
             }
         }
@@ -528,19 +557,32 @@ void wavefront_release(NanTracer *t) {
     hipFree(W->sh_d);
     hipFree(W->sh_c);
     hipFree(W->counters);
+    hipFree(W->frame);
     hipFree(W->spill);
     delete W;
     t->wf = nullptr;
 }
 
+template <int RING>
+static int occupancy_blocks(int *per_cu) {
+    int a = 0, b = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_extend<RING, false>, kTravBlock, 0), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_shadow<RING, false>, kTravBlock, 0), nullptr, 0);
+    *per_cu = std::max(1, std::min(a, b));
+    return 0;
+}
+
 static int wavefront_alloc(NanTracer *t, uint32_t spp) {
-    if (t->wf && (spp <= 1 || t->wf->acc)) return 0;
-    if (t->wf && spp > 1 && !t->wf->acc) {
-        HIP_TRY(hipMalloc((void **)&t->wf->acc, (size_t)t->wf->capacity * 16), nullptr, 0);
+    const WfTuning &tu = t->tuning;
+    if (t->wf && (t->wf->ring != tu.ring || t->wf->blocks_per_cu_req != tu.blocks_per_cu)) wavefront_release(t);
+    if (t->wf) {
+        if (spp > 1 && !t->wf->acc) HIP_TRY(hipMalloc((void **)&t->wf->acc, (size_t)t->wf->capacity * 16), nullptr, 0);
         return 0;
     }
     WfBuffers *W = new WfBuffers();
     t->wf = W;
+    W->ring = tu.ring;
+    W->blocks_per_cu_req = tu.blocks_per_cu;
     uint32_t cap = (uint32_t)std::max<uint64_t>(t->total_front, 1);
     W->capacity = cap;
     size_t c4 = (size_t)cap * 16;
@@ -558,17 +600,41 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     HIP_TRY(hipMalloc((void **)&W->sh_d, 2 * c4), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&W->sh_c, 2 * c4), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&W->counters, kCounterSlots * 4), nullptr, 0);
-    int dev = 0, cus = 0, per_cu_ext = 0, per_cu_sh = 0;
+    HIP_TRY(hipMalloc((void **)&W->frame, 2 * 4), nullptr, 0);
+    int dev = 0, cus = 0, per_cu = 1;
     HIP_TRY(hipGetDevice(&dev), nullptr, 0);
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), nullptr, 0);
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_ext, wf_extend<false>, kTravBlock, 0), nullptr, 0);
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_sh, wf_shadow<false>, kTravBlock, 0), nullptr, 0);
-    int per_cu = std::max(1, std::max(per_cu_ext, per_cu_sh));
+    int rc = W->ring == 16 ? occupancy_blocks<16>(&per_cu)
+             : W->ring == 24 ? occupancy_blocks<24>(&per_cu) : occupancy_blocks<32>(&per_cu);
+    if (rc) return rc;
+    if (tu.blocks_per_cu) per_cu = std::min(per_cu, (int)tu.blocks_per_cu);
     W->trav_blocks = (uint32_t)(cus * per_cu);
     W->stage_blocks = (uint32_t)(cus * 8);
     W->spill_lanes = W->trav_blocks * kTravBlock;
-    HIP_TRY(hipMalloc((void **)&W->spill, (size_t)(kMaxStack - kLdsStack) * W->spill_lanes * 4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->spill, (size_t)(kMaxStack - W->ring) * W->spill_lanes * 4), nullptr, 0);
     return 0;
+}
+
+template <int RING>
+static void launch_extend(const WfBuffers &W, const DevScene &S, uint32_t q, uint32_t count_slot, uint32_t fetch,
+                          uint32_t steps, unsigned long long *counters, hipStream_t stream) {
+    if (counters)
+        hipLaunchKernelGGL((wf_extend<RING, true>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
+                           count_slot, fetch, steps, counters);
+    else
+        hipLaunchKernelGGL((wf_extend<RING, false>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
+                           count_slot, fetch, steps, counters);
+}
+
+template <int RING>
+static void launch_shadow(const WfBuffers &W, const DevScene &S, uint32_t count_slot, uint32_t fetch, uint32_t steps,
+                          unsigned long long *counters, hipStream_t stream) {
+    if (counters)
+        hipLaunchKernelGGL((wf_shadow<RING, true>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
+                           count_slot, fetch, steps, counters);
+    else
+        hipLaunchKernelGGL((wf_shadow<RING, false>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
+                           count_slot, fetch, steps, counters);
 }
 
 int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
@@ -576,11 +642,14 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
     if (t->scene->info.max_stack > (uint32_t)kMaxStack)
         return fail(nullptr, 0, "wavefront: BVH needs a %u-entry stack (max %d)", t->scene->info.max_stack,
                     kMaxStack);
+    if (t->scene_count > 1 && !t->payload_table_set)
+        return fail(nullptr, 0, "wavefront: batch payloads are not set (nan_batch_set_payloads)");
     if (wavefront_alloc(t, spp)) return -1;
     WfBuffers &W = *t->wf;
     const DevScene &S = t->scene->dev;
     DevLayout Ly = dev_layout(t);
     uint32_t nspp = spp > 1 ? spp : 1;
+    uint32_t steps = std::max(1u, t->tuning.refill_steps);
     uint32_t slot = 0;  // next free counter slot
     hipLaunchKernelGGL(wf_zero_counters, dim3(1), dim3(1024), 0, stream, W.counters, kCounterSlots);
     for (uint32_t side = 0; side < 2; ++side) {
@@ -591,7 +660,7 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
         for (uint32_t k = 0; k < nspp; ++k) {
             if (slot + 4 * bounces + 8 > kCounterSlots) return fail(nullptr, 0, "wavefront: too many stages");
             uint32_t ext_slot = slot++, sh_slot = slot++;
-            GenParams gp{list, count, side, k, iteration, spp, ext_slot, sh_slot};
+            GenParams gp{t->device_frame ? W.frame : nullptr, list, count, side, k, iteration, spp, ext_slot, sh_slot};
             {
                 StageTimer tm(t, stream, kStageGenerate);
                 hipLaunchKernelGGL(wf_generate, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, Ly, W, gp);
@@ -600,13 +669,10 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
             for (uint32_t b = 1; b <= bounces; ++b) {
                 uint32_t fetch = slot++;
                 {
-                StageTimer tm(t, stream, kStageExtend);
-                if (counters)
-                    hipLaunchKernelGGL(wf_extend<true>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
-                                       ext_slot, fetch, counters);
-                else
-                    hipLaunchKernelGGL(wf_extend<false>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
-                                       ext_slot, fetch, counters);
+                    StageTimer tm(t, stream, kStageExtend);
+                    if (W.ring == 16) launch_extend<16>(W, S, q, ext_slot, fetch, steps, counters, stream);
+                    else if (W.ring == 24) launch_extend<24>(W, S, q, ext_slot, fetch, steps, counters, stream);
+                    else launch_extend<32>(W, S, q, ext_slot, fetch, steps, counters, stream);
                 }
                 uint32_t next_ext = slot++;
                 ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces};
@@ -616,19 +682,17 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                 }
                 uint32_t sfetch = slot++;
                 {
-                StageTimer tm(t, stream, kStageShadow);
-                if (counters)
-                    hipLaunchKernelGGL(wf_shadow<true>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
-                                       sh_slot, sfetch, counters);
-                else
-                    hipLaunchKernelGGL(wf_shadow<false>, dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
-                                       sh_slot, sfetch, counters);
+                    StageTimer tm(t, stream, kStageShadow);
+                    if (W.ring == 16) launch_shadow<16>(W, S, sh_slot, sfetch, steps, counters, stream);
+                    else if (W.ring == 24) launch_shadow<24>(W, S, sh_slot, sfetch, steps, counters, stream);
+                    else launch_shadow<32>(W, S, sh_slot, sfetch, steps, counters, stream);
                 }
                 sh_slot = slot++;
                 ext_slot = next_ext;
                 q = 1u - q;
             }
-            FinishParams fp{list, count, side, k, nspp, reset};
+            FinishParams fp{t->device_frame ? W.frame : nullptr, list, count, side, k, nspp, reset,
+                            t->scene_count > 1 ? t->payload_table : nullptr};
             StageTimer tm(t, stream, kStageFinish);
             if (counters)
                 hipLaunchKernelGGL(wf_finish<true>, dim3(gen_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
@@ -638,7 +702,15 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                                    counters);
         }
     }
+    if (t->device_frame) hipLaunchKernelGGL(wf_advance_frame, dim3(1), dim3(64), 0, stream, W.frame);
     return check_launch("wavefront");
+}
+
+int wavefront_set_frame(NanTracer *t, uint32_t iteration, uint32_t reset) {
+    if (wavefront_alloc(t, 1)) return -1;
+    uint32_t v[2] = {iteration, reset ? 1u : 0u};
+    HIP_TRY(hipMemcpy(t->wf->frame, v, sizeof(v), hipMemcpyHostToDevice), nullptr, 0);
+    return 0;
 }
 
 }  // namespace nanx

@@ -139,6 +139,49 @@ def world_positions_f32(transform: np.ndarray, positions: np.ndarray) -> np.ndar
     return out
 
 
+def instance_max_edges(scene_node: SceneNode) -> list[np.ndarray]:
+    """Longest world-space edge of every triangle, per instance (float64 norms
+    of the float32 ``M * (p, 1)`` positions, mesh_colors.py:215-224).
+    Instances sharing a mesh are transformed together."""
+    groups: dict[int, list[int]] = {}
+    for k, (mesh_id, _, _) in enumerate(scene_node.instances):
+        groups.setdefault(mesh_id, []).append(k)
+    out: list[np.ndarray | None] = [None] * len(scene_node.instances)
+    for mesh_id, members in groups.items():
+        mesh = scene_node.meshes[mesh_id]
+        p = mesh.vertices[:, 0:3].astype(np.float32)
+        idx = mesh.indices.astype(np.int64)
+        for start in range(0, len(members), max(1, (1 << 22) // max(1, len(p)))):
+            chunk = members[start:start + max(1, (1 << 22) // max(1, len(p)))]
+            m = np.stack([np.asarray(scene_node.transforms[scene_node.instances[k][2]].matrix, dtype=np.float32)
+                          for k in chunk])
+            world = np.empty((len(chunk), len(p), 3), dtype=np.float32)
+            for r in range(3):
+                acc = m[:, r, 0, None] * p[None, :, 0]
+                acc = acc + m[:, r, 1, None] * p[None, :, 1]
+                acc = acc + m[:, r, 2, None] * p[None, :, 2]
+                acc = acc + m[:, r, 3, None]
+                world[:, :, r] = acc
+            w = world.astype(np.float64)
+            a, b, c = w[:, idx[:, 0]], w[:, idx[:, 1]], w[:, idx[:, 2]]
+            e0 = np.sqrt(np.sum((b - a) ** 2, axis=2))
+            e1 = np.sqrt(np.sum((c - b) ** 2, axis=2))
+            e2 = np.sqrt(np.sum((a - c) ** 2, axis=2))
+            me = np.maximum(np.maximum(e0, e1), e2)
+            for row, k in enumerate(chunk):
+                out[k] = me[row]
+    return out  # type: ignore[return-value]
+
+
+def _resolutions(max_edge: np.ndarray, tpu: float, min_res: int, max_res: int) -> np.ndarray:
+    """``min(max_res, nextpow2(max(min_res, ceil(max_edge * tpu))))``
+    (mesh_colors.py:225-228)."""
+    cells = np.ceil(max_edge * tpu)
+    cells = np.maximum(cells, float(min_res))
+    cells = np.minimum(cells, float(max_res))
+    return np.minimum(_next_pow2_array(np.maximum(cells, 1).astype(np.int64)), max_res).astype(np.int64)
+
+
 class MeshColorsLayout:
     """Face, instance and side tables of one scene (mesh_colors.py:121-297).
 
@@ -220,22 +263,14 @@ class MeshColorsLayout:
         total_surface = 0
         total_payload = 0
         face_count_so_far = 0
-        # Resolutions depend only on (mesh, transform); cache them.
-        res_cache: dict[tuple[int, int], np.ndarray] = {}
+        max_edges = instance_max_edges(scene_node)
 
         for mesh_id, material_id, transform_id in scene_node.instances:
             mesh = scene_node.meshes[mesh_id]
             material = scene_node.materials[material_id]
             double_sided = bool(getattr(material, "double_sided", False))
             mult = 2 if double_sided else 1
-            key = (mesh_id, transform_id)
-            res = res_cache.get(key)
-            if res is None:
-                res = cls._unclamped_resolutions(
-                    mesh, scene_node.transforms[transform_id].matrix,
-                    texels_per_unit, min_resolution, max_resolution)
-                res_cache[key] = res
-            res = res.copy()
+            res = _resolutions(max_edges[len(instance_rows)], texels_per_unit, min_resolution, max_resolution)
             patch = (res + 1) * (res + 2) // 2
             local_end = np.cumsum(patch)  # local address after each triangle
             over = total_payload + local_end * mult > max_total_texels
@@ -285,21 +320,24 @@ class MeshColorsLayout:
             total_surface, total_payload, texels_per_unit, min_resolution, max_resolution,
         )
 
-    @staticmethod
-    def _unclamped_resolutions(mesh, matrix, tpu, min_res, max_res) -> np.ndarray:
-        """``min(max_res, nextpow2(max(min_res, ceil(max_edge * tpu))))`` per
-        triangle (mesh_colors.py:215-228)."""
-        world = world_positions_f32(matrix, mesh.vertices[:, 0:3]).astype(np.float64)
-        tri = world[mesh.indices.astype(np.int64)]  # [T,3,3]
-        e0 = np.sqrt(np.sum((tri[:, 1] - tri[:, 0]) ** 2, axis=1))
-        e1 = np.sqrt(np.sum((tri[:, 2] - tri[:, 1]) ** 2, axis=1))
-        e2 = np.sqrt(np.sum((tri[:, 0] - tri[:, 2]) ** 2, axis=1))
-        max_edge = np.maximum(np.maximum(e0, e1), e2)
-        cells = np.ceil(max_edge * tpu)
-        cells = np.maximum(cells, float(min_res))
-        cells = np.minimum(cells, float(1 << 40))
-        res = _next_pow2_array(np.maximum(cells, 1).astype(np.int64))
-        return np.minimum(res, max_res).astype(np.int64)
+    @classmethod
+    def natural_demand(cls, scene_node: SceneNode, *, texels_per_unit: float = 16.0, min_resolution: int = 4,
+                       max_resolution: int = 64, max_edges: list | None = None) -> int:
+        """Payload slots the layout needs when no triangle is halved for the
+        budget (build succeeds without halving iff this fits the budget).
+        ``max_edges`` (from :func:`instance_max_edges`) may be passed to
+        evaluate many parameter sets on one scene."""
+        texels_per_unit = max(float(texels_per_unit), 1e-6)
+        min_resolution = next_power_of_two(max(1, int(min_resolution)))
+        max_resolution = previous_power_of_two(max(1, int(max_resolution)))
+        if max_edges is None:
+            max_edges = instance_max_edges(scene_node)
+        total = 0
+        for (mesh_id, material_id, _), edges in zip(scene_node.instances, max_edges):
+            res = _resolutions(edges, texels_per_unit, min_resolution, max_resolution)
+            mult = 2 if getattr(scene_node.materials[material_id], "double_sided", False) else 1
+            total += int(((res + 1) * (res + 2) // 2).sum()) * mult
+        return total
 
     def summary(self) -> str:
         """The reference's one-line layout report

@@ -101,6 +101,12 @@ class TextureSpacePathTracer:
                                                        stream)
         _native.check(rc, "nan_tst_execute")
 
+    def tune(self, blocks_per_cu: int = 0, lds_stack: int = 16, refill_steps: int = 16) -> None:
+        """Launch shape of the traversal kernels (results do not depend on it)."""
+        _native.check(_native.lib().nan_tracer_tune(self.handle, int(blocks_per_cu), int(lds_stack),
+                                                    int(refill_steps)), "nan_tracer_tune")
+        self.tuning = (int(blocks_per_cu), int(lds_stack), int(refill_steps))
+
     def profile(self, enable: bool) -> None:
         """Bracket every kernel launch with HIP events (see stage_times)."""
         _native.check(_native.lib().nan_tracer_profile(self.handle, 1 if enable else 0), "nan_tracer_profile")
