@@ -23,7 +23,7 @@ extern "C" {
 
 #define NAN_API __attribute__((visibility("default")))
 
-#define NAN_HIP_ABI_VERSION 1
+#define NAN_HIP_ABI_VERSION 2
 
 /* Scene input: the buffers the reference's Scene uploads (scene.py:330-437).
  *   vertices        float[V][8]  object-space position, normal, uv
@@ -131,7 +131,11 @@ NAN_API int nan_tracer_set_mode(NanTracer *tracer, int mode);
  * and launch counts (uint32[NAN_STAGE_COUNT], may be NULL), then clears them.
  * No reference counterpart (the reference has no GPU timers on this path). */
 enum { NAN_STAGE_GENERATE = 0, NAN_STAGE_EXTEND, NAN_STAGE_SHADE, NAN_STAGE_SHADOW, NAN_STAGE_FINISH,
-       NAN_STAGE_MEGAKERNEL, NAN_STAGE_COUNT };
+       NAN_STAGE_MEGAKERNEL,
+       /* learned form (nan_tst_execute_learned): training paths, cache training
+        * step, first bounce of every texel sample, cache queries */
+       NAN_STAGE_LEARNED_PATHS, NAN_STAGE_CACHE_TRAIN, NAN_STAGE_FIRST_BOUNCE, NAN_STAGE_CACHE_QUERY,
+       NAN_STAGE_COUNT };
 NAN_API int nan_tracer_profile(NanTracer *tracer, int enable);
 NAN_API int nan_tracer_stage_times(NanTracer *tracer, double *ms, uint32_t *launches);
 
@@ -194,6 +198,89 @@ NAN_API int nan_resolve(NanTracer *tracer, const void *cache, int frozen, const 
  * (tone_mapper.slang:8-90) over n float4 pixels. */
 NAN_API int nan_accumulate(const float *input, float *accum, float *output, uint64_t n, int reset, void *stream);
 NAN_API int nan_tonemap(const float *input, float *output, uint64_t n, float exposure, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Learned radiance cache.  No reference counterpart: the reference has no
+ * learned cache; these entry points serve the learned form of the texel
+ * tracer (paths end in the cache after one bounce) and its tests.
+ *
+ * Network: multiresolution hash-grid encoding (levels x 2 features, 2^log2
+ * entries per level, FP16) of the position in the box [bbox_min, bbox_max],
+ * concatenated with normal and albedo, then hidden_layers ReLU layers of
+ * width and a linear RGB output, FP16 on MFMA with FP32 accumulation, no
+ * biases.  Trained with Adam on the relative L2 loss
+ * sum_c (p_c - t_c)^2 / (p_c^2 + 0.01) (p_c not differentiated in the
+ * denominator) averaged over valid records and channels.
+ * ------------------------------------------------------------------------ */
+typedef struct NanNrc NanNrc;
+
+typedef struct NanNrcConfig {
+    uint32_t levels;              /* 1..16 */
+    uint32_t features_per_level;  /* 2 */
+    uint32_t log2_table_size;     /* entries per level, 8..24 */
+    float base_resolution;        /* coarsest grid resolution */
+    float max_resolution;         /* finest grid resolution */
+    float bbox_min[3], bbox_max[3];
+    uint32_t width;               /* 64 */
+    uint32_t hidden_layers;       /* 4 */
+    float learning_rate, beta1, beta2, epsilon;
+    uint32_t seed;                /* parameter initialisation */
+    uint32_t dense_table_adam;    /* 0: table entries with zero gradient are not updated */
+    uint32_t minibatches;         /* Adam steps per training call (records split evenly), >= 1 */
+} NanNrcConfig;
+
+NAN_API int nan_nrc_create(const NanNrcConfig *config, NanNrc **out, char *err, size_t cap);
+NAN_API void nan_nrc_destroy(NanNrc *nrc);
+/* Number of MLP weights and hash-table parameters. */
+NAN_API int nan_nrc_param_counts(const NanNrc *nrc, uint64_t *mlp_weights, uint64_t *table_params);
+/* FP32 master parameters (host memory, either may be NULL): weights per layer
+ * as [out][in] (48->64, 3 x 64->64, 64->16), table as [level][entry][2]. */
+NAN_API int nan_nrc_get_params(const NanNrc *nrc, float *weights, float *table);
+NAN_API int nan_nrc_set_params(NanNrc *nrc, const float *weights, const float *table);
+/* Predicted RGB radiance (clamped at 0) for n queries: position, normal and
+ * albedo float[n][3], out float[n][3]; device memory. */
+NAN_API int nan_nrc_infer(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out, uint32_t n,
+                          void *stream);
+/* Training on n records with targets float[n][3]: forward, loss, backward and
+ * Adam, once per minibatch (config.minibatches consecutive slices); adds the
+ * last step's mean loss to *loss_out (device float, may be NULL). */
+NAN_API int nan_nrc_train(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, const float *target,
+                          uint32_t n, float *loss_out, void *stream);
+/* Testing hooks: the raw first four network outputs float[n][4], and the
+ * gradients of the mean loss for targets float[n][3] with respect to the MLP
+ * weights (float[mlp_weights], layout as nan_nrc_get_params) and to the
+ * network input float[n][48] (hash features, normal, albedo, padding);
+ * parameters are not updated. */
+NAN_API int nan_nrc_forward_raw(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out,
+                                uint32_t n, void *stream);
+NAN_API int nan_nrc_backward_raw(NanNrc *nrc, const float *pos, const float *nrm, const float *alb,
+                                 const float *target, uint32_t n, float *weight_grad, float *input_grad,
+                                 void *stream);
+/* Mean loss and valid record count of the last training step (host floats;
+ * waits for the device). */
+NAN_API int nan_nrc_stats(NanNrc *nrc, float *loss_and_count2);
+/* Encoded network input float[n][48] (hash features, normal, albedo, zeros). */
+NAN_API int nan_nrc_encode(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out, uint32_t n,
+                           void *stream);
+
+/* Learned form of nan_tst_execute: first trains the cache on train_samples
+ * plain path suffixes (texels drawn uniformly, plain max_bounces paths whose
+ * radiance leaving the first hit, emission excluded, is the target), then
+ * runs one iteration in which every texel sample takes the sun term and one
+ * cosine-sampled bounce as nan_tst_execute does and ends in the cache:
+ * E = sun + (az / pdf) * (Le(y1) + max(0, cache(y1))).  With
+ * NAN_LEARNED_EMITTER_NEE in flags, emissive triangles are also light-sampled
+ * (uniform in area, one shadow ray) at x0 and at every suffix vertex that has
+ * a successor, combined with the bounce-sampled emission by the power
+ * heuristic; the estimate's expectation is unchanged.  With
+ * NAN_LEARNED_RAMPED_MEAN, the k-th iteration since the reset enters the slot
+ * with weight proportional to k instead of 1 (the cache improves with the
+ * iterations).  counters as in nan_tst_execute_counted (may be NULL).  No
+ * reference counterpart. */
+enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2 };
+NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payload, uint32_t iteration,
+                                    uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
+                                    uint32_t train_samples, uint32_t flags, uint64_t *counters, void *stream);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,18 @@ class NanSceneInfo(ctypes.Structure):
     ]
 
 
+class NanNrcConfig(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_uint32), ("features_per_level", ctypes.c_uint32), ("log2_table_size", ctypes.c_uint32),
+        ("base_resolution", ctypes.c_float), ("max_resolution", ctypes.c_float),
+        ("bbox_min", ctypes.c_float * 3), ("bbox_max", ctypes.c_float * 3),
+        ("width", ctypes.c_uint32), ("hidden_layers", ctypes.c_uint32),
+        ("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+        ("epsilon", ctypes.c_float), ("seed", ctypes.c_uint32), ("dense_table_adam", ctypes.c_uint32),
+        ("minibatches", ctypes.c_uint32),
+    ]
+
+
 COUNTER_EXTEND_RAYS = 0
 COUNTER_SHADOW_RAYS = 1
 COUNTER_NODE_VISITS = 2
@@ -61,7 +73,11 @@ COUNTER_SHADOW_NODE_VISITS = 5
 COUNTER_SHADOW_TRI_TESTS = 6
 COUNTER_COUNT = 8
 
-STAGES = ("generate", "extend", "shade", "shadow", "finish", "megakernel")
+STAGES = ("generate", "extend", "shade", "shadow", "finish", "megakernel", "learned_paths", "cache_train",
+          "first_bounce", "cache_query")
+
+LEARNED_EMITTER_NEE = 1
+LEARNED_RAMPED_MEAN = 2
 
 TRACER_MEGAKERNEL = 0
 TRACER_WAVEFRONT = 1
@@ -73,6 +89,9 @@ EXPORTED_SYMBOLS = (
     "nan_tracer_set_mode", "nan_tracer_profile", "nan_tracer_stage_times", "nan_tst_execute", "nan_tst_execute_counted", "nan_tst_raygen", "nan_rgb9e5_pack", "nan_rgb9e5_unpack",
     "nan_resolve", "nan_accumulate", "nan_tonemap", "nan_tracer_tune", "nan_batch_create",
     "nan_batch_set_payloads", "nan_batch_set_bounces", "nan_tracer_device_frame",
+    "nan_nrc_create", "nan_nrc_destroy", "nan_nrc_param_counts", "nan_nrc_get_params", "nan_nrc_set_params",
+    "nan_nrc_infer", "nan_nrc_train", "nan_nrc_forward_raw", "nan_nrc_backward_raw", "nan_nrc_encode",
+    "nan_nrc_stats", "nan_tst_execute_learned",
 )
 
 _LIB = None
@@ -112,6 +131,18 @@ def _declare(lib) -> None:
         "nan_batch_set_payloads": (I, [vp, ctypes.POINTER(vp)]),
         "nan_batch_set_bounces": (I, [vp, u32p]),
         "nan_tracer_device_frame": (I, [vp, I, U32, U32]),
+        "nan_nrc_create": (I, [ctypes.POINTER(NanNrcConfig), ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]),
+        "nan_nrc_destroy": (None, [vp]),
+        "nan_nrc_param_counts": (I, [vp, u64p, u64p]),
+        "nan_nrc_get_params": (I, [vp, vp, vp]),
+        "nan_nrc_set_params": (I, [vp, vp, vp]),
+        "nan_nrc_infer": (I, [vp, vp, vp, vp, vp, U32, vp]),
+        "nan_nrc_train": (I, [vp, vp, vp, vp, vp, U32, vp, vp]),
+        "nan_nrc_forward_raw": (I, [vp, vp, vp, vp, vp, U32, vp]),
+        "nan_nrc_backward_raw": (I, [vp, vp, vp, vp, vp, U32, vp, vp, vp]),
+        "nan_nrc_encode": (I, [vp, vp, vp, vp, vp, U32, vp]),
+        "nan_nrc_stats": (I, [vp, f32p]),
+        "nan_tst_execute_learned": (I, [vp, vp, vp, U32, U32, U32, U32, U32, U32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

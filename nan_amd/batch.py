@@ -105,8 +105,8 @@ class SceneBatch:
         _native.check(_native.lib().nan_tracer_profile(self.handle, 1 if enable else 0), "nan_tracer_profile")
 
     def stage_times(self) -> dict:
-        ms = (ctypes.c_double * 8)()
-        n = (ctypes.c_uint32 * 8)()
+        ms = (ctypes.c_double * len(_native.STAGES))()
+        n = (ctypes.c_uint32 * len(_native.STAGES))()
         _native.check(_native.lib().nan_tracer_stage_times(self.handle, ms, n), "nan_tracer_stage_times")
         return {name: (ms[k], n[k]) for k, name in enumerate(_native.STAGES) if n[k]}
 

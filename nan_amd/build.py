@@ -8,6 +8,7 @@ kernels), and hipcc with packed-FP32 instructions disabled.
 from __future__ import annotations
 
 import os
+from concurrent.futures import ThreadPoolExecutor
 import subprocess
 from pathlib import Path
 
@@ -37,11 +38,12 @@ def build(verbose: bool = False) -> Path:
     bvh_obj = build_dir / "nan_bvh.o"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden", "-c",
           str(CSRC / "nan_bvh.cpp"), "-o", str(bvh_obj)])
-    objs = []
-    for src in ("nan_api.hip", "nan_wavefront.hip"):
-        obj = build_dir / (Path(src).stem + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(str(obj))
+    sources = ("nan_api.hip", "nan_wavefront.hip", "nan_nrc.hip", "nan_learned.hip")
+    objs = [str(build_dir / (Path(src).stem + ".o")) for src in sources]
+    cmds = [[HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", obj]
+            for src, obj in zip(sources, objs)]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as pool:
+        list(pool.map(_run, cmds))
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, str(bvh_obj), "-o", str(tmp)])
     os.replace(tmp, LIB)

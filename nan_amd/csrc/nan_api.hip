@@ -504,6 +504,57 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     s->h_itf.assign(d->inv_transposes, d->inv_transposes + (size_t)d->transform_count * 16);
     s->h_mats.assign(d->materials, d->materials + (size_t)d->material_count * 8);
     s->dev.root = bvh.root_entry;
+    {
+        // emissive triangles (material emission > 0), areas in float64
+        std::vector<float> em;
+        std::vector<double> areas;
+        for (uint64_t t = 0; t < tri_total; ++t) {
+            const uint32_t *id = d->instance_descs + 4 * (size_t)tri_ids[2 * t];
+            const uint32_t *mw = d->materials + 8 * (size_t)id[1];
+            float le[3];
+            std::memcpy(le, mw + 4, 12);
+            if (!(le[0] > 0.0f || le[1] > 0.0f || le[2] > 0.0f)) continue;
+            const float *w = world.data() + 9 * t;
+            double e1[3], e2[3], c[3];
+            for (int r = 0; r < 3; ++r) {
+                e1[r] = (double)w[3 + r] - w[r];
+                e2[r] = (double)w[6 + r] - w[r];
+            }
+            c[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            c[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            c[2] = e1[0] * e2[1] - e1[1] * e2[0];
+            double len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            if (!(len > 0.0)) continue;
+            float rec[20] = {w[0], w[1], w[2], (float)(0.5 * len),
+                             (float)e1[0], (float)e1[1], (float)e1[2], 0.0f,
+                             (float)e2[0], (float)e2[1], (float)e2[2], 0.0f,
+                             (float)(c[0] / len), (float)(c[1] / len), (float)(c[2] / len), 0.0f,
+                             le[0], le[1], le[2], 0.0f};
+            em.insert(em.end(), rec, rec + 20);
+            areas.push_back(0.5 * len);
+        }
+        if (!areas.empty()) {
+            double total = 0.0;
+            for (double a : areas) total += a;
+            std::vector<float> cdf(areas.size());
+            double run = 0.0;
+            for (size_t k = 0; k < areas.size(); ++k) {
+                run += areas[k];
+                cdf[k] = (float)(run / total);
+            }
+            cdf.back() = 1.0f;
+            float4 *demit;
+            float *dcdf;
+            if (up(&demit, em.data(), em.size() * 4) || up(&dcdf, cdf.data(), cdf.size() * 4)) {
+                delete s;
+                return -1;
+            }
+            s->dev.emitters = demit;
+            s->dev.emitter_cdf = dcdf;
+            s->dev.emitter_count = (uint32_t)areas.size();
+            s->dev.emitter_area = (float)total;
+        }
+    }
     float max_coord = 1.0f;
     for (float v : world) max_coord = std::max(max_coord, std::fabs(v));
     s->dev.slack = max_coord * 0x1p-18f;

@@ -129,6 +129,12 @@ struct DevScene {
     float slack;                  // absolute box-test slack (see kPad)
     Light light;
     const SceneRec *scenes;       // one record per scene traced together (>= 1)
+    // emissive triangles for light sampling (learned form only; null in batches):
+    // 5 float4 each: v0 area | e1 | e2 | unit normal | emission
+    const float4 *emitters;
+    const float *emitter_cdf;     // cumulative area fraction, last entry 1
+    uint32_t emitter_count;
+    float emitter_area;           // total emissive area
 };
 
 struct Ray {

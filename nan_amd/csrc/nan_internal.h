@@ -104,6 +104,7 @@ struct NanTracer {
     uint64_t total_front = 0, total_payload = 0;
     int mode = 1;                     // 0 megakernel, 1 wavefront
     struct WfBuffers *wf = nullptr;   // wavefront working set, allocated on first use
+    struct LearnedBuffers *learned = nullptr;  // learned-cache working set, allocated on first use
     nanx::WfTuning tuning;
     // stage timing: (stage, start, end) event triples recorded while profiling
     bool profile = false;
@@ -115,10 +116,12 @@ struct NanTracer {
 
 namespace nanx {
 void wavefront_release(NanTracer *t);
+void learned_release(NanTracer *t);
 }
 
 inline NanTracer::~NanTracer() {
     nanx::wavefront_release(this);
+    nanx::learned_release(this);
     for (hipEvent_t e : event_pool) hipEventDestroy(e);
     hipFree(face_scene);
     hipFree(scene_inst_base);
@@ -165,7 +168,19 @@ inline DevLayout dev_layout(const NanTracer *t) {
 }
 
 // Stage ids of nan_tracer_stage_times.
-enum Stage { kStageGenerate = 0, kStageExtend, kStageShade, kStageShadow, kStageFinish, kStageMegakernel, kStageCount };
+enum Stage {
+    kStageGenerate = 0,
+    kStageExtend,
+    kStageShade,
+    kStageShadow,
+    kStageFinish,
+    kStageMegakernel,
+    kStageLearnedPaths,
+    kStageCacheTrain,
+    kStageFirstBounce,
+    kStageCacheQuery,
+    kStageCount
+};
 
 // Brackets one kernel launch with events when the tracer is profiling.
 struct StageTimer {

@@ -1,0 +1,519 @@
+// Learned form of the texel tracer: plain path suffixes train the radiance
+// cache, then every texel sample takes the sun term and one cosine-sampled
+// bounce as the plain tracer does (texture_space_path_tracer.slang:86-117)
+// and ends in the cache at the first hit y1:
+//   plain   E = sun(x0) + (az/pdf) * (Le(y1) + suffix(y1))
+//   learned E = sun(x0) + (az/pdf) * (Le(y1) + max(0, cache(y1)))
+// where suffix(y1) is the radiance trace_radiance (slang:57-84) gathers after
+// y1's own emission, i.e. the cache's training target.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "nan_nrc_internal.h"
+#include "nan_tracer.h"
+
+using namespace nanx;
+
+// Device working set (plain pointers, passed to kernels by value).
+struct LearnedDev {
+    float *tr_pos, *tr_nrm, *tr_alb, *tr_target;  // training records [n][3]
+    uint32_t *tr_valid;
+    float *q_pos, *q_nrm, *q_alb, *q_w;  // cache queries [slots * spp][3], weight (0: none)
+    float4 *q_pred;
+    float4 *partial;  // per payload slot: sum over samples of the terms outside the cache
+};
+
+struct LearnedBuffers {
+    LearnedDev d{};
+    uint32_t cap_train = 0;
+    uint64_t cap_query = 0;
+    ~LearnedBuffers() {
+        for (void *p : {(void *)d.tr_pos, (void *)d.tr_nrm, (void *)d.tr_alb, (void *)d.tr_target,
+                        (void *)d.tr_valid, (void *)d.q_pos, (void *)d.q_nrm, (void *)d.q_alb, (void *)d.q_w,
+                        (void *)d.q_pred, (void *)d.partial})
+            hipFree(p);
+    }
+};
+
+namespace {
+
+// 32-bit integer hash ("lowbias32", C. Wellons, hash-prospector, 2018).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ void put3(float *a, uint64_t i, f3 v) {
+    a[3 * i + 0] = v.x;
+    a[3 * i + 1] = v.y;
+    a[3 * i + 2] = v.z;
+}
+
+// ---------------------------------------------------------------------------
+// light sampling of emissive triangles (NEE), combined with the cosine-sampled
+// bounce by the power heuristic (Veach, "Robust Monte Carlo Methods for Light
+// Transport Simulation", 1997, ch. 9).  Emission is two-sided, as shading
+// reports it for either face, so pdfs use |cos| at the emitter.
+// ---------------------------------------------------------------------------
+struct EmitterSample {
+    f3 dir, le;
+    float dist, pdf;  // pdf in solid angle
+};
+
+__device__ __forceinline__ float power_weight(float a, float b) {
+    float a2 = a * a, b2 = b * b;
+    return a2 / (a2 + b2);
+}
+
+// Uniform point over the emitters' total area: triangle by area CDF, point by
+// the square-root parametrisation.
+__device__ __forceinline__ bool sample_emitter(const DevScene &S, f3 o, float u0, float u1, float u2,
+                                               EmitterSample &es) {
+    uint32_t lo = 0, hi = S.emitter_count - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (u0 < S.emitter_cdf[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    const float4 *e = S.emitters + 5 * (size_t)lo;
+    float4 r0 = e[0], r1 = e[1], r2 = e[2], r3 = e[3], r4 = e[4];
+    float su = sqrtf(u1);
+    float a = su * (1.0f - u2), b = su * u2;
+    f3 p = mk(r0.x + a * r1.x + b * r2.x, r0.y + a * r1.y + b * r2.y, r0.z + a * r1.z + b * r2.z);
+    f3 d = p - o;
+    float d2 = dot(d, d);
+    if (!(d2 > 1e-12f)) return false;
+    es.dist = sqrtf(d2);
+    es.dir = d * (1.0f / es.dist);
+    float cy = fabsf(dot(mk(r3.x, r3.y, r3.z), es.dir));
+    if (!(cy > 1e-6f)) return false;
+    es.pdf = d2 / (cy * S.emitter_area);
+    es.le = mk(r4.x, r4.y, r4.z);
+    return true;
+}
+
+// Solid-angle pdf with which light sampling picks the point hit at distance t
+// along dir on triangle tri.
+__device__ __forceinline__ float emitter_pdf(const DevScene &S, uint32_t tri, f3 dir, float t) {
+    float cy = fabsf(dot(world_flat_normal(S, tri), dir));
+    return cy > 1e-6f ? t * t / (cy * S.emitter_area) : INFINITY;
+}
+
+template <int STACK, int BLOCK, bool COUNT>
+__device__ __forceinline__ bool visible_to(const DevScene &S, f3 o, f3 dir, float tmax, uint32_t *col,
+                                           PathCounters &pc) {
+    Ray r;
+    r.o = o;
+    r.d = dir;
+    r.tmin = 0.0f;
+    r.tmax = tmax;
+    ray_setup(r);
+    Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+    TraversalStats st{0, 0};
+    bool occluded = traverse<STACK, BLOCK, true, COUNT>(S, r, h, col, st);
+    if (COUNT) {
+        pc.shadow++;
+        pc.shadow_nodes += st.nodes;
+        pc.shadow_tris += st.tris;
+    }
+    return !occluded;
+}
+
+// Light-sampled emission reaching the surface sd, MIS-weighted against the
+// cosine-sampled bounce; returns Le * cos / pdf * w (cosine-weighted, no BSDF).
+template <int STACK, int BLOCK, bool COUNT>
+__device__ __forceinline__ f3 emitter_nee(const DevScene &S, const Shading &sd, Rng &rng, uint32_t *col,
+                                          PathCounters &pc) {
+    float u0 = rng.next_1d(), u1 = rng.next_1d(), u2 = rng.next_1d();
+    EmitterSample es;
+    if (!sample_emitter(S, sd.origin_front, u0, u1, u2, es)) return mk(0.0f, 0.0f, 0.0f);
+    float cx = dot(sd.frame.n, es.dir);
+    if (!(cx > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
+    if (!visible_to<STACK, BLOCK, COUNT>(S, sd.origin_front, es.dir, es.dist * (1.0f - 1e-4f), col, pc))
+        return mk(0.0f, 0.0f, 0.0f);
+    float w = power_weight(es.pdf, cx * kInvPi);
+    return es.le * (cx / es.pdf * w);
+}
+
+// Radiance leaving y1 (shading sd, reached by the first bounce) without its
+// emission: trace_radiance (texture_space_path_tracer.slang:57-84) from its
+// sun term at y1 on, for the remaining max_bounces - 1 bounces.  With nee,
+// emission is also light-sampled at every vertex that has a successor, and
+// bounce-sampled emission carries the matching MIS weight.
+template <int STACK, int BLOCK, bool COUNT>
+__device__ f3 suffix_radiance(const DevScene &S, Shading sd, uint32_t max_bounces, bool nee, Rng &rng,
+                              uint32_t *col, PathCounters &pc) {
+    const Light &L = S.light;
+    f3 T = mk(1.0f, 1.0f, 1.0f), rad = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t bounce = 0;; ++bounce) {
+        float ux = rng.next_1d(), uy = rng.next_1d();
+        f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+        float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+        f3 f = sd.diffuse * kInvPi;
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+            rad = rad + T * ((f * L.sun_illum) * ndl);
+        if (bounce + 1 >= max_bounces) break;
+        if (nee) rad = rad + T * (f * emitter_nee<STACK, BLOCK, COUNT>(S, sd, rng, col, pc));
+        float ex = rng.next_1d(), ey = rng.next_1d();
+        f3 sdir;
+        float pdf;
+        cosine_sample(ex, ey, sdir, pdf);
+        if (pdf <= 1e-6f) break;
+        float az = fabsf(sdir.z);
+        T = T * mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf);
+        Ray r;
+        r.o = sd.origin_front;
+        r.d = to_global(sd.frame, sdir);
+        r.tmin = 0.0f;
+        r.tmax = 1000.0f;
+        ray_setup(r);
+        Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+        TraversalStats st{0, 0};
+        bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
+        if (COUNT) {
+            pc.extend++;
+            pc.nodes += st.nodes;
+            pc.tris += st.tris;
+        }
+        if (!hit) {
+            rad = rad + T * L.sky;
+            break;
+        }
+        f3 dir = r.d;
+        shading_data(S, h.id, h.b1, h.b2, -dir, sd);
+        if (sd.emissive.x > 0.0f || sd.emissive.y > 0.0f || sd.emissive.z > 0.0f) {
+            float w = nee ? power_weight(pdf, emitter_pdf(S, h.id, dir, h.t)) : 1.0f;
+            rad = rad + T * sd.emissive * w;
+        }
+    }
+    return rad;
+}
+
+struct FirstHit {
+    float weight;  // az / pdf of the cosine-sampled direction (0: no direction)
+    float pdf;
+    float t;
+    uint32_t tri;
+    f3 dir;
+};
+
+// One cosine-sampled bounce from sd; on a hit fills y1's shading.
+template <int STACK, int BLOCK, bool COUNT>
+__device__ __forceinline__ bool first_bounce(const DevScene &S, const Shading &sd, float ex, float ey, FirstHit &fh,
+                                             Shading &y1, uint32_t *col, PathCounters &pc) {
+    f3 sdir;
+    float pdf;
+    cosine_sample(ex, ey, sdir, pdf);
+    fh.weight = 0.0f;
+    fh.pdf = pdf;
+    if (pdf <= 1e-6f) return false;
+    fh.weight = fabsf(sdir.z) / pdf;
+    Ray r;
+    r.o = sd.origin_front;
+    r.d = to_global(sd.frame, sdir);
+    r.tmin = 0.0f;
+    r.tmax = 1000.0f;
+    ray_setup(r);
+    fh.dir = r.d;
+    Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
+    TraversalStats st{0, 0};
+    bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
+    if (COUNT) {
+        pc.extend++;
+        pc.nodes += st.nodes;
+        pc.tris += st.tris;
+    }
+    fh.t = h.t;
+    fh.tri = h.id;
+    if (hit) shading_data(S, h.id, h.b1, h.b2, -fh.dir, y1);
+    return hit;
+}
+
+// Training records: record j draws a front texel uniformly (and, on a
+// double-sided instance, a side), follows one plain path from it and records
+// y1 (position, normal, albedo) with the suffix radiance as target.
+template <int STACK, int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayout Ly, uint32_t iteration,
+                                                              uint32_t spp, uint32_t max_bounces, uint32_t n,
+                                                              uint32_t flags, LearnedDev B,
+                                                              unsigned long long *counters) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    uint32_t *col = stack + threadIdx.x;
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
+    uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) {
+        bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && S.emitter_count > 0;
+        uint32_t h = mix32(j ^ mix32(iteration * 0x9E3779B9u + 0x632BE5ABu));
+        uint64_t w = ((uint64_t)h * Ly.total_front) >> 32;
+        Rng rng;
+        TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
+        bool back = (ts.side.y & 1u) && (mix32(h ^ 0x5bd1e995u) & 1u);
+        rng.state = mix32(h + 0x68E31DA4u);
+        f3 nf = world_flat_normal(S, ts.tri);
+        Shading sd0, y1;
+        shading_data(S, ts.tri, ts.b1, ts.b2, back ? -nf : nf, sd0);
+        float ex = rng.next_1d(), ey = rng.next_1d();
+        FirstHit fh;
+        bool valid = first_bounce<STACK, BLOCK, COUNT>(S, sd0, ex, ey, fh, y1, col, pc);
+        f3 target = mk(0.0f, 0.0f, 0.0f);
+        if (valid) {
+            target = suffix_radiance<STACK, BLOCK, COUNT>(S, y1, max_bounces, nee, rng, col, pc);
+            put3(B.tr_pos, j, y1.origin_front);
+            put3(B.tr_nrm, j, y1.frame.n);
+            put3(B.tr_alb, j, y1.diffuse);
+        } else {
+            put3(B.tr_pos, j, mk(0.0f, 0.0f, 0.0f));
+            put3(B.tr_nrm, j, mk(0.0f, 0.0f, 0.0f));
+            put3(B.tr_alb, j, mk(0.0f, 0.0f, 0.0f));
+        }
+        put3(B.tr_target, j, target);
+        B.tr_valid[j] = valid ? 1u : 0u;
+    }
+    if (COUNT) flush_counters(pc, counters);
+}
+
+__device__ __forceinline__ void texel_slots(const DevLayout &Ly, uint64_t w, uint32_t &front, uint32_t &back,
+                                            bool &two_sided) {
+    uint32_t f = Ly.texel_face[w];
+    uint32_t inst = Ly.face_inst[f];
+    uint4 ii = Ly.instances[inst];
+    uint2 side = Ly.sides[inst];
+    uint32_t local = (uint32_t)(w - Ly.front_prefix[inst]);
+    front = ii.z + local;
+    back = side.x + local;
+    two_sided = (side.y & 1u) != 0;
+}
+
+// Every texel sample: sun term at x0, (with nee) light-sampled emission, and
+// the first bounce; queries for the cache go to q = slot * spp + k (weight 0:
+// no query).
+template <int STACK, int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLayout Ly, uint32_t iteration,
+                                                               uint32_t spp, uint32_t flags, LearnedDev B,
+                                                               unsigned long long *counters) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    uint32_t *col = stack + threadIdx.x;
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
+    uint64_t w = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (w < Ly.total_front) {
+        const Light &L = S.light;
+        bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && S.emitter_count > 0;
+        Rng rng;
+        TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
+        uint32_t nspp = spp > 1u ? spp : 1u;
+        f3 nf = world_flat_normal(S, ts.tri);
+        for (int side = 0; side < 2; ++side) {
+            if (side == 1 && !(ts.side.y & 1u)) break;
+            uint32_t slot = side ? ts.side.x + ts.local : ts.global_texel;
+            Shading sd, y1;
+            shading_data(S, ts.tri, ts.b1, ts.b2, side ? -nf : nf, sd);
+            f3 acc = mk(0.0f, 0.0f, 0.0f);
+            for (uint32_t k = 0; k < nspp; ++k) {
+                float ux = rng.next_1d(), uy = rng.next_1d();
+                f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+                float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+                if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+                    acc = acc + L.sun_illum * ndl;
+                if (nee) acc = acc + emitter_nee<STACK, BLOCK, COUNT>(S, sd, rng, col, pc);
+                float ex = rng.next_1d(), ey = rng.next_1d();
+                FirstHit fh;
+                uint64_t q = (uint64_t)slot * nspp + k;
+                bool hit = first_bounce<STACK, BLOCK, COUNT>(S, sd, ex, ey, fh, y1, col, pc);
+                if (hit) {
+                    if (y1.emissive.x > 0.0f || y1.emissive.y > 0.0f || y1.emissive.z > 0.0f) {
+                        float wm = nee ? power_weight(fh.pdf, emitter_pdf(S, fh.tri, fh.dir, fh.t)) : 1.0f;
+                        acc = acc + y1.emissive * (fh.weight * wm);
+                    }
+                    put3(B.q_pos, q, y1.origin_front);
+                    put3(B.q_nrm, q, y1.frame.n);
+                    put3(B.q_alb, q, y1.diffuse);
+                    B.q_w[q] = fh.weight;
+                } else {
+                    if (fh.weight > 0.0f) acc = acc + L.sky * fh.weight;
+                    put3(B.q_pos, q, mk(0.0f, 0.0f, 0.0f));
+                    put3(B.q_nrm, q, mk(0.0f, 0.0f, 0.0f));
+                    put3(B.q_alb, q, mk(0.0f, 0.0f, 0.0f));
+                    B.q_w[q] = 0.0f;
+                }
+            }
+            B.partial[slot] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+        }
+    }
+    if (COUNT) flush_counters(pc, counters);
+}
+
+// Progressive mean with frame k (1-based since the reset) weighted by k:
+// m_k = m_{k-1} + 2 / (k + 1) * (e_k - m_{k-1}).  The cache improves over the
+// frames, so later estimates are the better ones.
+__device__ __forceinline__ void store_ramped(float4 *payload, uint32_t slot, f3 e, uint32_t reset) {
+    float4 p = payload[slot];
+    uint32_t prev_count = reset ? 0u : __float_as_uint(p.w);
+    f3 prev = prev_count == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(p.x, p.y, p.z);
+    uint32_t next = prev_count + 1 < kMaxTexelSamples ? prev_count + 1 : kMaxTexelSamples;
+    float w = 2.0f / (float)(prev_count + 2);
+    payload[slot] = make_float4(prev.x + w * (e.x - prev.x), prev.y + w * (e.y - prev.y), prev.z + w * (e.z - prev.z),
+                                __uint_as_float(next));
+}
+
+template <bool COUNT>
+__global__ void learned_finish(DevLayout Ly, LearnedDev B, float4 *payload, uint32_t spp, uint32_t reset,
+                               uint32_t flags, unsigned long long *counters) {
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
+    uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < Ly.total_front) {
+        uint32_t slots[2];
+        bool two;
+        texel_slots(Ly, w, slots[0], slots[1], two);
+        uint32_t nspp = spp > 1u ? spp : 1u;
+        for (int side = 0; side < (two ? 2 : 1); ++side) {
+            uint32_t slot = slots[side];
+            float4 p = B.partial[slot];
+            f3 e = mk(p.x, p.y, p.z);
+            for (uint32_t k = 0; k < nspp; ++k) {
+                uint64_t q = (uint64_t)slot * nspp + k;
+                float wq = B.q_w[q];
+                if (wq > 0.0f) {
+                    float4 c = B.q_pred[q];
+                    e = e + mk(fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f)) * wq;
+                }
+            }
+            float fs = (float)nspp;
+            if (flags & NAN_LEARNED_RAMPED_MEAN) store_ramped(payload, slot, mk(e.x / fs, e.y / fs, e.z / fs), reset);
+            else store_accumulated(payload, slot, mk(e.x / fs, e.y / fs, e.z / fs), reset);
+            if (COUNT) pc.slots++;
+        }
+    }
+    if (COUNT) flush_counters(pc, counters);
+}
+
+template <typename T>
+int grow(T **p, size_t bytes) {
+    hipFree(*p);
+    *p = nullptr;
+    HIP_TRY(hipMalloc((void **)p, bytes), nullptr, 0);
+    return 0;
+}
+
+int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query) {
+    if (!t->learned) t->learned = new LearnedBuffers();
+    LearnedBuffers &B = *t->learned;
+    LearnedDev &d = B.d;
+    if (n_train > B.cap_train || n_query > B.cap_query) HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
+    if (n_train > B.cap_train) {
+        size_t n = n_train;
+        B.cap_train = 0;
+        if (grow(&d.tr_pos, n * 12) || grow(&d.tr_nrm, n * 12) || grow(&d.tr_alb, n * 12) ||
+            grow(&d.tr_target, n * 12) || grow(&d.tr_valid, n * 4))
+            return -1;
+        B.cap_train = n_train;
+    }
+    if (n_query > B.cap_query) {
+        size_t n = n_query;
+        B.cap_query = 0;
+        if (grow(&d.q_pos, n * 12) || grow(&d.q_nrm, n * 12) || grow(&d.q_alb, n * 12) || grow(&d.q_w, n * 4) ||
+            grow(&d.q_pred, n * 16))
+            return -1;
+        B.cap_query = n_query;
+    }
+    if (!d.partial) {
+        if (grow(&d.partial, std::max<uint64_t>(t->total_payload, 1) * 16)) return -1;
+    }
+    return 0;
+}
+
+template <int STACK>
+int launch(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
+           uint32_t reset, uint32_t n_train, uint32_t flags, unsigned long long *counters, hipStream_t s) {
+    const DevScene &S = t->scene->dev;
+    DevLayout Ly = dev_layout(t);
+    const LearnedDev &B = t->learned->d;
+    uint32_t nspp = spp > 1u ? spp : 1u;
+    {
+        StageTimer tm(t, s, kStageLearnedPaths);
+        if (n_train) {
+            if (counters)
+                hipLaunchKernelGGL((learned_train_paths<STACK, kTraceBlock, true>), dim3(grid_for(n_train, kTraceBlock)),
+                                   dim3(kTraceBlock), 0, s, S, Ly, iteration, nspp, bounces, n_train, flags, B, counters);
+            else
+                hipLaunchKernelGGL((learned_train_paths<STACK, kTraceBlock, false>),
+                                   dim3(grid_for(n_train, kTraceBlock)), dim3(kTraceBlock), 0, s, S, Ly, iteration,
+                                   nspp, bounces, n_train, flags, B, counters);
+        }
+    }
+    if (n_train) {
+        StageTimer tm(t, s, kStageCacheTrain);
+        if (nrc_train_step(c, B.tr_pos, B.tr_nrm, B.tr_alb, B.tr_target, B.tr_valid, n_train, s)) return -1;
+    }
+    {
+        StageTimer tm(t, s, kStageFirstBounce);
+        uint32_t grid = grid_for(t->total_front, kTraceBlock);
+        if (counters)
+            hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s,
+                               S, Ly, iteration, nspp, flags, B, counters);
+        else
+            hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0,
+                               s, S, Ly, iteration, nspp, flags, B, counters);
+    }
+    {
+        StageTimer tm(t, s, kStageCacheQuery);
+        uint64_t nq = t->total_payload * nspp;
+        if (nrc_predict(c, B.q_pos, B.q_nrm, B.q_alb, (uint32_t)nq, B.q_pred, s)) return -1;
+    }
+    {
+        StageTimer tm(t, s, kStageFinish);
+        uint32_t grid = grid_for(t->total_front, kFlatBlock);
+        if (counters)
+            hipLaunchKernelGGL((learned_finish<true>), dim3(grid), dim3(kFlatBlock), 0, s, Ly, B, payload, nspp,
+                               reset, flags, counters);
+        else
+            hipLaunchKernelGGL((learned_finish<false>), dim3(grid), dim3(kFlatBlock), 0, s, Ly, B, payload, nspp,
+                               reset, flags, counters);
+    }
+    return check_launch("learned tracer");
+}
+
+}  // namespace
+
+namespace nanx {
+void learned_release(NanTracer *t) {
+    delete t->learned;
+    t->learned = nullptr;
+}
+}  // namespace nanx
+
+extern "C" {
+
+NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payload, uint32_t iteration,
+                                    uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
+                                    uint32_t train_samples, uint32_t flags, uint64_t *counters, void *stream) {
+    if (!tracer || !nrc || !payload) return fail(nullptr, 0, "nan_tst_execute_learned: null argument");
+    if (tracer->scene->batch || tracer->scene_count != 1)
+        return fail(nullptr, 0, "nan_tst_execute_learned: scene batches are not supported");
+    if (tracer->device_frame)
+        return fail(nullptr, 0, "nan_tst_execute_learned: device-frame mode is not supported");
+    if (tracer->scene->stack_depth == 0)
+        return fail(nullptr, 0, "nan_tst_execute_learned: BVH too deep for the per-texel kernels");
+    if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_execute_learned: max_bounces must be at least 1");
+    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN))
+        return fail(nullptr, 0, "nan_tst_execute_learned: unknown flags");
+    if (tracer->total_front == 0) return 0;
+    uint32_t nspp = samples_per_texel > 1u ? samples_per_texel : 1u;
+    uint64_t nq = tracer->total_payload * nspp;
+    if (nq >= (1ull << 31)) return fail(nullptr, 0, "nan_tst_execute_learned: too many cache queries per frame");
+    if (reserve(tracer, train_samples, nq) || nrc_reserve(nrc, (uint32_t)nq) || nrc_reserve_train(nrc, train_samples))
+        return -1;
+    unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
+    hipStream_t s = as_stream(stream);
+    if (tracer->scene->stack_depth <= 32)
+        return launch<32>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
+                          s);
+    return launch<64>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
+                      s);
+}
+
+}  // extern "C"

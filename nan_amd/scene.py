@@ -70,6 +70,12 @@ class Scene:
         except Exception:
             pass
 
+    def bounds(self) -> tuple:
+        """World-space bounding box (min, max float64[3]) of every instance."""
+        from .scene_arrays import world_bounds
+
+        return world_bounds(self.arrays)
+
     @property
     def triangle_count(self) -> int:
         return int(self.info.triangle_count)

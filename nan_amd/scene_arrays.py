@@ -65,3 +65,23 @@ def instance_triangle_offsets(arrays: dict) -> np.ndarray:
     if len(out) > 1:
         np.cumsum(tri_per_inst[:-1], out=out[1:])
     return out.astype(np.uint32)
+
+
+def world_bounds(arrays: dict) -> tuple[np.ndarray, np.ndarray]:
+    """Conservative world-space bounding box (min, max float64[3]): each
+    instance's object-space mesh box, its 8 corners transformed."""
+    verts = arrays["vertices"][:, :3].astype(np.float64)
+    descs = arrays["mesh_descs"].astype(np.int64)
+    mesh_lo = np.empty((len(descs), 3))
+    mesh_hi = np.empty((len(descs), 3))
+    for k, (vc, _ic, vo, _io) in enumerate(descs):
+        v = verts[vo:vo + vc]
+        mesh_lo[k] = v.min(axis=0) if vc else 0.0
+        mesh_hi[k] = v.max(axis=0) if vc else 0.0
+    inst = arrays["instance_descs"].astype(np.int64)
+    lo, hi = mesh_lo[inst[:, 0]], mesh_hi[inst[:, 0]]
+    m = arrays["transforms"].astype(np.float64).reshape(-1, 4, 4)[inst[:, 2]]
+    corners = np.stack([np.where(np.array([(c >> a) & 1 for a in range(3)], dtype=bool), hi, lo)
+                        for c in range(8)], axis=1)  # [I, 8, 3]
+    world = np.einsum("iab,icb->ica", m[:, :3, :3], corners) + m[:, None, :3, 3]
+    return world.reshape(-1, 3).min(axis=0), world.reshape(-1, 3).max(axis=0)

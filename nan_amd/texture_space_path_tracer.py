@@ -101,6 +101,28 @@ class TextureSpacePathTracer:
                                                        stream)
         _native.check(rc, "nan_tst_execute")
 
+    def execute_learned(self, command_encoder: CommandEncoder, output: torch.Tensor, iteration: int, cache, *,
+                        reset: bool = False, train_samples: int = 65536, emitter_nee: bool = True,
+                        ramped_mean: bool = True, counters: torch.Tensor | None = None) -> None:
+        """One iteration of the learned form: trains ``cache`` (a
+        :class:`~nan_amd.radiance_cache.RadianceCache`) on ``train_samples``
+        plain path suffixes, then every texel sample takes the sun term and
+        one bounce and ends in the cache.  ``emitter_nee`` adds light
+        sampling of emissive triangles with MIS (same expectation, lower
+        variance where emitters light the scene); ``ramped_mean`` weights the
+        k-th iteration since the reset by k in the slot's mean, so the early,
+        less trained cache fades out.  No counterpart in the reference."""
+        as_payload_tensor(output, self.layout.total_payload_count)
+        if counters is not None and (counters.dtype != torch.int64 or counters.numel() < _native.COUNTER_COUNT
+                                     or not counters.is_cuda):
+            raise ValueError("counters must be an int64 device tensor of at least 8 entries")
+        rc = _native.lib().nan_tst_execute_learned(
+            self.handle, cache.handle, output.data_ptr(), max(0, int(iteration)), self.samples_per_texel,
+            self.max_bounces, 1 if reset else 0, int(train_samples),
+            (_native.LEARNED_EMITTER_NEE if emitter_nee else 0) | (_native.LEARNED_RAMPED_MEAN if ramped_mean else 0),
+            None if counters is None else counters.data_ptr(), command_encoder.stream_handle)
+        _native.check(rc, "nan_tst_execute_learned")
+
     def tune(self, blocks_per_cu: int = 0, lds_stack: int = 16, refill_steps: int = 16) -> None:
         """Launch shape of the traversal kernels (results do not depend on it)."""
         _native.check(_native.lib().nan_tracer_tune(self.handle, int(blocks_per_cu), int(lds_stack),
@@ -114,8 +136,8 @@ class TextureSpacePathTracer:
     def stage_times(self) -> dict:
         """Milliseconds and launches per stage since profiling was enabled
         (waits for the recorded events)."""
-        ms = (ctypes.c_double * 8)()
-        n = (ctypes.c_uint32 * 8)()
+        ms = (ctypes.c_double * len(_native.STAGES))()
+        n = (ctypes.c_uint32 * len(_native.STAGES))()
         _native.check(_native.lib().nan_tracer_stage_times(self.handle, ms, n), "nan_tracer_stage_times")
         return {name: (ms[k], n[k]) for k, name in enumerate(_native.STAGES) if n[k]}
 
