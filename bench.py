@@ -5,7 +5,9 @@ the seeded synthetic 1.21M-triangle scene (512x512-quad value-noise
 heightfield, 64x64 extruded boxes, 500 level-3 icospheres; sun at 35 deg +
 sky 0.3), texels_per_unit 8, min_resolution 2, max_resolution 16, budget
 16,777,216 slots, 1 sample/texel, 3 bounces, FP32.  ``--workload cornell``
-runs the small Cornell box instead.
+runs the small Cornell box instead.  The default run also measures the
+learned radiance cache against the plain tracer at equal RMSE on the Cornell
+box (``learned_vs_plain``; ``--workload learned`` runs only that).
 
 Prints one JSON line.  ``value`` is texel-path samples/s (slot updates per
 second) with the scene, layout and cache resident in HBM.  Every tracer
@@ -111,6 +113,162 @@ def cpu_baseline(node, layout, threads: int, target_seconds: float = 12.0) -> di
     }
 
 
+MFMA_F16_PEAK_TFLOPS = 2500.0     # dense FP16 MFMA peak of MI355X
+MLP_FLOPS_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 16)   # executed (padded) forward FLOPs: 32,768
+
+
+def _plain_frames_for(curve: dict, r: float) -> float:
+    """Plain frames that reach RMSE r: log-log interpolation of the measured
+    plain curve, extended beyond its ends with the 1/sqrt(frames) law."""
+    import math
+
+    import numpy as np
+
+    ks = np.array(sorted(curve), dtype=np.float64)
+    rs = np.array([curve[k] for k in sorted(curve)], dtype=np.float64)
+    if r >= rs[0]:
+        return float(ks[0] * (rs[0] / r) ** 2)
+    if r <= rs[-1]:
+        return float(ks[-1] * (rs[-1] / r) ** 2)
+    return float(np.exp(np.interp(-math.log(r), -np.log(rs), np.log(ks))))
+
+
+def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, plain_curve_frames: int = 2048,
+                        train_samples: int = 65536, mlp_queries: int = 1 << 22) -> dict:
+    """Learned radiance cache against the plain tracer on the Cornell box
+    (texels_per_unit 16, res 4-64, 1 spp, 3 bounces): RMSE of both against a
+    plain bake, the plain frames that reach each learned RMSE, the
+    effective converged texel-path samples/s, and the fused MLP forward
+    kernel's MFMA rate on a large query batch."""
+    import numpy as np
+    import torch
+
+    from nan_amd.mesh_colors import MeshColorsLayout
+    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+    from nan_amd.scene import Scene
+    from nan_amd.scenes import cornell_box
+    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
+    node = cornell_box()
+    layout = MeshColorsLayout.build(node, texels_per_unit=16.0, min_resolution=4, max_resolution=64)
+    scene = Scene(device, node)
+    tracer = TextureSpacePathTracer(device, scene, layout, samples_per_texel=1, max_bounces=3, mode="megakernel")
+    enc = device.create_command_encoder()
+    stream = torch.cuda.current_stream(device.torch_device)
+    slots = layout.total_payload_count
+    dev = device.torch_device
+    bake = torch.zeros((slots, 4), dtype=torch.float32, device=dev)
+    for it in range(bake_frames):  # its own RNG streams: iterations 1,000,000 + k
+        tracer.execute(enc, bake, 1_000_000 + it, reset=(it == 0))
+    b = bake[:, :3].double()
+
+    def rmse(p):
+        return float(torch.sqrt(torch.mean((p[:, :3].double() - b) ** 2)))
+
+    plain = torch.zeros_like(bake)
+    curve = {}
+    for it in range(plain_curve_frames):
+        tracer.execute(enc, plain, it, reset=(it == 0))
+        if (it + 1) & it == 0:
+            curve[it + 1] = rmse(plain)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for it in range(512):
+        tracer.execute(enc, plain, it, reset=(it == 0))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    plain_ms = ev0.elapsed_time(ev1) / 512
+
+    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=4))
+    learned = torch.zeros_like(bake)
+    out_frames = {}
+    elapsed_ms = 0.0
+    done = 0
+    tracer.profile(True)
+    for target in frames:
+        ev0.record(stream)
+        for it in range(done, target):
+            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=train_samples)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        elapsed_ms += ev0.elapsed_time(ev1)
+        done = target
+        r = rmse(learned)
+        m = _plain_frames_for(curve, r)
+        learned_ms = elapsed_ms / target
+        out_frames[str(target)] = {
+            "rmse": r,
+            "plain_frames_for_equal_rmse": m,
+            "learned_ms_per_frame": learned_ms,
+            "effective_samples_per_s": m * slots / (elapsed_ms * 1e-3),
+            "speedup_vs_plain": (m * plain_ms) / elapsed_ms,
+        }
+    stages = tracer.stage_times()
+    tracer.profile(False)
+    # fused MLP forward alone on a large batch of queries inside the scene box
+    rng = np.random.default_rng(5)
+    lo, hi = scene.bounds()
+    q = mlp_queries
+    pos = torch.from_numpy(rng.uniform(lo, hi, size=(q, 3)).astype(np.float32)).to(dev)
+    nrm = torch.nn.functional.normalize(torch.randn((q, 3), device=dev), dim=1).contiguous()
+    alb = torch.rand((q, 3), device=dev)
+    mlp_ms = cache.time_forward(pos, nrm, alb, repeats=20, encoder=enc)
+    tflops = MLP_FLOPS_PER_QUERY * q / (mlp_ms * 1e-3) / 1e12
+    last = out_frames[str(frames[-1])]
+    return {
+        "workload": ("Cornell box (36 tris), texels_per_unit 16, res 4-64, 1 spp, 3 bounces; learned cache: "
+                     "16-level hash grid x 2 FP16 features, 2^19 entries/level, normal + albedo, 4x64 ReLU MLP "
+                     f"FP16 MFMA / FP32 accumulate, Adam lr 1e-2, relative L2, {train_samples} training suffixes "
+                     "per frame in 4 Adam steps, paths end in the cache after 1 bounce, emitter light sampling "
+                     "with MIS, ramped mean"),
+        "bake": f"{bake_frames}-frame plain bake (independent RNG streams)",
+        "payload_slots": slots,
+        "plain_ms_per_frame": plain_ms,
+        "plain_samples_per_s": slots / (plain_ms * 1e-3),
+        "plain_rmse": {str(k): v for k, v in curve.items()},
+        "learned": out_frames,
+        "value": last["effective_samples_per_s"],
+        "speedup_vs_plain": last["speedup_vs_plain"],
+        "learned_stage_ms_per_frame": {k: v[0] / frames[-1] for k, v in stages.items()},
+        "mlp_roofline": {
+            "bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
+            "flops_per_query": MLP_FLOPS_PER_QUERY, "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": tflops / MFMA_F16_PEAK_TFLOPS,
+        },
+    }
+
+
+def bench_learned(args) -> int:
+    """The learned radiance cache against the plain tracer at equal RMSE."""
+    from nan_amd.device import Device
+
+    device = Device(0)
+    t0 = time.perf_counter()
+    res = learned_measurement(device)
+    log(f"learned measurement {time.perf_counter() - t0:.1f} s")
+    out = {
+        "metric": METRIC,
+        "value": res["value"],
+        "unit": "effective converged texel-path samples/s (learned, 256 frames)",
+        "n_gpus": 1,
+        "steps": 256,
+        "warmup": 0,
+        "ms_per_step": res["learned"]["256"]["learned_ms_per_frame"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 MFMA / f32",
+        "data": "synthetic",
+        "config": {"workload": res["workload"], "parallelism": "single GPU"},
+        "learned_vs_plain": res,
+        "roofline": {k: v for k, v in res["mlp_roofline"].items()},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_batch(args) -> int:
     """64 seeded scenes (1k-200k triangles, 16k-2M slot budgets, 1-8 bounces)
     traced by one set of wavefront launches captured in a hipGraph, after the
@@ -181,7 +339,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["synthetic", "cornell", "batch"], default="synthetic")
+    ap.add_argument("--workload", choices=["synthetic", "cornell", "batch", "learned"], default="synthetic")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-cache comparison")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -217,6 +376,8 @@ def main() -> int:
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "learned":
+        return bench_learned(args)
 
     t0 = time.perf_counter()
     node, layout, layout_kw, desc = build_workload(args.workload)
@@ -345,6 +506,11 @@ def main() -> int:
             out["roofline"]["traffic"] = rec["traffic_bytes_per_launch"]
             out["roofline"]["l2_hit_rate"] = rec.get("l2_hit_rate")
             out["roofline"]["traffic_source"] = rec.get("source")
+    if not args.no_learned and args.workload == "synthetic":
+        t0 = time.perf_counter()
+        res = learned_measurement(device)
+        log(f"learned measurement {time.perf_counter() - t0:.1f} s")
+        out["learned_vs_plain"] = res
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)

@@ -259,6 +259,11 @@ NAN_API int nan_nrc_backward_raw(NanNrc *nrc, const float *pos, const float *nrm
 /* Mean loss and valid record count of the last training step (host floats;
  * waits for the device). */
 NAN_API int nan_nrc_stats(NanNrc *nrc, float *loss_and_count2);
+/* Measurement hook: encodes n queries, then launches the fused MLP forward
+ * kernel alone `repeats` times between two HIP events on `stream` and
+ * returns the average milliseconds per launch (waits for the device). */
+NAN_API int nan_nrc_time_forward(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, uint32_t n,
+                                 uint32_t repeats, float *ms_out, void *stream);
 /* Encoded network input float[n][48] (hash features, normal, albedo, zeros). */
 NAN_API int nan_nrc_encode(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out, uint32_t n,
                            void *stream);

@@ -841,6 +841,36 @@ NAN_API int nan_nrc_stats(NanNrc *nrc, float *out) {
     return 0;
 }
 
+NAN_API int nan_nrc_time_forward(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, uint32_t n,
+                                 uint32_t repeats, float *ms_out, void *stream) {
+    if (!nrc || !ms_out || (n && (!pos || !nrm || !alb))) return fail(nullptr, 0, "nan_nrc_time_forward: null argument");
+    *ms_out = 0.0f;
+    if (n == 0 || repeats == 0) return 0;
+    hipStream_t s = as_stream(stream);
+    if (nrc_reserve(nrc, n)) return -1;
+    uint32_t groups = groups_for(n);
+    encode(nrc, pos, nrm, alb, n, nullptr, s);
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a), nullptr, 0);
+    if (hipEventCreate(&b) != hipSuccess) {
+        hipEventDestroy(a);
+        return fail(nullptr, 0, "nan_nrc_time_forward: hipEventCreate failed");
+    }
+    hipEventRecord(a, s);
+    for (uint32_t r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL((nrc_mlp_fwd<false>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, nrc->w16t,
+                           nrc->enc, groups, n, nrc->pred, (_Float16 *)nullptr, nrc->cap_groups, (float *)nullptr);
+    hipEventRecord(b, s);
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (e != hipSuccess) return fail(nullptr, 0, "nan_nrc_time_forward: %s", hipGetErrorString(e));
+    *ms_out = ms / (float)repeats;
+    return check_launch("nrc forward timing");
+}
+
 NAN_API int nan_nrc_encode(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out, uint32_t n,
                            void *stream) {
     if (!nrc || (n && (!pos || !nrm || !alb || !out))) return fail(nullptr, 0, "nan_nrc_encode: null argument");

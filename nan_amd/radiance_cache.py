@@ -183,6 +183,16 @@ class RadianceCache:
                                                    self._stream(encoder)), "nan_nrc_encode")
         return out
 
+    def time_forward(self, pos, nrm, alb, repeats: int = 10, encoder: CommandEncoder | None = None) -> float:
+        """Milliseconds per launch of the fused MLP forward kernel alone on
+        these (already encoded) queries, from HIP events on the stream."""
+        n = self._inputs(pos, nrm, alb)
+        ms = ctypes.c_float()
+        _native.check(_native.lib().nan_nrc_time_forward(self.handle, _ptr(pos), _ptr(nrm), _ptr(alb), n,
+                                                         int(repeats), ctypes.byref(ms), self._stream(encoder)),
+                      "nan_nrc_time_forward")
+        return float(ms.value)
+
     def backward_raw(self, pos, nrm, alb, target, encoder: CommandEncoder | None = None):
         """Gradients of the mean loss: (weights float32[16384], inputs
         float32[n, 48]); parameters are not updated."""
