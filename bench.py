@@ -269,6 +269,168 @@ def bench_learned(args) -> int:
     return 0
 
 
+COMBINED_LAYOUT = dict(texels_per_unit=2.0, min_resolution=1, max_resolution=4, max_total_texels=16_777_216)
+
+
+def bench_combined(args) -> int:
+    """Combined full-size run: the synthetic generator scaled 3.3x (4.0M
+    triangles; 45% of object materials double-sided, terrain single-sided, so
+    20.5% of the 16.7M slots are back slots), traced by the batched persistent
+    wavefront with every frame a replayed hipGraph (per-frame latency
+    percentiles), a 3840x2160 resolve, and the learned form on the same scene
+    against a plain bake."""
+    import numpy as np
+    import torch
+
+    from nan_amd import _native
+    from nan_amd.autotune import autotune
+    from nan_amd.batch import SceneBatch
+    from nan_amd.device import Device
+    from nan_amd.passes import MeshColorsResolve
+    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+    from nan_amd.scenes import synthetic_scene
+
+    device = Device(0)
+    dev = device.torch_device
+    t0 = time.perf_counter()
+    node = synthetic_scene(0, 3.3, double_sided=0.45, single_sided_terrain=True)
+    batch = SceneBatch(device, [node], [COMBINED_LAYOUT], [3])
+    layout, scene, tracer = batch.layouts[0], batch.scenes[0], batch.tracers[0]
+    slots = layout.total_payload_count
+    log(f"combined scene {node.triangle_count} tris, {slots} slots ({slots - layout.total_surface_texels} back), "
+        f"built in {time.perf_counter() - t0:.1f} s")
+    enc = device.create_command_encoder()
+    stream = torch.cuda.current_stream(dev)
+    tune = autotune(batch, enc, space={"blocks_per_cu": (0, 4), "lds_stack": (16, 32), "refill_steps": (16, 32)},
+                    frames=1)
+    log(f"autotuned {tune['best']}")
+    batch.execute(enc, 0, reset=True)
+    torch.cuda.synchronize()
+    graph = batch.capture(enc, 1)
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    frames = args.steps
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(frames)]
+    for a, b in evs:
+        a.record(stream)
+        graph.replay()
+        b.record(stream)
+    torch.cuda.synchronize()
+    lat = np.array([a.elapsed_time(b) for a, b in evs])
+    batch.clear_device_frame()
+    counters = torch.zeros(8, dtype=torch.int64, device=dev)
+    scratch_iter = 1 + args.warmup
+    batch.execute(enc, scratch_iter, counters=counters)  # counts of one frame (payload keeps accumulating)
+    torch.cuda.synchronize()
+    c = counters.cpu().numpy().astype(np.int64)
+    ext, shd = int(c[_native.COUNTER_EXTEND_RAYS]), int(c[_native.COUNTER_SHADOW_RAYS])
+    alg = (BYTES_PER_NODE * int(c[_native.COUNTER_NODE_VISITS] + c[_native.COUNTER_SHADOW_NODE_VISITS])
+           + BYTES_PER_TRI * int(c[_native.COUNTER_TRI_TESTS] + c[_native.COUNTER_SHADOW_TRI_TESTS])
+           + (BYTES_PER_RAY + BYTES_PER_HIT) * ext + BYTES_PER_SHADOW_RAY * shd
+           + BYTES_PER_SLOT_UPDATE * int(c[_native.COUNTER_SLOT_UPDATES]) + BYTES_PER_SHADED_HIT * ext)
+    mean_ms = float(lat.mean())
+
+    # 3840x2160 resolve of the accumulated cache
+    resolve = MeshColorsResolve(device, scene, layout, tracer)
+    image = torch.zeros((2160, 3840, 4), dtype=torch.float32, device=dev)
+    resolve.execute(enc, batch.payloads[0], image)
+    r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    r0.record(stream)
+    for _ in range(5):
+        resolve.execute(enc, batch.payloads[0], image)
+    r1.record(stream)
+    torch.cuda.synchronize()
+    resolve_ms = r0.elapsed_time(r1) / 5
+
+    # learned form against a plain bake, on the scene's own tracer
+    bake = torch.zeros((slots, 4), dtype=torch.float32, device=dev)
+    tb = time.perf_counter()
+    for it in range(args.bake_frames):
+        tracer.execute(enc, bake, 1_000_000 + it, reset=(it == 0))
+        if it % 256 == 255:
+            torch.cuda.synchronize()
+            log(f"bake {it + 1}/{args.bake_frames} ({time.perf_counter() - tb:.0f} s)")
+    torch.cuda.synchronize()
+    bk = bake[:, :3].double()
+
+    def rmse(p):
+        return float(torch.sqrt(torch.mean((p[:, :3].double() - bk) ** 2)))
+
+    plain = torch.zeros_like(bake)
+    curve = {}
+    for it in range(64):
+        tracer.execute(enc, plain, it, reset=(it == 0))
+        if (it + 1) & it == 0:
+            curve[it + 1] = rmse(plain)
+    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=4))
+    learned = torch.zeros_like(bake)
+    learned_frames = {}
+    elapsed = 0.0
+    done = 0
+    for target in (16, 64):
+        r0.record(stream)
+        for it in range(done, target):
+            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0))
+        r1.record(stream)
+        torch.cuda.synchronize()
+        elapsed += r0.elapsed_time(r1)
+        done = target
+        r = rmse(learned)
+        m = _plain_frames_for(curve, r)
+        learned_frames[str(target)] = {"rmse": r, "plain_frames_for_equal_rmse": m,
+                                       "learned_ms_per_frame": elapsed / target,
+                                       "speedup_vs_plain": m * mean_ms / elapsed}
+    q = slots  # one cache query per texel sample
+    rng = np.random.default_rng(3)
+    lo, hi = scene.bounds()
+    pos = torch.from_numpy(rng.uniform(lo, hi, size=(q, 3)).astype(np.float32)).to(dev)
+    nrm = torch.nn.functional.normalize(torch.randn((q, 3), device=dev), dim=1).contiguous()
+    alb = torch.rand((q, 3), device=dev)
+    mlp_ms = cache.time_forward(pos, nrm, alb, repeats=5, encoder=enc)
+    tflops = MLP_FLOPS_PER_QUERY * q / (mlp_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": slots / (mean_ms * 1e-3),
+        "unit": "texel-path samples/s",
+        "n_gpus": 1,
+        "steps": frames,
+        "warmup": args.warmup,
+        "ms_per_step": mean_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("combined: synthetic generator scaled 3.3x (seed 0, 45% of object materials "
+                         "double-sided, terrain single-sided), layout texels_per_unit 2, res 1-4, budget "
+                         "16,777,216; batched persistent wavefront (batch of one scene), every frame a replayed "
+                         "hipGraph; 1 spp, 3 bounces; resolve 3840x2160; learned form on the same scene"),
+            "triangles": node.triangle_count,
+            "payload_slots": slots,
+            "back_slots": slots - layout.total_surface_texels,
+            "layout": COMBINED_LAYOUT,
+            "parallelism": "single GPU",
+        },
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "min": float(lat.min()), "max": float(lat.max())},
+        "mrays_per_s": (ext + shd) / (mean_ms * 1e-3) / 1e6,
+        "rays_per_step": {"extend": ext, "shadow": shd},
+        "algorithmic_gb_per_s": alg / (mean_ms * 1e-3) / 1e9,
+        "autotune": tune["best"],
+        "resolve_3840x2160_ms": resolve_ms,
+        "learned_vs_plain": {"bake_frames": args.bake_frames, "plain_rmse": {str(k): v for k, v in curve.items()},
+                             "learned": learned_frames},
+        "roofline": {"bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
+                     "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / MFMA_F16_PEAK_TFLOPS},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_batch(args) -> int:
     """64 seeded scenes (1k-200k triangles, 16k-2M slot budgets, 1-8 bounces)
     traced by one set of wavefront launches captured in a hipGraph, after the
@@ -339,7 +501,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["synthetic", "cornell", "batch", "learned"], default="synthetic")
+    ap.add_argument("--workload", choices=["synthetic", "cornell", "batch", "learned", "combined"],
+                    default="synthetic")
+    ap.add_argument("--bake-frames", type=int, default=1024, help="plain bake length of --workload combined")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-cache comparison")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -378,6 +542,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "combined":
+        return bench_combined(args)
 
     t0 = time.perf_counter()
     node, layout, layout_kw, desc = build_workload(args.workload)

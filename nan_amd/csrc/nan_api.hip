@@ -719,7 +719,7 @@ NAN_API int nan_tracer_tables(const NanTracer *tracer, const void **face_table, 
 
 static int tst_execute(NanTracer *t, void *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                        uint32_t reset, uint64_t *counters, void *stream) {
-    if (!t || (!payload && t->scene_count == 1)) return fail(nullptr, 0, "nan_tst_execute: null argument");
+    if (!t || (!payload && !t->scene->batch)) return fail(nullptr, 0, "nan_tst_execute: null argument");
     if (t->total_front == 0) return 0;
     if (t->scene->batch && t->mode != NAN_TRACER_WAVEFRONT)
         return fail(nullptr, 0, "nan_tst_execute: scene batches run in wavefront mode only");
@@ -1016,7 +1016,7 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
 }
 
 NAN_API int nan_batch_set_payloads(NanTracer *batch, void *const *payloads) {
-    if (!batch || !payloads || batch->scene_count < 2 || !batch->payload_table)
+    if (!batch || !payloads || !batch->scene->batch || !batch->payload_table)
         return fail(nullptr, 0, "nan_batch_set_payloads: not a batch or null payloads");
     for (uint32_t k = 0; k < batch->scene_count; ++k)
         if (!payloads[k]) return fail(nullptr, 0, "nan_batch_set_payloads: payload %u is null", k);

@@ -642,7 +642,7 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
     if (t->scene->info.max_stack > (uint32_t)kMaxStack)
         return fail(nullptr, 0, "wavefront: BVH needs a %u-entry stack (max %d)", t->scene->info.max_stack,
                     kMaxStack);
-    if (t->scene_count > 1 && !t->payload_table_set)
+    if (t->scene->batch && !t->payload_table_set)
         return fail(nullptr, 0, "wavefront: batch payloads are not set (nan_batch_set_payloads)");
     if (wavefront_alloc(t, spp)) return -1;
     WfBuffers &W = *t->wf;
@@ -692,7 +692,7 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                 q = 1u - q;
             }
             FinishParams fp{t->device_frame ? W.frame : nullptr, list, count, side, k, nspp, reset,
-                            t->scene_count > 1 ? t->payload_table : nullptr};
+                            t->scene->batch ? t->payload_table : nullptr};
             StageTimer tm(t, stream, kStageFinish);
             if (counters)
                 hipLaunchKernelGGL(wf_finish<true>, dim3(gen_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,

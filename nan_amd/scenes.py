@@ -178,8 +178,13 @@ def _terrain_height(mesh: Mesh, quads: int, extent: float, x: float, z: float) -
 # large synthetic scene
 # ---------------------------------------------------------------------------
 def synthetic_scene(seed: int = 0, scale: float = 1.0, *, terrain_quads: int = 512, box_grid: int = 64,
-                    spheres: int = 500, sphere_level: int = 3, extent: float = 64.0) -> SceneNode:
-    """Heightfield + extruded boxes + icospheres (see module docstring)."""
+                    spheres: int = 500, sphere_level: int = 3, extent: float = 64.0,
+                    double_sided: float = 0.05, single_sided_terrain: bool = False) -> SceneNode:
+    """Heightfield + extruded boxes + icospheres (see module docstring);
+    ``scale`` multiplies the triangle count (terrain and box grid by its
+    square root per side, spheres linearly) and ``double_sided`` is the
+    fraction of object materials that are double-sided (the terrain's draw
+    is overridden to single-sided with ``single_sided_terrain``)."""
     rng = np.random.default_rng(seed)
     node = SceneNode()
     node.lighting = Lighting.sun_and_sky(35.0, 30.0, sky=0.3)
@@ -188,7 +193,9 @@ def synthetic_scene(seed: int = 0, scale: float = 1.0, *, terrain_quads: int = 5
     n_spheres = int(round(spheres * scale))
     terrain = heightfield_mesh(rng, quads, extent, 2.0)
     n_objects = 1 + grid * grid + n_spheres
-    double_sided = rng.random(n_objects) < 0.05
+    double_sided = rng.random(n_objects) < double_sided
+    if single_sided_terrain:
+        double_sided[0] = False
     albedo = rng.uniform(0.1, 0.8, size=(n_objects, 3))
     identity = node.add_transform(Transform())
 

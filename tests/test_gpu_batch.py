@@ -112,3 +112,15 @@ def test_autotuner_picks_a_measured_candidate(device):
     assert len(res["results"]) == 8
     assert res["best"]["ms_per_frame"] == min(r["ms_per_frame"] for r in res["results"])
     assert batch.tuning == (res["best"]["blocks_per_cu"], res["best"]["lds_stack"], res["best"]["refill_steps"])
+
+
+def test_batch_of_one_scene_equals_the_scene_alone(device):
+    from nan_amd.batch import SceneBatch
+
+    _say("batch of one scene")
+    batch = SceneBatch(device, [batch_scene(41, 6000)],
+                       [dict(texels_per_unit=4.0, min_resolution=1, max_resolution=8, max_total_texels=500_000)], [3])
+    enc = device.create_command_encoder()
+    for it in range(2):
+        batch.execute(enc, it, reset=(it == 0))
+    _assert_same(batch, _single_runs(batch, 2), "batch of one")
