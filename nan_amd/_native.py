@@ -91,7 +91,7 @@ EXPORTED_SYMBOLS = (
     "nan_batch_set_payloads", "nan_batch_set_bounces", "nan_tracer_device_frame",
     "nan_nrc_create", "nan_nrc_destroy", "nan_nrc_param_counts", "nan_nrc_get_params", "nan_nrc_set_params",
     "nan_nrc_infer", "nan_nrc_train", "nan_nrc_forward_raw", "nan_nrc_backward_raw", "nan_nrc_encode",
-    "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned",
+    "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned", "nan_surface_filter",
 )
 
 _LIB = None
@@ -144,6 +144,7 @@ def _declare(lib) -> None:
         "nan_nrc_stats": (I, [vp, f32p]),
         "nan_nrc_time_forward": (I, [vp, vp, vp, vp, U32, U32, f32p, vp]),
         "nan_tst_execute_learned": (I, [vp, vp, vp, U32, U32, U32, U32, U32, U32, vp, vp]),
+        "nan_surface_filter": (I, [vp, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -114,47 +114,6 @@ struct DevCamera {
     float jx, jy, near, far;
 };
 
-__device__ __forceinline__ uint32_t side_texel(uint32_t front, uint32_t texel_offset, uint2 side, bool back) {
-    return back ? side.x + (front - texel_offset) : front;
-}
-
-template <bool FROZEN>
-__device__ __forceinline__ f3 load_cache(const void *cache, uint32_t slot) {
-    if (FROZEN) return unpack_rgb9e5(static_cast<const uint32_t *>(cache)[slot]);
-    float4 p = static_cast<const float4 *>(cache)[slot];
-    return mk(p.x, p.y, p.z);
-}
-
-// mesh_colors_sample_irradiance (mesh_colors.slang:365-426)
-template <bool FROZEN>
-__device__ f3 sample_irradiance(const void *cache, f3 b, uint32_t texel_offset, uint2 face, uint2 side, bool back) {
-    uint32_t R = face.y;
-    float fr = (float)R;
-    f3 sc = mk(saturate(b.x) * fr, saturate(b.y) * fr, saturate(b.z) * fr);
-    uint32_t i = (uint32_t)sc.x, j = (uint32_t)sc.y;
-    i = i > R ? R : i;
-    j = j > R - i ? R - i : j;
-    f3 w = mk(fracf(sc.x), fracf(sc.y), fracf(sc.z));
-    bool upper = fabsf(w.x + w.y + w.z - 2.0f) <= 1e-3f;
-    if (upper) w = mk(1.0f - w.x, 1.0f - w.y, 1.0f - w.z);
-    if (w.x + w.y + w.z <= 1e-6f || i >= R || j >= R - i)
-        return load_cache<FROZEN>(cache, side_texel(lattice_address(i, j, texel_offset, face), texel_offset, side, back));
-    uint32_t ta, tb, tc;
-    if (upper) {
-        ta = lattice_address(i, j + 1, texel_offset, face);
-        tb = lattice_address(i + 1, j, texel_offset, face);
-        tc = lattice_address(i + 1, j + 1, texel_offset, face);
-    } else {
-        ta = lattice_address(i + 1, j, texel_offset, face);
-        tb = lattice_address(i, j + 1, texel_offset, face);
-        tc = lattice_address(i, j, texel_offset, face);
-    }
-    f3 A = load_cache<FROZEN>(cache, side_texel(ta, texel_offset, side, back));
-    f3 B = load_cache<FROZEN>(cache, side_texel(tb, texel_offset, side, back));
-    f3 C = load_cache<FROZEN>(cache, side_texel(tc, texel_offset, side, back));
-    return A * w.x + B * w.y + C * w.z;
-}
-
 __device__ f3 sky_with_sun_disk(const Light &L, f3 dir) {
     f3 out = L.sky;
     const float ground = 6.360f, vy = 6.360f + 0.0002f;

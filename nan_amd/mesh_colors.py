@@ -192,8 +192,16 @@ class MeshColorsLayout:
     """
 
     def __init__(self, face_table, instance_table, side_table, total_surface_texels,
-                 total_payload_count, texels_per_unit, min_resolution, max_resolution):
+                 total_payload_count, texels_per_unit, min_resolution, max_resolution,
+                 adjacency_table=None, adjacency_diagnostics=None):
+        from .mesh_colors_adjacency import MESH_COLORS_INVALID_ADJACENCY, MeshColorsAdjacencyDiagnostics
+
         self.face_table = np.ascontiguousarray(face_table, dtype=np.uint32)
+        if adjacency_table is None:
+            adjacency_table = np.full((len(self.face_table), 4), MESH_COLORS_INVALID_ADJACENCY, dtype=np.uint32)
+        # uint32[F, 4]: packed edge records per global face, adjacent faces instance-local
+        self.adjacency_table = np.ascontiguousarray(adjacency_table, dtype=np.uint32)
+        self.adjacency_diagnostics = adjacency_diagnostics or MeshColorsAdjacencyDiagnostics()
         self.instance_table = np.ascontiguousarray(instance_table, dtype=np.uint32)
         self.side_table = np.ascontiguousarray(side_table, dtype=np.uint32)
         self.total_surface_texels = int(total_surface_texels)
@@ -209,6 +217,13 @@ class MeshColorsLayout:
     @property
     def face_infos(self) -> tuple[MeshColorsFaceInfo, ...]:
         return tuple(MeshColorsFaceInfo(int(a), int(r)) for a, r in self.face_table)
+
+    @property
+    def adjacency_infos(self):
+        """Per global face, its three edge records (mesh_colors.py:124)."""
+        from .mesh_colors_adjacency import _FaceView
+
+        return _FaceView(self.adjacency_table)
 
     @property
     def instance_infos(self) -> tuple[MeshColorsInstanceInfo, ...]:
@@ -313,11 +328,26 @@ class MeshColorsLayout:
             raise ValueError("Mesh Colors payload exceeds 32-bit slot indices")
 
         faces = np.concatenate(face_chunks, axis=0) if face_chunks else np.zeros((0, 2))
+        # edge adjacency of every mesh once, repeated per instance
+        # (mesh_colors.py:174-206, 256)
+        from .mesh_colors_adjacency import MeshColorsAdjacencyDiagnostics, build_triangle_adjacency
+
+        mesh_adjacency = {}
+        for mesh_id, _, _ in scene_node.instances:
+            if mesh_id not in mesh_adjacency:
+                mesh_adjacency[mesh_id] = build_triangle_adjacency(
+                    np.asarray(scene_node.meshes[mesh_id].indices).reshape(-1, 3))
+        diagnostics = MeshColorsAdjacencyDiagnostics()
+        for adj in mesh_adjacency.values():
+            diagnostics = diagnostics + adj.diagnostics
+        adjacency = (np.concatenate([mesh_adjacency[m].packed for m, _, _ in scene_node.instances], axis=0)
+                     if scene_node.instances else np.zeros((0, 4), dtype=np.uint32))
         return cls(
             faces.astype(np.uint32),
             np.array(instance_rows, dtype=np.uint64).astype(np.uint32).reshape(-1, 4),
             np.array(side_rows, dtype=np.uint64).astype(np.uint32).reshape(-1, 2),
             total_surface, total_payload, texels_per_unit, min_resolution, max_resolution,
+            adjacency, diagnostics,
         )
 
     @classmethod

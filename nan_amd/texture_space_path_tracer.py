@@ -63,7 +63,7 @@ class TextureSpacePathTracer:
         self.face_infos_buffer = torch.from_numpy(layout.face_table.view(np.int32)).to(dev)
         self.instance_infos_buffer = torch.from_numpy(layout.instance_table.view(np.int32)).to(dev)
         self.side_infos_buffer = torch.from_numpy(layout.side_table.view(np.int32)).to(dev)
-        self.adjacency_infos_buffer = None
+        self.adjacency_infos_buffer = torch.from_numpy(layout.adjacency_table.view(np.int32)).to(dev)
 
     def set_mode(self, mode: str) -> None:
         """``"wavefront"`` (staged kernels, default) or ``"megakernel"`` (one

@@ -5,9 +5,10 @@ cache and chains trace -> resolve -> accumulate -> tone map
 The state machine is the reference's: reset / pause / save-to-RGB9E5 /
 frozen resolve, camera-move resets, and the save failure path that resets
 the texture space.  UI controls are optional attributes that stay ``None``
-in headless use (their defaults then apply, as in the reference).  The
-pre-save surface filter is used when a ``surface_filter`` object is
-attached; without one the save packs the working cache directly.
+in headless use (their defaults then apply, as in the reference).  Before
+a save, the Mesh Colors surface filter runs its ping-pong passes
+(texture_space_path_tracing_renderer.py:181-231); with no ``surface_filter``
+attached the save packs the working cache directly.
 """
 
 from __future__ import annotations
@@ -18,6 +19,7 @@ import torch
 
 from .device import CommandEncoder, Device
 from .mesh_colors import MESH_COLORS_PAYLOAD_SIZE, MeshColorsLayout
+from .mesh_colors_surface_filter import MeshColorsSurfaceFilter
 from .passes import Accumulator, MeshColorsResolve, MeshColorsRGB9E5Packer, ToneMapper
 from .render_data import RenderData
 from .scene import Scene
@@ -69,6 +71,13 @@ class TextureSpacePathTracingRenderer:
         self.texture_space_path_tracer = TextureSpacePathTracer(
             device, scene, self.layout, samples_per_texel=self.samples_per_texel, max_bounces=self.max_bounces)
         self.resolve = MeshColorsResolve(device, scene, self.layout, self.texture_space_path_tracer)
+        self.surface_filter = MeshColorsSurfaceFilter(
+            device, scene, self.layout,
+            face_infos_buffer=self.texture_space_path_tracer.face_infos_buffer,
+            instance_infos_buffer=self.texture_space_path_tracer.instance_infos_buffer,
+            side_infos_buffer=self.texture_space_path_tracer.side_infos_buffer,
+            adjacency_infos_buffer=self.texture_space_path_tracer.adjacency_infos_buffer,
+            texture_space_path_tracer=self.texture_space_path_tracer)
         self.rgb9e5_packer = MeshColorsRGB9E5Packer(device)
         self.accumulator = Accumulator(device, resource_key="texture_space_renderer.accumulator_history")
         self.tone_mapper = ToneMapper(device)

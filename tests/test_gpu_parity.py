@@ -328,6 +328,25 @@ def test_renderer_frames_and_freeze(device, nan):
     assert r.texture_iteration == 4
     img = out.cpu().numpy()
     assert np.isfinite(img).all() and img[..., :3].max() > 0.05
+    # the save runs 5 surface-filter passes, then packs (texture_space_path_tracing_renderer.py:181-231)
+    from nan_amd.mesh_colors_surface_filter import face_normals
+    from nan_amd.texture_space_path_tracing_renderer import MESH_COLORS_BUFFER_KEY
+    from oracle import filter_oracle
+
+    cache = rd.get_buffer(MESH_COLORS_BUFFER_KEY, struct_size=16,
+                          element_count=r.layout.total_payload_count).cpu().numpy().view(np.float32).reshape(-1, 4)
     r._request_rgb9e5_save()
     r.render(device.create_command_encoder(), out, 5, device, sc, rd)
     assert r._frozen_rgb9e5 is not None and r.texture_iteration == 4
+    lay = r.layout
+    normals = face_normals(node, lay)
+    want = cache
+    for _ in range(5):
+        want = filter_oracle.surface_filter(lay.face_table, lay.instance_table, lay.side_table, lay.adjacency_table,
+                                            normals, want, np.exp(-0.5), np.radians(30.0))
+    words = r._frozen_rgb9e5.cpu().numpy().view(np.uint32).reshape(-1)[:lay.total_payload_count]
+    got = np.stack([oracle.unpack_rgb9e5(int(w))[:3] for w in words[::97]])
+    ref = want[::97, :3].astype(np.float64)
+    # RGB9E5 keeps 9 mantissa bits of the largest channel
+    tol = np.maximum(ref.max(axis=1, keepdims=True), 1e-6) * 2.0 ** -8
+    assert (np.abs(got - ref) <= tol).all()
