@@ -15,6 +15,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 import torch
 
 from .device import CommandEncoder, Device
@@ -129,6 +131,30 @@ class TextureSpacePathTracingRenderer:
         self._save_error = None
         if self.save_rgb9e5_button is not None:
             self.save_rgb9e5_button.enabled = True
+
+    # -- on-disk frozen cache (no reference counterpart: it freezes in memory) --
+    def write_frozen_cache(self, path) -> None:
+        """Writes the current RGB9E5 frozen cache with its layout tables
+        (:mod:`nan_amd.frozen_cache`)."""
+        from .frozen_cache import save_frozen_cache
+
+        if self._frozen_rgb9e5 is None:
+            raise RuntimeError("no frozen RGB9E5 cache to write; request a save first")
+        save_frozen_cache(path, self.layout, self._frozen_rgb9e5)
+
+    def read_frozen_cache(self, path, render_data: RenderData) -> None:
+        """Loads a frozen cache written for this renderer's layout and shows
+        it (frozen resolve) until the texture space is reset."""
+        from .frozen_cache import load_frozen_cache
+
+        words, _ = load_frozen_cache(path, self.layout)
+        packed = render_data.get_buffer(MESH_COLORS_RGB9E5_KEY, struct_size=4,
+                                        element_count=self.layout.total_payload_count)
+        packed.view(-1).copy_(torch.from_numpy(words.view(np.float32)))
+        self._frozen_rgb9e5 = packed
+        self._save_requested = False
+        self._save_error = None
+        self.reset_accumulator = True
 
     def _request_rgb9e5_save(self) -> None:
         if self._save_requested or self._frozen_rgb9e5 is not None:
