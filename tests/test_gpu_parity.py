@@ -350,3 +350,22 @@ def test_renderer_frames_and_freeze(device, nan):
     # RGB9E5 keeps 9 mantissa bits of the largest channel
     tol = np.maximum(ref.max(axis=1, keepdims=True), 1e-6) * 2.0 ** -8
     assert (np.abs(got - ref) <= tol).all()
+    # write the frozen cache to disk, load it into a fresh renderer: same frozen resolve
+    import tempfile
+    from pathlib import Path
+
+    node.camera.begin_frame(64, 64)
+    r.reset_accumulator = True  # one frame of the frozen resolve, same camera jitter for both renderers
+    r.render(device.create_command_encoder(), out, 6, device, sc, rd)
+    frozen_img = out.clone()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = Path(tmp) / "cornell.nmc"
+        r.write_frozen_cache(path)
+        r2 = TextureSpacePathTracingRenderer()
+        r2.initialize(device, sc)
+        rd2 = RenderData(device)
+        r2.read_frozen_cache(path, rd2)
+    out2 = torch.zeros_like(out)
+    r2.render(device.create_command_encoder(), out2, 6, device, sc, rd2)
+    assert r2.texture_iteration == 0 and r2._frozen_rgb9e5 is not None
+    assert torch.equal(out2, frozen_img)
