@@ -97,6 +97,12 @@ NAN_API void nan_scene_destroy(NanScene *scene);
 NAN_API int nan_scene_info(const NanScene *scene, NanSceneInfo *out);
 /* Replaces the lighting record (scene.slang:7-17 uniforms). */
 NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16);
+/* Attaches the sky-view table (float[200][200][4], device memory the caller keeps
+ * alive; NULL detaches): rays that leave the scene then read Scene.sample_sky_lut
+ * (scene.slang:469-518) times the sun's outer-space illuminance instead of the
+ * constant sky of the lighting record.  Batches copy the table of each scene at
+ * nan_batch_create. */
+NAN_API int nan_scene_set_sky_view_lut(NanScene *scene, const float *sky_view_lut);
 
 /* Closest hit and occlusion for rays float[n][8] (origin, tmin, dir, tmax) in device
  * memory; writes hits float[n][4] (t, b1, b2, 0; t < 0 on a miss), ids uint32[n][2]
@@ -193,6 +199,35 @@ NAN_API int nan_rgb9e5_unpack(const uint32_t *packed, float *out, uint64_t n, vo
  * camera16 (host memory): position, jitter_x, image_u, jitter_y, image_v, near, image_w, far. */
 NAN_API int nan_resolve(NanTracer *tracer, const void *cache, int frozen, const float *camera16, float *out,
                         uint32_t width, uint32_t height, void *stream);
+
+/* One pass of the Mesh Colors surface filter (MeshColorsSurfaceFilter.execute,
+ * mesh_colors_surface_filter.py:96-128; kernel mesh_colors_surface_filter.slang:90-253):
+ * every slot of the tracer's layout gets its one-ring, normal-weighted mean
+ * from src into dst (payload slots, distinct buffers; sample counts copied).
+ * adjacency uint32[F][4] (packed edge records of mesh_colors_adjacency.py:20-57,
+ * adjacent faces instance-local), face_normals float[F][4] (world-space unit
+ * normals); device memory.  neighbor_weight = exp(-0.5 / spatial_sigma^2). */
+NAN_API int nan_surface_filter(NanTracer *tracer, const uint32_t *adjacency, const float *face_normals,
+                               const void *src, void *dst, float neighbor_weight, float normal_sigma_radians,
+                               void *stream);
+
+/* Atmosphere lookup tables (AtmosphereTransmittanceLUT, AtmosphereMultiScatteringLUT,
+ * AtmosphereSkyViewLUT .generate, atmosphere.py:4-225; kernels atmosphere.slang:143-397).
+ * Each table is a float4 image in device memory, rows [y][x]: transmittance 256x64,
+ * multiple scattering 32x32 (reads the transmittance table), sky view 200x200 (reads
+ * both, for a sun direction given on the host). */
+NAN_API int nan_atmosphere_transmittance_lut(float *out, void *stream);
+NAN_API int nan_atmosphere_multiscattering_lut(const float *transmittance, float *out, void *stream);
+NAN_API int nan_atmosphere_sky_view_lut(const float *transmittance, const float *multiscattering,
+                                        const float *sun_direction3, float *out, void *stream);
+/* Scene.get_sun_transmittance_at_view (scene.slang:997-1009): the transmittance table
+ * at the view point and sun direction (host float[3]) into out4 (device float[4]). */
+NAN_API int nan_atmosphere_view_transmittance(const float *transmittance, const float *sun_direction3, float *out4,
+                                              void *stream);
+/* Scene.sample_sky_lut (scene.slang:469-518) for n directions dirs float[n][4]
+ * (device) into out float[n][4] (device), with the sun of lighting16 (host). */
+NAN_API int nan_sky_view_lookup(const float *sky_view_lut, const float *lighting16, const float *dirs, uint64_t n,
+                                float *out, void *stream);
 
 /* Screen accumulation (accumulator.slang:3-29) and LPM tone map + Rec.709
  * (tone_mapper.slang:8-90) over n float4 pixels. */

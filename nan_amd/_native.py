@@ -92,6 +92,8 @@ EXPORTED_SYMBOLS = (
     "nan_nrc_create", "nan_nrc_destroy", "nan_nrc_param_counts", "nan_nrc_get_params", "nan_nrc_set_params",
     "nan_nrc_infer", "nan_nrc_train", "nan_nrc_forward_raw", "nan_nrc_backward_raw", "nan_nrc_encode",
     "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned", "nan_surface_filter",
+    "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
+    "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
 )
 
 _LIB = None
@@ -109,6 +111,12 @@ def _declare(lib) -> None:
         "nan_scene_destroy": (None, [vp]),
         "nan_scene_info": (I, [vp, ctypes.POINTER(NanSceneInfo)]),
         "nan_scene_set_lighting": (I, [vp, f32p]),
+        "nan_scene_set_sky_view_lut": (I, [vp, vp]),
+        "nan_atmosphere_transmittance_lut": (I, [vp, vp]),
+        "nan_atmosphere_multiscattering_lut": (I, [vp, vp, vp]),
+        "nan_atmosphere_sky_view_lut": (I, [vp, vp, f32p, vp, vp]),
+        "nan_atmosphere_view_transmittance": (I, [vp, f32p, vp, vp]),
+        "nan_sky_view_lookup": (I, [vp, f32p, vp, U64, vp, vp]),
         "nan_trace_rays": (I, [vp, vp, U64, vp, vp, vp, vp]),
         "nan_tracer_create": (I, [vp, u32p, U32, u32p, u32p, U32, U64, U64, ctypes.POINTER(vp), ctypes.c_char_p,
                                   ctypes.c_size_t]),

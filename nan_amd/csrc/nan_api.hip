@@ -115,7 +115,7 @@ struct DevCamera {
 };
 
 __device__ f3 sky_with_sun_disk(const Light &L, f3 dir) {
-    f3 out = L.sky;
+    f3 out = miss_radiance(L, dir);
     const float ground = 6.360f, vy = 6.360f + 0.0002f;
     f3 oc = mk(0.0f, vy, 0.0f);
     float a = dot(dir, dir), bq = 2.0f * dot(oc, dir), c = dot(oc, oc) - ground * ground;
@@ -251,11 +251,19 @@ __global__ void tonemap_kernel(const float4 *in, float4 *out, uint64_t n, float 
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-namespace {
+namespace nanx {
 
+// Horizon angle seen from the view point 200 m above the ground
+// (scene.slang:474): FP32 up to the arccosine, which is taken in double and
+// rounded, so every build of this library (and the oracle) holds one value.
+float atmosphere_horizon() {
+    const float ground = 6.360f, vy = 6.360f + 0.0002f;
+    float h = sqrtf(vy * vy);
+    float q = sqrtf(h * h - ground * ground) / h;
+    return (float)std::acos((double)std::fmin(std::fmax(q, -1.0f), 1.0f)) - 0.5f * kPi;
+}
 
-
-Light make_light(const float *L) {
+Light make_light(const float *L, const float4 *sky_lut) {
     Light l;
     l.sun_dir = mk(L[0], L[1], L[2]);
     l.cos_half = L[3];
@@ -264,8 +272,16 @@ Light make_light(const float *L) {
     l.sun_illum = mk(L[4] * L[8], L[5] * L[9], L[6] * L[10]);
     l.sky = mk(L[12], L[13], L[14]);
     l.sun_on = l.sun_illum.x > 0.0f || l.sun_illum.y > 0.0f || l.sun_illum.z > 0.0f;
+    l.sky_lut = sky_lut;
+    l.horizon = atmosphere_horizon();
     return l;
 }
+
+}  // namespace nanx
+
+namespace {
+
+
 
 template <int STACK>
 void launch_megakernel(const NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
@@ -517,7 +533,8 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     float max_coord = 1.0f;
     for (float v : world) max_coord = std::max(max_coord, std::fabs(v));
     s->dev.slack = max_coord * 0x1p-18f;
-    s->dev.light = make_light(d->lighting);
+    std::memcpy(s->lighting, d->lighting, sizeof(s->lighting));
+    s->dev.light = make_light(s->lighting, nullptr);
     {
         SceneRec rec{};
         rec.root = s->dev.root;
@@ -555,15 +572,27 @@ NAN_API int nan_scene_info(const NanScene *scene, NanSceneInfo *out) {
     return 0;
 }
 
-NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16) {
-    if (!scene || !lighting16) return fail(nullptr, 0, "nan_scene_set_lighting: null argument");
-    scene->dev.light = make_light(lighting16);
+static int refresh_light(NanScene *scene) {
+    scene->dev.light = make_light(scene->lighting, scene->sky_lut);
     SceneRec rec{};
     rec.root = scene->dev.root;
     rec.slack = scene->dev.slack;
     rec.light = scene->dev.light;
     HIP_TRY(hipMemcpy((void *)scene->dev.scenes, &rec, sizeof(rec), hipMemcpyHostToDevice), nullptr, 0);
     return 0;
+}
+
+NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16) {
+    if (!scene || !lighting16) return fail(nullptr, 0, "nan_scene_set_lighting: null argument");
+    std::memcpy(scene->lighting, lighting16, sizeof(scene->lighting));
+    return refresh_light(scene);
+}
+
+NAN_API int nan_scene_set_sky_view_lut(NanScene *scene, const float *sky_view_lut) {
+    if (!scene) return fail(nullptr, 0, "nan_scene_set_sky_view_lut: null scene");
+    if (scene->batch) return fail(nullptr, 0, "nan_scene_set_sky_view_lut: set the table on each scene of a batch");
+    scene->sky_lut = reinterpret_cast<const float4 *>(sky_view_lut);
+    return refresh_light(scene);
 }
 
 NAN_API int nan_trace_rays(NanScene *scene, const float *rays, uint64_t n, float *hits, uint32_t *ids,

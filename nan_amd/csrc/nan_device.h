@@ -54,6 +54,30 @@ __device__ __forceinline__ float saturate(float x) { return fminf(fmaxf(x, 0.0f)
 __device__ __forceinline__ float fracf(float x) { return x - floorf(x); }
 __device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
+// atan2 and asin from correctly rounded operations only, so that the sky
+// lookup below is bit-identical to the oracle's: arctan on [0, 1] by the
+// polynomial of Abramowitz & Stegun 4.4.49 (|error| <= 2e-8), extended to
+// all octants by atan(a) = pi/2 - atan(1/a) and the signs of y and x.
+__device__ __forceinline__ float nan_atan2(float y, float x) {
+    const float c[9] = {1.0f,           -0.3333314528f, 0.1999355085f, -0.1420889944f, 0.1065626393f,
+                        -0.0752896400f, 0.0429096138f,  -0.0161657367f, 0.0028662257f};
+    float ax = fabsf(x), ay = fabsf(y);
+    float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    if (!(mx > 0.0f)) return 0.0f;
+    float a = mn / mx, s = a * a;
+    float p = c[8];
+    for (int k = 7; k >= 0; --k) p = fmaf(p, s, c[k]);
+    float r = a * p;
+    if (ay > ax) r = 0.5f * kPi - r;
+    if (x < 0.0f) r = kPi - r;
+    return y < 0.0f ? -r : r;
+}
+__device__ __forceinline__ float nan_asin(float x) {
+    x = fminf(fmaxf(x, -1.0f), 1.0f);
+    return nan_atan2(x, sqrtf((1.0f - x) * (1.0f + x)));
+}
+
+
 // sin(2*pi*u), cos(2*pi*u) for u in [0, 1]: exact quarter-turn reduction on
 // u, then degree-9/10 Taylor polynomials on [-pi/4, pi/4].
 __device__ __forceinline__ void sincos_2pi(float u, float *s_out, float *c_out) {
@@ -100,9 +124,55 @@ struct Light {
     f3 sun_illum;     // color * intensity * transmittance
     f3 outer;         // color * intensity
     f3 transmittance;
-    f3 sky;
+    f3 sky;                 // constant sky radiance (used when sky_lut is null)
     int sun_on;
+    const float4 *sky_lut;  // sky-view table float4[200][200] (scene.slang:469-518), or null
+    float horizon;          // horizon angle of the view point, radians (sky_lut only)
 };
+
+// Sky-view table lookup (Scene.sample_sky_lut, scene.slang:469-518): the
+// inverse of the table's altitude/azimuth mapping around the sun, bilinear
+// with wrap-around addressing (the reference's default sampler), times the
+// sun's outer-space illuminance.  A direction along the vertical has no
+// azimuth; it reads azimuth 0.
+__device__ __forceinline__ f3 sky_view_radiance(const Light &L, f3 dir) {
+    const f3 view = mk(0.0f, 6.360f + 0.0002f, 0.0f);
+    float height = sqrtf(dot(view, view));
+    f3 up = mk(view.x / height, view.y / height, view.z / height);
+    float alt = fminf(fmaxf(nan_asin(dot(dir, up)), -0.5f * kPi + 1e-2f), 0.5f * kPi - 1e-2f);
+    float az = 0.0f;
+    if (!(fabsf(alt) > 0.5f * kPi - 0.0001f)) {
+        f3 right = cross(L.sun_dir, up);
+        f3 forward = cross(up, right);
+        f3 p = dir - up * dot(dir, up);
+        if (dot(p, p) > 0.0f) {
+            p = normalize(p);
+            az = nan_atan2(dot(p, right), dot(p, forward));
+        }
+    }
+    float adj = (alt + L.horizon) / (0.5f * kPi);
+    float v = adj < 0.0f ? 0.5f * (1.0f - sqrtf(fminf(fmaxf(-adj, 0.0f), 1.0f)))
+                         : 0.5f * (1.0f + sqrtf(fminf(fmaxf(adj, 0.0f), 1.0f)));
+    float u = az / (2.0f * kPi) + 0.5f;
+    if (!(u >= 0.0f && u <= 1.0f)) u = 0.5f;  // a NaN direction reads a valid texel
+    if (!(v >= 0.0f && v <= 1.0f)) v = 0.5f;
+    float x = u * 200.0f - 0.5f, y = v * 200.0f - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float tx = x - fx, ty = y - fy;
+    int x0 = (int)fx, y0 = (int)fy;  // u, v lie in [0, 1]: x0, y0 in [-1, 199]
+    int xa = (x0 + 200) % 200, xb = (x0 + 201) % 200, ya = (y0 + 200) % 200, yb = (y0 + 201) % 200;
+    float4 a = L.sky_lut[ya * 200 + xa], b = L.sky_lut[ya * 200 + xb];
+    float4 c = L.sky_lut[yb * 200 + xa], d = L.sky_lut[yb * 200 + xb];
+    f3 r0 = mk(a.x, a.y, a.z) * (1.0f - tx) + mk(b.x, b.y, b.z) * tx;
+    f3 r1 = mk(c.x, c.y, c.z) * (1.0f - tx) + mk(d.x, d.y, d.z) * tx;
+    return (r0 * (1.0f - ty) + r1 * ty) * L.outer;
+}
+
+// radiance of a ray that leaves the scene: the sky-view table when the scene
+// has one, else the constant sky
+__device__ __forceinline__ f3 miss_radiance(const Light &L, f3 dir) {
+    return L.sky_lut ? sky_view_radiance(L, dir) : L.sky;
+}
 
 // Per-scene record for kernels that trace several scenes in one launch.
 struct SceneRec {

@@ -75,6 +75,8 @@ struct NanScene {
     std::vector<float> h_tris, h_shade, h_xf, h_itf;
     std::vector<uint32_t> h_mats;
     bool batch = false;  // assembled from several scenes (wavefront only)
+    float lighting[16] = {};            // the lighting record last set
+    const float4 *sky_lut = nullptr;    // sky-view table (caller-owned device memory) or null
     ~NanScene() {
         for (void *p : allocations) hipFree(p);
     }
@@ -209,6 +211,10 @@ struct StageTimer {
 };
 
 inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
+
+// Lighting record (float[16]) plus the scene's sky-view table -> kernel Light.
+Light make_light(const float *lighting16, const float4 *sky_lut);
+float atmosphere_horizon();
 
 inline int check_launch(const char *what) {
     hipError_t e = hipGetLastError();

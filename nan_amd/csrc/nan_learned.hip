@@ -181,7 +181,7 @@ __device__ f3 suffix_radiance(const DevScene &S, Shading sd, uint32_t max_bounce
             pc.tris += st.tris;
         }
         if (!hit) {
-            rad = rad + T * L.sky;
+            rad = rad + T * miss_radiance(L, r.d);
             break;
         }
         f3 dir = r.d;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLay
                     put3(B.q_alb, q, y1.diffuse);
                     B.q_w[q] = fh.weight;
                 } else {
-                    if (fh.weight > 0.0f) acc = acc + L.sky * fh.weight;
+                    if (fh.weight > 0.0f) acc = acc + miss_radiance(L, fh.dir) * fh.weight;
                     put3(B.q_pos, q, mk(0.0f, 0.0f, 0.0f));
                     put3(B.q_nrm, q, mk(0.0f, 0.0f, 0.0f));
                     put3(B.q_alb, q, mk(0.0f, 0.0f, 0.0f));

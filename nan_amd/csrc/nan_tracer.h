@@ -52,7 +52,7 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
             pc.nodes += st.nodes;
             pc.tris += st.tris;
         }
-        if (!hit) return rad + T * L.sky;
+        if (!hit) return rad + T * miss_radiance(L, dir);
         Shading sd;
         shading_data(S, h.id, h.b1, h.b2, -dir, sd);
         rad = rad + T * sd.emissive;

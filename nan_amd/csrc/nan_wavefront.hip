@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W,
             f3 T = mk(Tv.x, Tv.y, Tv.z), rad = mk(Rv.x, Rv.y, Rv.z);
             uint32_t id = __float_as_uint(hv.w);
             if (id == kInvalid) {
-                rad = rad + T * L.sky;
+                rad = rad + T * miss_radiance(L, mk(rd.x, rd.y, rd.z));
             } else {
                 Rng rng;
                 rng.state = W.rng[p];

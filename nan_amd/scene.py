@@ -37,6 +37,8 @@ class Scene:
         self.scene_node = scene_node
         self.camera = scene_node.camera
         self.arrays = flatten_scene(scene_node)
+        self.atmosphere = None
+        self._apply_atmosphere(scene_node.lighting)
         a = self.arrays
         desc = _native.NanSceneDesc(
             _f32(a["vertices"]), a["vertices"].shape[0],
@@ -58,6 +60,26 @@ class Scene:
         self.info = _native.NanSceneInfo()
         _native.check(lib.nan_scene_info(self.handle, ctypes.byref(self.info)), "nan_scene_info")
         self.frame_index = 0
+        self._attach_sky()
+
+    def _apply_atmosphere(self, lighting) -> None:
+        """With ``lighting.atmosphere``, (re)generates the tables for its sun
+        and writes the view-point transmittance into the lighting record."""
+        if not lighting.atmosphere:
+            return
+        from .atmosphere import SkyAtmosphere
+
+        sun = self.arrays["lighting"][0:3].copy()
+        if self.atmosphere is None:
+            self.atmosphere = SkyAtmosphere(self.device, sun)
+        else:
+            self.atmosphere.set_sun_direction(sun)
+        self.arrays["lighting"][8:11] = self.atmosphere.get_sun_transmittance_at_view()
+
+    def _attach_sky(self) -> None:
+        lut = self.atmosphere.sky_view_lut.data_ptr() \
+            if self.atmosphere is not None and self.scene_node.lighting.atmosphere else None
+        _native.check(_native.lib().nan_scene_set_sky_view_lut(self.handle, lut), "nan_scene_set_sky_view_lut")
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -87,8 +109,10 @@ class Scene:
     def set_lighting(self, lighting) -> None:
         self.scene_node.lighting = lighting
         self.arrays["lighting"] = lighting.pack()
+        self._apply_atmosphere(lighting)
         _native.check(_native.lib().nan_scene_set_lighting(self.handle, _f32(self.arrays["lighting"])),
                       "nan_scene_set_lighting")
+        self._attach_sky()
 
     def trace_rays(self, rays: torch.Tensor, *, occlusion: bool = True):
         """Closest hit (and occlusion) for ``rays`` float32[n, 8] on the

@@ -23,7 +23,7 @@ from .transform import Transform
 
 @dataclass
 class Lighting:
-    """Directional sun plus a constant sky.
+    """Directional sun plus a constant sky or the physical atmosphere.
 
     ``sun_direction`` points toward the sun.  Sun next-event estimation
     returns ``visibility * color * intensity * transmittance * n.l``
@@ -37,8 +37,19 @@ class Lighting:
     sun_intensity: float = math.pi
     sun_transmittance: tuple = (1.0, 1.0, 1.0)
     sky_radiance: tuple = (0.0, 0.0, 0.0)
+    # physical sky: misses read the sky-view table and the sun's transmittance
+    # comes from the transmittance table at the view point (scene.slang:469-518,
+    # 997-1009); sun_transmittance and sky_radiance are then not used
+    atmosphere: bool = False
 
     RECORD_FLOATS = 16
+
+    @classmethod
+    def reference_default(cls) -> "Lighting":
+        """The reference's default environment (scene.py:136-147): sun toward
+        (0, 0.5, 0.866), intensity pi, white, 0.5357 degree disk, and the
+        atmosphere."""
+        return cls(atmosphere=True)
 
     @classmethod
     def black(cls) -> "Lighting":

@@ -83,6 +83,16 @@ def liboracle():
     lib.orc_set_crosscheck.argtypes = [ctypes.c_int]
     lib.orc_crosscheck_result.restype = ctypes.c_int
     lib.orc_crosscheck_result.argtypes = [f32p]
+    lib.orc_scene_set_sky_view_lut.argtypes = [ctypes.c_void_p, f32p]
+    lib.orc_sky_view_lookup.argtypes = [f32p, f32p, f32p, ctypes.c_uint64, f32p]
+    lib.orc_atmo_scattering.argtypes = [f32p, f32p]
+    lib.orc_atmo_sphere_hit.restype = ctypes.c_float
+    lib.orc_atmo_sphere_hit.argtypes = [f32p, f32p, ctypes.c_float]
+    lib.orc_atmo_sun_transmittance.argtypes = [f32p, f32p, f32p]
+    lib.orc_atmo_transmittance_lut.argtypes = [f32p]
+    lib.orc_atmo_multiscattering_lut.argtypes = [f32p, f32p]
+    lib.orc_atmo_sky_view_lut.argtypes = [f32p, f32p, f32p, f32p]
+    lib.orc_atmo_view_transmittance.argtypes = [f32p, f32p, f32p]
     _LIB = lib
     return lib
 
@@ -169,6 +179,11 @@ class OracleScene:
             _p(a["lighting"], f32p),
         )
         self.handle = self.lib.orc_scene_create(ctypes.byref(self.desc), 1 if use_bvh else 0)
+
+    def set_sky_view_lut(self, lut) -> None:
+        """Attaches a sky-view table float32[200, 200, 4] (None detaches)."""
+        self.sky_lut = None if lut is None else np.ascontiguousarray(lut, dtype=np.float32).reshape(200, 200, 4)
+        self.lib.orc_scene_set_sky_view_lut(self.handle, _p(self.sky_lut, f32p))
 
     def close(self):
         if self.handle:

@@ -327,6 +327,7 @@ typedef struct {
     uint32_t node_count;
     uint32_t *order;     /* leaf triangle order */
     int use_bvh;
+    const float *sky_lut; /* sky-view table [200][200][4] (caller-owned) or NULL */
     float slack;         /* absolute box-test slack, see box_hit */
 } orc_scene;
 
@@ -471,6 +472,8 @@ ORC_API orc_scene *orc_scene_create(const orc_scene_desc *desc, int use_bvh) {
     if (use_bvh) build_bvh(s);
     return s;
 }
+
+ORC_API void orc_scene_set_sky_view_lut(orc_scene *s, const float *sky_lut) { s->sky_lut = sky_lut; }
 
 ORC_API void orc_scene_destroy(orc_scene *s) {
     if (!s) return;
@@ -879,18 +882,99 @@ typedef struct {
 typedef struct {
     f3 sun_dir; float cos_half;
     f3 sun_illum;  /* color * intensity * transmittance */
+    f3 outer;      /* color * intensity */
     f3 sky;
     int sun_on;
+    const float *sky_lut;  /* sky-view table [200][200][4] or NULL */
+    float horizon;
 } orc_light;
 
-static orc_light load_light(const float *L) {
+/* horizon angle of the view point (scene.slang:474): FP32 up to the
+ * arccosine, taken in double and rounded */
+static float atmosphere_horizon(void) {
+    const float ground = 6.360f, vy = 6.360f + 0.0002f;
+    float h = sqrtf(vy * vy);
+    float q = sqrtf(h * h - ground * ground) / h;
+    return (float)acos((double)fminf(fmaxf(q, -1.0f), 1.0f)) - 0.5f * ORC_PI;
+}
+
+static orc_light load_light_lut(const float *L, const float *sky_lut) {
     orc_light l;
     l.sun_dir = mk(L[0], L[1], L[2]);
     l.cos_half = L[3];
     l.sun_illum = mk(L[4] * L[8], L[5] * L[9], L[6] * L[10]);
+    l.outer = mk(L[4], L[5], L[6]);
     l.sky = mk(L[12], L[13], L[14]);
     l.sun_on = l.sun_illum.x > 0.0f || l.sun_illum.y > 0.0f || l.sun_illum.z > 0.0f;
+    l.sky_lut = sky_lut;
+    l.horizon = atmosphere_horizon();
     return l;
+}
+
+/* atan2 from correctly rounded operations: Abramowitz & Stegun 4.4.49 on
+ * [0, 1] (|error| <= 2e-8) with octant reduction; asin through it. */
+static float nan_atan2(float y, float x) {
+    static const float c[9] = {1.0f,           -0.3333314528f, 0.1999355085f, -0.1420889944f, 0.1065626393f,
+                               -0.0752896400f, 0.0429096138f,  -0.0161657367f, 0.0028662257f};
+    float ax = fabsf(x), ay = fabsf(y);
+    float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    if (!(mx > 0.0f)) return 0.0f;
+    float a = mn / mx, s2 = a * a;
+    float p = c[8];
+    for (int k = 7; k >= 0; --k) p = fmaf(p, s2, c[k]);
+    float r = a * p;
+    if (ay > ax) r = 0.5f * ORC_PI - r;
+    if (x < 0.0f) r = ORC_PI - r;
+    return y < 0.0f ? -r : r;
+}
+static float nan_asin(float x) {
+    x = fminf(fmaxf(x, -1.0f), 1.0f);
+    return nan_atan2(x, sqrtf((1.0f - x) * (1.0f + x)));
+}
+
+/* Scene.sample_sky_lut (scene.slang:469-518), bilinear with wrap addressing */
+static f3 sky_view_radiance(const orc_light *L, f3 dir) {
+    const f3 view = mk(0.0f, 6.360f + 0.0002f, 0.0f);
+    float height = sqrtf(dot(view, view));
+    f3 up = mk(view.x / height, view.y / height, view.z / height);
+    float alt = fminf(fmaxf(nan_asin(dot(dir, up)), -0.5f * ORC_PI + 1e-2f), 0.5f * ORC_PI - 1e-2f);
+    float az = 0.0f;
+    if (!(fabsf(alt) > 0.5f * ORC_PI - 0.0001f)) {
+        f3 right = cross(L->sun_dir, up);
+        f3 forward = cross(up, right);
+        f3 p = sub(dir, muls(up, dot(dir, up)));
+        if (dot(p, p) > 0.0f) {
+            p = normalize(p);
+            az = nan_atan2(dot(p, right), dot(p, forward));
+        }
+    }
+    float adj = (alt + L->horizon) / (0.5f * ORC_PI);
+    float v = adj < 0.0f ? 0.5f * (1.0f - sqrtf(fminf(fmaxf(-adj, 0.0f), 1.0f)))
+                         : 0.5f * (1.0f + sqrtf(fminf(fmaxf(adj, 0.0f), 1.0f)));
+    float u = az / (2.0f * ORC_PI) + 0.5f;
+    if (!(u >= 0.0f && u <= 1.0f)) u = 0.5f;
+    if (!(v >= 0.0f && v <= 1.0f)) v = 0.5f;
+    float x = u * 200.0f - 0.5f, y = v * 200.0f - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float tx = x - fx, ty = y - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    int xa = (x0 + 200) % 200, xb = (x0 + 201) % 200, ya = (y0 + 200) % 200, yb = (y0 + 201) % 200;
+    const float *a = L->sky_lut + 4 * (ya * 200 + xa), *b = L->sky_lut + 4 * (ya * 200 + xb);
+    const float *c = L->sky_lut + 4 * (yb * 200 + xa), *d = L->sky_lut + 4 * (yb * 200 + xb);
+    f3 r0 = add(muls(mk(a[0], a[1], a[2]), 1.0f - tx), muls(mk(b[0], b[1], b[2]), tx));
+    f3 r1 = add(muls(mk(c[0], c[1], c[2]), 1.0f - tx), muls(mk(d[0], d[1], d[2]), tx));
+    return mul(add(muls(r0, 1.0f - ty), muls(r1, ty)), L->outer);
+}
+
+static f3 miss_radiance(const orc_light *L, f3 dir) { return L->sky_lut ? sky_view_radiance(L, dir) : L->sky; }
+
+ORC_API void orc_sky_view_lookup(const float *sky_lut, const float *lighting16, const float *dirs, uint64_t n,
+                                 float *out) {
+    orc_light L = load_light_lut(lighting16, sky_lut);
+    for (uint64_t k = 0; k < n; ++k) {
+        f3 r = sky_view_radiance(&L, mk(dirs[4 * k], dirs[4 * k + 1], dirs[4 * k + 2]));
+        out[4 * k] = r.x; out[4 * k + 1] = r.y; out[4 * k + 2] = r.z; out[4 * k + 3] = 0.0f;
+    }
 }
 
 static int shadow_visible(const orc_scene *s, f3 origin, f3 dir, orc_counters *cnt) {
@@ -924,7 +1008,7 @@ static f3 trace_radiance(const orc_scene *s, const orc_light *L, f3 origin, f3 d
             if (b.valid != h.valid || b.id != h.id || b.t != h.t) record_bad(&r, 1);
         }
         if (!h.valid) {
-            rad = add(rad, mul(T, L->sky));
+            rad = add(rad, mul(T, miss_radiance(L, dir)));
             return rad;
         }
         orc_shading sd;
@@ -1029,7 +1113,7 @@ static void trace_texel(orc_job *j, uint32_t instance, uint32_t local) {
     const uint32_t *face;
     if (!texel_raygen(j->faces, j->instances, instance, local, j->iteration, j->spp, &h, &rng, &face)) return;
     uint32_t spp = j->spp > 1 ? j->spp : 1;
-    orc_light L = load_light(j->s->d.lighting);
+    orc_light L = load_light_lut(j->s->d.lighting, j->s->sky_lut);
     f3 n = world_flat_normal(j->s, instance, h.prim);
     f3 e = estimate_irradiance(j->s, &L, instance, h.prim, h.b1, h.b2, n, spp, j->max_bounces, &rng, &j->counters);
     uint32_t global_texel = inst[2] + local;
@@ -1162,7 +1246,7 @@ static f3 sample_irradiance(const float *payload, const uint32_t *packed, f3 b, 
 
 /* sky plus sun disk for a primary miss (scene.slang:431-527) */
 static f3 sky_with_sun_disk(const orc_light *L, const float *lighting, f3 dir) {
-    f3 out = L->sky;
+    f3 out = miss_radiance(L, dir);
     /* ground test: ray from (0, 6.3602, 0) against the planet sphere of radius 6.36 */
     const float ground = 6.360f, vy = 6.360f + 0.0002f;
     f3 oc = mk(0.0f, vy, 0.0f);
@@ -1192,7 +1276,7 @@ static f3 sky_with_sun_disk(const orc_light *L, const float *lighting, f3 dir) {
 ORC_API void orc_resolve(const orc_scene *s, const uint32_t *faces, const uint32_t *instances,
                          const uint32_t *sides, const float *payload, const uint32_t *packed, const float *camera,
                          uint32_t width, uint32_t height, uint32_t row_begin, uint32_t row_end, float *out) {
-    orc_light L = load_light(s->d.lighting);
+    orc_light L = load_light_lut(s->d.lighting, s->sky_lut);
     f3 pos = mk(camera[0], camera[1], camera[2]);
     f3 U = mk(camera[4], camera[5], camera[6]), Vv = mk(camera[8], camera[9], camera[10]);
     f3 W = mk(camera[12], camera[13], camera[14]);
