@@ -7,7 +7,10 @@ sky 0.3), texels_per_unit 8, min_resolution 2, max_resolution 16, budget
 16,777,216 slots, 1 sample/texel, 3 bounces, FP32.  ``--workload cornell``
 runs the small Cornell box instead.  The default run also measures the
 learned radiance cache against the plain tracer at equal RMSE on the Cornell
-box (``learned_vs_plain``; ``--workload learned`` runs only that).
+box (``learned_vs_plain``; ``--workload learned`` runs only that) and, on the
+1.21M-triangle scene itself, both learned forms (one bounce into the cache,
+and the cache queried at the texel) against a 512-frame plain bake
+(``learned_vs_plain_at_scale``).
 
 Prints one JSON line.  ``value`` is texel-path samples/s (slot updates per
 second) with the scene, layout and cache resident in HBM.  Every tracer
@@ -237,6 +240,80 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
             "unit": "TFLOP/s", "frac": tflops / MFMA_F16_PEAK_TFLOPS,
         },
     }
+
+
+LEARNED_FORMS = {
+    # name: (texel_query, training records per frame, Adam minibatches)
+    "one_bounce": (False, 65536, 4),
+    "texel_query": (True, 262144, 8),
+}
+
+
+def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 512, frames=(1, 4, 16, 64)) -> dict:
+    """Both learned forms against the plain tracer on the workload's own
+    scene: RMSE against a plain bake at each frame count, the plain frames
+    that reach it and the equal-RMSE speedup (plain frames x plain ms over
+    learned frames x learned ms).  ``one_bounce`` is the configured form
+    (65,536 suffixes per frame, paths end in the cache after one bounce);
+    ``texel_query`` queries the cache at the texel itself (no bounce ray;
+    262,144 training paths per frame in 8 Adam steps)."""
+    import torch
+
+    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+
+    enc = device.create_command_encoder()
+    stream = torch.cuda.current_stream(device.torch_device)
+    slots = layout.total_payload_count
+    dev = device.torch_device
+    bake = torch.zeros((slots, 4), dtype=torch.float32, device=dev)
+    for it in range(bake_frames):  # its own RNG streams: iterations 1,000,000 + k
+        tracer.execute(enc, bake, 1_000_000 + it, reset=(it == 0))
+        if it % 128 == 127:
+            torch.cuda.synchronize()
+            log(f"scale bake {it + 1}/{bake_frames}")
+    b = bake[:, :3].double()
+
+    def rmse(p):
+        return float(torch.sqrt(torch.mean((p[:, :3].double() - b) ** 2)))
+
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    plain = torch.zeros_like(bake)
+    curve = {}
+    plain_ms = 0.0
+    for it in range(max(frames)):
+        ev0.record(stream)
+        tracer.execute(enc, plain, it, reset=(it == 0))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms += ev0.elapsed_time(ev1)
+        if (it + 1) & it == 0:
+            curve[it + 1] = rmse(plain)
+    plain_ms /= max(frames)
+    del plain
+    forms = {}
+    for name, (texel_query, n_train, mb) in LEARNED_FORMS.items():
+        cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=mb))
+        learned = torch.zeros_like(bake)
+        res = {}
+        elapsed = 0.0
+        for it in range(max(frames)):
+            ev0.record(stream)
+            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=n_train,
+                                   texel_query=texel_query)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            elapsed += ev0.elapsed_time(ev1)
+            if it + 1 in frames:
+                r = rmse(learned)
+                m = _plain_frames_for(curve, r)
+                res[str(it + 1)] = {"rmse": r, "plain_frames_for_equal_rmse": m,
+                                    "learned_ms_per_frame": elapsed / (it + 1),
+                                    "speedup_vs_plain": m * plain_ms / elapsed}
+        forms[name] = {"train_samples": n_train, "minibatches": mb, "frames": res}
+        log(f"learned at scale, {name}: " + ", ".join(f"{k}: x{v['speedup_vs_plain']:.2f}" for k, v in res.items()))
+        del cache, learned
+    return {"bake_frames": bake_frames, "plain_ms_per_frame": plain_ms,
+            "plain_rmse": {str(k): v for k, v in curve.items()}, "forms": forms}
 
 
 def bench_learned(args) -> int:
@@ -505,6 +582,8 @@ def main() -> int:
                     default="synthetic")
     ap.add_argument("--bake-frames", type=int, default=1024, help="plain bake length of --workload combined")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-cache comparison")
+    ap.add_argument("--scale-bake-frames", type=int, default=512,
+                    help="plain bake length of the learned comparison on the workload's own scene")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -677,6 +756,11 @@ def main() -> int:
         res = learned_measurement(device)
         log(f"learned measurement {time.perf_counter() - t0:.1f} s")
         out["learned_vs_plain"] = res
+        t0 = time.perf_counter()
+        del payload
+        out["learned_vs_plain_at_scale"] = learned_at_scale(device, scene, layout, tracer,
+                                                             bake_frames=args.scale_bake_frames)
+        log(f"learned at scale {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, 16)

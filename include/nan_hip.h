@@ -315,9 +315,15 @@ NAN_API int nan_nrc_encode(NanNrc *nrc, const float *pos, const float *nrm, cons
  * heuristic; the estimate's expectation is unchanged.  With
  * NAN_LEARNED_RAMPED_MEAN, the k-th iteration since the reset enters the slot
  * with weight proportional to k instead of 1 (the cache improves with the
- * iterations).  counters as in nan_tst_execute_counted (may be NULL).  No
- * reference counterpart. */
-enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2 };
+ * iterations).  With NAN_LEARNED_TEXEL_QUERY, paths end in the cache at x0
+ * itself: training targets are the irradiance of plain paths from x0 less
+ * the sun term (and, with NAN_LEARNED_EMITTER_NEE, with bounce-sampled
+ * emission MIS-weighted against light sampling), and each texel sample is
+ * E = sun + NEE(emitters) + max(0, cache(x0)), with no bounce ray; the cache
+ * then fits the irradiance field over the surfaces (biased by its fit, no
+ * per-texel sampling noise in the indirect term).  counters as in
+ * nan_tst_execute_counted (may be NULL).  No reference counterpart. */
+enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2, NAN_LEARNED_TEXEL_QUERY = 4 };
 NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payload, uint32_t iteration,
                                     uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
                                     uint32_t train_samples, uint32_t flags, uint64_t *counters, void *stream);

@@ -78,6 +78,7 @@ STAGES = ("generate", "extend", "shade", "shadow", "finish", "megakernel", "lear
 
 LEARNED_EMITTER_NEE = 1
 LEARNED_RAMPED_MEAN = 2
+LEARNED_TEXEL_QUERY = 4
 
 TRACER_MEGAKERNEL = 0
 TRACER_WAVEFRONT = 1

@@ -236,7 +236,11 @@ __device__ __forceinline__ bool first_bounce(const DevScene &S, const Shading &s
 
 // Training records: record j draws a front texel uniformly (and, on a
 // double-sided instance, a side), follows one plain path from it and records
-// y1 (position, normal, albedo) with the suffix radiance as target.
+// y1 (position, normal, albedo) with the suffix radiance as target.  In the
+// texel-query form it records x0 instead, with the indirect irradiance of
+// the path as target: (az/pdf) times the sky on a miss, else times the
+// suffix radiance of y1 plus y1's emission (MIS-weighted against the light
+// sampling x0 does when emitters are light-sampled).
 template <int STACK, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayout Ly, uint32_t iteration,
                                                               uint32_t spp, uint32_t max_bounces, uint32_t n,
@@ -248,6 +252,7 @@ __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayo
     uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
     if (j < n) {
         bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && S.emitter_count > 0;
+        bool at_x0 = (flags & NAN_LEARNED_TEXEL_QUERY) != 0;
         uint32_t h = mix32(j ^ mix32(iteration * 0x9E3779B9u + 0x632BE5ABu));
         uint64_t w = ((uint64_t)h * Ly.total_front) >> 32;
         Rng rng;
@@ -261,7 +266,19 @@ __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayo
         FirstHit fh;
         bool valid = first_bounce<STACK, BLOCK, COUNT>(S, sd0, ex, ey, fh, y1, col, pc);
         f3 target = mk(0.0f, 0.0f, 0.0f);
-        if (valid) {
+        if (at_x0) {
+            if (valid) {
+                target = suffix_radiance<STACK, BLOCK, COUNT>(S, y1, max_bounces, nee, rng, col, pc);
+                float wm = nee ? power_weight(fh.pdf, emitter_pdf(S, fh.tri, fh.dir, fh.t)) : 1.0f;
+                target = (target + y1.emissive * wm) * fh.weight;
+            } else if (fh.weight > 0.0f) {
+                target = miss_radiance(S.light, fh.dir) * fh.weight;
+            }
+            valid = true;
+            put3(B.tr_pos, j, sd0.origin_front);
+            put3(B.tr_nrm, j, sd0.frame.n);
+            put3(B.tr_alb, j, sd0.diffuse);
+        } else if (valid) {
             target = suffix_radiance<STACK, BLOCK, COUNT>(S, y1, max_bounces, nee, rng, col, pc);
             put3(B.tr_pos, j, y1.origin_front);
             put3(B.tr_nrm, j, y1.frame.n);
@@ -340,6 +357,49 @@ __global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLay
                     put3(B.q_alb, q, mk(0.0f, 0.0f, 0.0f));
                     B.q_w[q] = 0.0f;
                 }
+            }
+            B.partial[slot] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+        }
+    }
+    if (COUNT) flush_counters(pc, counters);
+}
+
+// Texel-query form: every texel sample takes the sun term at x0 and (with
+// nee) MIS-weighted light-sampled emission, and queries the cache at x0
+// itself for the rest of its irradiance; no bounce ray is traced.
+template <int STACK, int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void learned_texel_direct(DevScene S, DevLayout Ly, uint32_t iteration,
+                                                               uint32_t spp, uint32_t flags, LearnedDev B,
+                                                               unsigned long long *counters) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    uint32_t *col = stack + threadIdx.x;
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
+    uint64_t w = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (w < Ly.total_front) {
+        const Light &L = S.light;
+        bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && S.emitter_count > 0;
+        Rng rng;
+        TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
+        uint32_t nspp = spp > 1u ? spp : 1u;
+        f3 nf = world_flat_normal(S, ts.tri);
+        for (int side = 0; side < 2; ++side) {
+            if (side == 1 && !(ts.side.y & 1u)) break;
+            uint32_t slot = side ? ts.side.x + ts.local : ts.global_texel;
+            Shading sd;
+            shading_data(S, ts.tri, ts.b1, ts.b2, side ? -nf : nf, sd);
+            f3 acc = mk(0.0f, 0.0f, 0.0f);
+            for (uint32_t k = 0; k < nspp; ++k) {
+                float ux = rng.next_1d(), uy = rng.next_1d();
+                f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+                float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+                if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+                    acc = acc + L.sun_illum * ndl;
+                if (nee) acc = acc + emitter_nee<STACK, BLOCK, COUNT>(S, sd, rng, col, pc);
+                uint64_t q = (uint64_t)slot * nspp + k;
+                put3(B.q_pos, q, sd.origin_front);
+                put3(B.q_nrm, q, sd.frame.n);
+                put3(B.q_alb, q, sd.diffuse);
+                B.q_w[q] = 1.0f;
             }
             B.partial[slot] = make_float4(acc.x, acc.y, acc.z, 0.0f);
         }
@@ -452,12 +512,22 @@ int launch(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_
     {
         StageTimer tm(t, s, kStageFirstBounce);
         uint32_t grid = grid_for(t->total_front, kTraceBlock);
-        if (counters)
-            hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s,
-                               S, Ly, iteration, nspp, flags, B, counters);
-        else
-            hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0,
-                               s, S, Ly, iteration, nspp, flags, B, counters);
+        bool at_x0 = (flags & NAN_LEARNED_TEXEL_QUERY) != 0;
+        if (counters) {
+            if (at_x0)
+                hipLaunchKernelGGL((learned_texel_direct<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0,
+                                   s, S, Ly, iteration, nspp, flags, B, counters);
+            else
+                hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0,
+                                   s, S, Ly, iteration, nspp, flags, B, counters);
+        } else {
+            if (at_x0)
+                hipLaunchKernelGGL((learned_texel_direct<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock),
+                                   0, s, S, Ly, iteration, nspp, flags, B, counters);
+            else
+                hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock),
+                                   0, s, S, Ly, iteration, nspp, flags, B, counters);
+        }
     }
     {
         StageTimer tm(t, s, kStageCacheQuery);
@@ -499,7 +569,7 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
     if (tracer->scene->stack_depth == 0)
         return fail(nullptr, 0, "nan_tst_execute_learned: BVH too deep for the per-texel kernels");
     if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_execute_learned: max_bounces must be at least 1");
-    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN))
+    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY))
         return fail(nullptr, 0, "nan_tst_execute_learned: unknown flags");
     if (tracer->total_front == 0) return 0;
     uint32_t nspp = samples_per_texel > 1u ? samples_per_texel : 1u;
