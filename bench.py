@@ -37,7 +37,7 @@ if str(ROOT) not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 # bumped whenever the tracer kernels change, so stale rocprof summaries are ignored
-KERNEL_VERSION = 2
+KERNEL_VERSION = 3
 METRIC = "texel-path samples/s (Mrays/s) at equal RMSE vs 4096-spp bake; learned vs plain"
 
 # algorithmic bytes (see the module docstring)

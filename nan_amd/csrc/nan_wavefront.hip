@@ -21,6 +21,14 @@
 // Traversal kernels are persistent: a fixed grid sized from the occupancy
 // calculator; a lane whose ray is done takes the next queued ray (refills of a
 // wave share one atomic).
+//
+// Queues are split into kParts partitions with one counter each, on its own
+// 64-byte line.  A device-scope atomic on one address is serialized at the
+// memory side (about 11 ns on MI355X whatever the caller); on 8 addresses they
+// proceed in parallel.  Producers append per block (one atomic per block and
+// queue) into partition blockIdx % kParts; consumers walk the partitions in
+// order, traversal waves starting at their own and moving on when it is empty.
+// Which lane traces which ray changes nothing in the results.
 // Their stack is a 16-entry ring per lane in LDS that spills its oldest
 // entries to a per-lane column in HBM when a traversal goes deeper.
 #include "nan_tracer.h"
@@ -36,6 +44,9 @@ constexpr int kMaxStack = 128;      // deepest traversal the spill area holds
 constexpr uint32_t kTargetSun = 0u;
 constexpr uint32_t kTargetRad = 1u;
 constexpr uint32_t kCounterSlots = 1024;
+constexpr uint32_t kParts = 8;       // queue partitions, one counter each
+constexpr uint32_t kCtrStride = 16;  // uint32 words between counters (one 64-byte line each)
+constexpr uint32_t kCounterWords = kCounterSlots * kParts * kCtrStride;
 
 }  // namespace
 
@@ -46,20 +57,24 @@ struct WfBuffers {
     float4 *rad = nullptr;      // [cap] path radiance.xyz, pdf of the first bounce
     float4 *sun = nullptr;      // [cap] surface sun irradiance of the current sample
     float4 *acc = nullptr;      // [cap] sum of samples over spp (spp > 1 only)
-    float4 *ray_o[2] = {nullptr, nullptr};  // [cap] origin.xyz, path id bits
-    float4 *ray_d[2] = {nullptr, nullptr};  // [cap] direction.xyz
-    float4 *hits = nullptr;     // [cap] t, b1, b2, triangle id bits
-    float4 *sh_o = nullptr;     // [2 cap] origin.xyz, path id bits
-    float4 *sh_d = nullptr;     // [2 cap] direction.xyz, target bits
-    float4 *sh_c = nullptr;     // [2 cap] contribution.xyz
-    uint32_t *counters = nullptr;   // [kCounterSlots]
+    // queues: kParts partitions of qcap (ray) or 2 qcap (shadow) entries
+    float4 *ray_o[2] = {nullptr, nullptr};  // origin.xyz, path id bits
+    float4 *ray_d[2] = {nullptr, nullptr};  // direction.xyz, scene bits
+    float4 *hits = nullptr;     // t, b1, b2, triangle id bits (indexed like the ray queue)
+    float4 *sh_o = nullptr;     // origin.xyz, path id bits
+    float4 *sh_d = nullptr;     // direction.xyz, target bits
+    float4 *sh_c = nullptr;     // contribution.xyz, scene bits
+    uint32_t qcap = 0;          // entries per partition of a ray queue
+    uint32_t *counters = nullptr;   // [kCounterWords]: slot s, partition j at (s * kParts + j) * kCtrStride
     uint32_t *frame = nullptr;      // [iteration, reset] in device-frame mode
     uint32_t *spill = nullptr;      // [(kMaxStack - ring) * spill_lanes]
     uint32_t spill_lanes = 0;
     uint32_t trav_blocks = 0;       // persistent grid of the traversal kernels
     uint32_t ring = 16;             // LDS ring entries per lane
     uint32_t blocks_per_cu_req = 0; // requested persistent blocks per CU (0: occupancy limit)
-    uint32_t stage_blocks = 0;      // grid of the stage kernels
+    uint32_t gen_blocks = 0;        // grids of the stage kernels: resident capacity (CUs x occupancy),
+    uint32_t shade_blocks = 0;      // so a grid-stride loop never waits for a second round of blocks
+    uint32_t finish_blocks = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -67,19 +82,73 @@ struct WfBuffers {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Appends `want` lanes of the wave to a queue with one atomic per wave;
-// returns the lane's slot (valid when want).
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
-    unsigned long long mask = __ballot(want);
-    uint32_t n = (uint32_t)__popcll(mask);
-    uint32_t base = 0;
-    if (n) {
-        int leader = __ffsll((long long)mask) - 1;
-        if ((int)lane_id() == leader) base = atomicAdd(count, n);
-        base = __shfl(base, leader);
+__device__ __forceinline__ uint32_t *queue_counter(uint32_t *counters, uint32_t slot, uint32_t part) {
+    return counters + ((size_t)slot * kParts + part) * kCtrStride;
+}
+
+// Entry counts of a queue's partitions, each clamped to the partition size.
+__device__ __forceinline__ void queue_counts(const uint32_t *counters, uint32_t slot, uint32_t part_cap,
+                                             uint32_t (&c)[kParts]) {
+#pragma unroll
+    for (uint32_t j = 0; j < kParts; ++j) c[j] = min(counters[((size_t)slot * kParts + j) * kCtrStride], part_cap);
+}
+
+__device__ __forceinline__ uint32_t pick(const uint32_t (&c)[kParts], uint32_t j) {
+    uint32_t v = c[0];
+#pragma unroll
+    for (uint32_t t = 1; t < kParts; ++t) v = j == t ? c[t] : v;
+    return v;
+}
+
+// Storage index of the k-th entry of a queue (partitions in order).
+__device__ __forceinline__ uint32_t queue_entry(const uint32_t (&c)[kParts], uint32_t k, uint32_t part_cap) {
+    uint32_t j = 0, off = k;
+#pragma unroll
+    for (uint32_t t = 0; t + 1 < kParts; ++t) {
+        bool past = j == t && off >= c[t];
+        off -= past ? c[t] : 0u;
+        j += past ? 1u : 0u;
     }
-    unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
-    return base + (uint32_t)__popcll(below);
+    return j * part_cap + off;
+}
+
+// Appends the wanting threads of the block to two queues (partition
+// blockIdx % kParts of each) with one atomic per queue; every thread of the
+// block must call it.  Returns storage indexes (valid where wanted); an
+// entry that would not fit its partition gets 0xFFFFFFFF and is dropped (the
+// partition sizes make that impossible for grids that are multiples of
+// kParts).
+__device__ __forceinline__ void block_append2(bool want_a, bool want_b, uint32_t *ctr_a, uint32_t *ctr_b,
+                                              uint32_t cap_a, uint32_t cap_b, uint32_t &pos_a, uint32_t &pos_b) {
+    constexpr uint32_t kWaves = kWfBlock / 64;
+    __shared__ uint32_t s_a[kWaves + 1], s_b[kWaves + 1];
+    unsigned long long ma = __ballot(want_a), mb = __ballot(want_b);
+    uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_a[wave] = (uint32_t)__popcll(ma);
+        s_b[wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0, tb = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            uint32_t ca = s_a[w], cb = s_b[w];
+            s_a[w] = ta;
+            s_b[w] = tb;
+            ta += ca;
+            tb += cb;
+        }
+        s_a[kWaves] = ta ? atomicAdd(ctr_a, ta) : 0u;
+        s_b[kWaves] = tb ? atomicAdd(ctr_b, tb) : 0u;
+    }
+    __syncthreads();
+    unsigned long long below = (1ull << lane_id()) - 1ull;
+    uint32_t part = blockIdx.x % kParts;
+    uint32_t ia = s_a[kWaves] + s_a[wave] + (uint32_t)__popcll(ma & below);
+    uint32_t ib = s_b[kWaves] + s_b[wave] + (uint32_t)__popcll(mb & below);
+    pos_a = ia < cap_a ? part * cap_a + ia : 0xFFFFFFFFu;
+    pos_b = ib < cap_b ? part * cap_b + ib : 0xFFFFFFFFu;
+    __syncthreads();  // s_a / s_b are reused by the next call
 }
 
 // Resumable traversal with a RING-entry LDS ring per lane that spills
@@ -207,9 +276,12 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, float
 // refilling lanes), so a wave keeps tracing while any rays remain instead
 // of waiting for its slowest lane.
 template <int RING, bool ANY_HIT, bool COUNT, typename Load, typename Store>
-__device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t count, uint32_t *fetch, uint32_t *lds_col,
+__device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counters, uint32_t count_slot,
+                                            uint32_t fetch_slot, uint32_t part_cap, uint32_t *lds_col,
                                             uint32_t *spill_col, uint32_t spill_stride, uint32_t steps, Load load,
                                             Store store, uint32_t &n_rays, uint32_t &n_nodes, uint32_t &n_tris) {
+    uint32_t cnt[kParts];
+    queue_counts(counters, count_slot, part_cap, cnt);
     Ray r;
     Hit h;
     float slack = 0.0f;
@@ -218,30 +290,35 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t count, u
     uint32_t k = 0;
     bool has = false;
     bool drained = false;
+    uint32_t part = blockIdx.x % kParts, tried = 0;  // wave-uniform
     while (true) {
-        if (!drained) {
-            bool need = !has;
-            unsigned long long mask = __ballot(need);
+        // refill idle lanes from the current partition, moving on while it is empty
+        while (!drained) {
+            unsigned long long mask = __ballot(!has);
             uint32_t n = (uint32_t)__popcll(mask);
-            if (n) {
-                uint32_t base = 0;
-                int leader = __ffsll((long long)mask) - 1;
-                if ((int)lane_id() == leader) base = atomicAdd(fetch, n);
-                base = __shfl(base, leader);
-                if (need) {
-                    uint32_t kk = base + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
-                    if (kk < count) {
-                        k = kk;
-                        uint32_t root;
-                        load(k, r, root, slack);
-                        ray_setup(r);
-                        h = Hit{r.tmax, 0.0f, 0.0f, kInvalid};
-                        ts = TravState{root, 0};
-                        st = TraversalStats{0, 0};
-                        has = true;
-                    }
+            if (!n) break;
+            uint32_t c = pick(cnt, part);
+            uint32_t base = 0;
+            int leader = __ffsll((long long)mask) - 1;
+            if ((int)lane_id() == leader) base = atomicAdd(queue_counter(counters, fetch_slot, part), n);
+            base = __shfl(base, leader);
+            uint32_t avail = base < c ? min(n, c - base) : 0u;
+            if (!has) {
+                uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+                if (rank < avail) {
+                    k = part * part_cap + base + rank;
+                    uint32_t root;
+                    load(k, r, root, slack);
+                    ray_setup(r);
+                    h = Hit{r.tmax, 0.0f, 0.0f, kInvalid};
+                    ts = TravState{root, 0};
+                    st = TraversalStats{0, 0};
+                    has = true;
                 }
-                if (base + n >= count) drained = true;
+            }
+            if (base + n >= c) {
+                part = part + 1 == kParts ? 0u : part + 1;
+                if (++tried == kParts) drained = true;
             }
         }
         if (__ballot(has) == 0ull) break;
@@ -269,7 +346,7 @@ __device__ __forceinline__ void add_counter(unsigned long long *counters, int sl
 // kernels
 // ---------------------------------------------------------------------------
 __global__ void wf_zero_counters(uint32_t *c, uint32_t n) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = 0u;
 }
 
 // End of a frame in device-frame mode: next iteration, no reset.
@@ -292,8 +369,8 @@ struct GenParams {
 
 __global__ __launch_bounds__(kWfBlock, 6) void wf_generate(DevScene S, DevLayout Ly, WfBuffers W, GenParams P) {
     uint32_t stride = gridDim.x * kWfBlock;
-    uint32_t *ext_count = W.counters + P.ext_slot;
-    uint32_t *sh_count = W.counters + P.sh_slot;
+    uint32_t *ext_count = queue_counter(W.counters, P.ext_slot, blockIdx.x % kParts);
+    uint32_t *sh_count = queue_counter(W.counters, P.sh_slot, blockIdx.x % kParts);
     for (uint32_t base = blockIdx.x * kWfBlock; base < P.count; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool active = i < P.count;
@@ -328,14 +405,14 @@ __global__ __launch_bounds__(kWfBlock, 6) void wf_generate(DevScene S, DevLayout
             W.sun[w] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             W.rng[w] = rng.state;
         }
-        uint32_t qs = wave_append(active && want_shadow, sh_count);
-        if (active && want_shadow) {
+        uint32_t qs, qr;
+        block_append2(active && want_shadow, active && want_ray, sh_count, ext_count, 2 * W.qcap, W.qcap, qs, qr);
+        if (active && want_shadow && qs != 0xFFFFFFFFu) {
             W.sh_o[qs] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
             W.sh_d[qs] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(kTargetSun));
             W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(scene));
         }
-        uint32_t qr = wave_append(active && want_ray, ext_count);
-        if (active && want_ray) {
+        if (active && want_ray && qr != 0xFFFFFFFFu) {
             W.ray_o[0][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
             W.ray_d[0][qr] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scene));
         }
@@ -348,12 +425,11 @@ __global__ __launch_bounds__(kTravBlock, 8) void wf_extend(DevScene S, WfBuffers
                                                           unsigned long long *counters) {
     __shared__ uint32_t stack[RING * kTravBlock];
     uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
-    const uint32_t count = W.counters[count_slot];
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
     const float4 *ro = W.ray_o[q], *rd = W.ray_d[q];
     float4 *hits = W.hits;
     trace_queue<RING, false, COUNT>(
-        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
+        S, W.counters, count_slot, fetch_slot, W.qcap, stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
         [&](uint32_t k, Ray &r, uint32_t &root, float &slack) {
             float4 o = ro[k], d = rd[k];
             r.o = mk(o.x, o.y, o.z);
@@ -379,10 +455,9 @@ __global__ __launch_bounds__(kTravBlock, 8) void wf_shadow(DevScene S, WfBuffers
                                                           unsigned long long *counters) {
     __shared__ uint32_t stack[RING * kTravBlock];
     uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
-    const uint32_t count = W.counters[count_slot];
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
     trace_queue<RING, true, COUNT>(
-        S, count, &W.counters[fetch_slot], stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
+        S, W.counters, count_slot, fetch_slot, 2 * W.qcap, stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
         [&](uint32_t k, Ray &r, uint32_t &root, float &slack) {
             float4 o = W.sh_o[k], d = W.sh_d[k];
             r.o = mk(o.x, o.y, o.z);
@@ -423,11 +498,18 @@ struct ShadeParams {
 
 // one iteration of trace_radiance's loop body (texture_space_path_tracer.slang:60-83)
 __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W, ShadeParams P) {
-    const uint32_t count = W.counters[P.in_slot];
+    uint32_t cnt[kParts];
+    queue_counts(W.counters, P.in_slot, W.qcap, cnt);
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kParts; ++j) count += cnt[j];
+    uint32_t *sh_count = queue_counter(W.counters, P.sh_slot, blockIdx.x % kParts);
+    uint32_t *out_count = queue_counter(W.counters, P.out_slot, blockIdx.x % kParts);
     uint32_t stride = gridDim.x * kWfBlock;
     for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += stride) {
-        uint32_t k = base + threadIdx.x;
-        bool active = k < count;
+        uint32_t kq = base + threadIdx.x;
+        bool active = kq < count;
+        uint32_t k = active ? queue_entry(cnt, kq, W.qcap) : 0u;
         bool want_shadow = false, want_ray = false;
         f3 origin = mk(0.0f, 0.0f, 0.0f), ld = origin, dir = origin, contrib = origin;
         uint32_t p = 0, scene = 0;
@@ -477,14 +559,14 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W,
             }
             W.rad[p] = make_float4(rad.x, rad.y, rad.z, Rv.w);
         }
-        uint32_t qs = wave_append(active && want_shadow, W.counters + P.sh_slot);
-        if (active && want_shadow) {
+        uint32_t qs, qr;
+        block_append2(active && want_shadow, active && want_ray, sh_count, out_count, 2 * W.qcap, W.qcap, qs, qr);
+        if (active && want_shadow && qs != 0xFFFFFFFFu) {
             W.sh_o[qs] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(p));
             W.sh_d[qs] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(kTargetRad));
             W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(scene));
         }
-        uint32_t qr = wave_append(active && want_ray, W.counters + P.out_slot);
-        if (active && want_ray) {
+        if (active && want_ray && qr != 0xFFFFFFFFu) {
             W.ray_o[P.q_out][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(p));
             W.ray_d[P.q_out][qr] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scene));
         }
@@ -591,15 +673,21 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     HIP_TRY(hipMalloc((void **)&W->rad, c4), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&W->sun, c4), nullptr, 0);
     if (spp > 1) HIP_TRY(hipMalloc((void **)&W->acc, c4), nullptr, 0);
+    // A stage grid that is a multiple of kParts hands partition j the
+    // 256-entry chunks c = j (mod kParts) of its input, so no partition
+    // receives more than ceil(n / 256 / kParts) * 256 <= n / kParts + 256
+    // entries per producer launch (shadow queues take two producers).
+    W->qcap = (uint32_t)(((cap + kParts - 1) / kParts + 256 + 255) / 256 * 256);
+    size_t q4 = (size_t)W->qcap * kParts * 16;
     for (int k = 0; k < 2; ++k) {
-        HIP_TRY(hipMalloc((void **)&W->ray_o[k], c4), nullptr, 0);
-        HIP_TRY(hipMalloc((void **)&W->ray_d[k], c4), nullptr, 0);
+        HIP_TRY(hipMalloc((void **)&W->ray_o[k], q4), nullptr, 0);
+        HIP_TRY(hipMalloc((void **)&W->ray_d[k], q4), nullptr, 0);
     }
-    HIP_TRY(hipMalloc((void **)&W->hits, c4), nullptr, 0);
-    HIP_TRY(hipMalloc((void **)&W->sh_o, 2 * c4), nullptr, 0);
-    HIP_TRY(hipMalloc((void **)&W->sh_d, 2 * c4), nullptr, 0);
-    HIP_TRY(hipMalloc((void **)&W->sh_c, 2 * c4), nullptr, 0);
-    HIP_TRY(hipMalloc((void **)&W->counters, kCounterSlots * 4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->hits, q4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->sh_o, 2 * q4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->sh_d, 2 * q4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->sh_c, 2 * q4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->counters, (size_t)kCounterWords * 4), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&W->frame, 2 * 4), nullptr, 0);
     int dev = 0, cus = 0, per_cu = 1;
     HIP_TRY(hipGetDevice(&dev), nullptr, 0);
@@ -609,7 +697,14 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     if (rc) return rc;
     if (tu.blocks_per_cu) per_cu = std::min(per_cu, (int)tu.blocks_per_cu);
     W->trav_blocks = (uint32_t)(cus * per_cu);
-    W->stage_blocks = (uint32_t)(cus * 8);
+    int og = 1, os = 1, of = 1;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&og, wf_generate, kWfBlock, 0), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os, wf_shade, kWfBlock, 0), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&of, wf_finish<false>, kWfBlock, 0), nullptr, 0);
+    auto parts_multiple = [](int blocks) { return (uint32_t)std::max<int>(kParts, blocks / kParts * kParts); };
+    W->gen_blocks = parts_multiple(cus * std::max(1, og));
+    W->shade_blocks = parts_multiple(cus * std::max(1, os));
+    W->finish_blocks = (uint32_t)(cus * std::max(1, of));
     W->spill_lanes = W->trav_blocks * kTravBlock;
     HIP_TRY(hipMalloc((void **)&W->spill, (size_t)(kMaxStack - W->ring) * W->spill_lanes * 4), nullptr, 0);
     return 0;
@@ -651,12 +746,15 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
     uint32_t nspp = spp > 1 ? spp : 1;
     uint32_t steps = std::max(1u, t->tuning.refill_steps);
     uint32_t slot = 0;  // next free counter slot
-    hipLaunchKernelGGL(wf_zero_counters, dim3(1), dim3(1024), 0, stream, W.counters, kCounterSlots);
+    hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream, W.counters, kCounterWords);
     for (uint32_t side = 0; side < 2; ++side) {
         uint32_t count = side ? t->ds_count : (uint32_t)t->total_front;
         const uint32_t *list = side ? t->ds_list : nullptr;
         if (count == 0) continue;
-        uint32_t gen_blocks = std::min<uint32_t>(W.stage_blocks, grid_for(count, kWfBlock));
+        uint32_t round8 = (grid_for(count, kWfBlock) + kParts - 1) / kParts * kParts;
+        uint32_t gen_blocks = std::min<uint32_t>(W.gen_blocks, round8);
+        uint32_t shade_blocks = std::min<uint32_t>(W.shade_blocks, round8);
+        uint32_t finish_blocks = std::min<uint32_t>(W.finish_blocks, grid_for(count, kWfBlock));
         for (uint32_t k = 0; k < nspp; ++k) {
             if (slot + 4 * bounces + 8 > kCounterSlots) return fail(nullptr, 0, "wavefront: too many stages");
             uint32_t ext_slot = slot++, sh_slot = slot++;
@@ -678,7 +776,7 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                 ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces};
                 {
                     StageTimer tm(t, stream, kStageShade);
-                    hipLaunchKernelGGL(wf_shade, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
+                    hipLaunchKernelGGL(wf_shade, dim3(shade_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
                 }
                 uint32_t sfetch = slot++;
                 {
@@ -695,10 +793,10 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                             t->scene->batch ? t->payload_table : nullptr};
             StageTimer tm(t, stream, kStageFinish);
             if (counters)
-                hipLaunchKernelGGL(wf_finish<true>, dim3(gen_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
+                hipLaunchKernelGGL(wf_finish<true>, dim3(finish_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
                                    counters);
             else
-                hipLaunchKernelGGL(wf_finish<false>, dim3(gen_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
+                hipLaunchKernelGGL(wf_finish<false>, dim3(finish_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
                                    counters);
         }
     }
