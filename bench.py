@@ -309,7 +309,13 @@ def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 512, fram
                 res[str(it + 1)] = {"rmse": r, "plain_frames_for_equal_rmse": m,
                                     "learned_ms_per_frame": elapsed / (it + 1),
                                     "speedup_vs_plain": m * plain_ms / elapsed}
-        forms[name] = {"train_samples": n_train, "minibatches": mb, "frames": res}
+        tracer.profile(True)
+        for it in range(2):
+            tracer.execute_learned(enc, learned, max(frames) + it, cache, train_samples=n_train,
+                                   texel_query=texel_query)
+        stages = {k: v[0] / 2 for k, v in tracer.stage_times().items()}
+        tracer.profile(False)
+        forms[name] = {"train_samples": n_train, "minibatches": mb, "frames": res, "stage_ms_per_frame": stages}
         log(f"learned at scale, {name}: " + ", ".join(f"{k}: x{v['speedup_vs_plain']:.2f}" for k, v in res.items()))
         del cache, learned
     return {"bake_frames": bake_frames, "plain_ms_per_frame": plain_ms,

@@ -327,6 +327,14 @@ enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2, NAN_LEARNED_TEX
 NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payload, uint32_t iteration,
                                     uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
                                     uint32_t train_samples, uint32_t flags, uint64_t *counters, void *stream);
+/* The training records of one iteration of nan_tst_execute_learned (n plain
+ * paths from uniformly drawn texels): features pos/nrm/alb float[n][3], target
+ * float[n][3], valid uint32[n], device memory; built by the wavefront queues
+ * in wavefront mode (when no emitter is light-sampled), else by the per-texel
+ * kernel, with identical results.  Test hook; no reference counterpart. */
+NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint32_t n, uint32_t max_bounces,
+                                    uint32_t flags, float *pos, float *nrm, float *alb, float *target,
+                                    uint32_t *valid, void *stream);
 
 #ifdef __cplusplus
 }

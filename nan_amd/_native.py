@@ -95,6 +95,7 @@ EXPORTED_SYMBOLS = (
     "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned", "nan_surface_filter",
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
+    "nan_tst_learned_records",
 )
 
 _LIB = None
@@ -118,6 +119,7 @@ def _declare(lib) -> None:
         "nan_atmosphere_sky_view_lut": (I, [vp, vp, f32p, vp, vp]),
         "nan_atmosphere_view_transmittance": (I, [vp, f32p, vp, vp]),
         "nan_sky_view_lookup": (I, [vp, f32p, vp, U64, vp, vp]),
+        "nan_tst_learned_records": (I, [vp, U32, U32, U32, U32, vp, vp, vp, vp, vp, vp]),
         "nan_trace_rays": (I, [vp, vp, U64, vp, vp, vp, vp]),
         "nan_tracer_create": (I, [vp, u32p, U32, u32p, u32p, U32, U64, U64, ctypes.POINTER(vp), ctypes.c_char_p,
                                   ctypes.c_size_t]),

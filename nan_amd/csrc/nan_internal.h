@@ -226,5 +226,28 @@ inline int check_launch(const char *what) {
 int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                       uint32_t reset, unsigned long long *counters, hipStream_t stream);
 
+// Device buffers of the learned forms (plain pointers, owned by nan_learned.hip).
+struct LearnedIO {
+    float *tr_pos, *tr_nrm, *tr_alb, *tr_target;  // training records [n][3]
+    uint32_t *tr_valid;
+    float *q_pos, *q_nrm, *q_alb, *q_w;  // cache queries [n][3], weight (0: none)
+    float4 *q_pred;
+};
+
+// Learned forms on the wavefront queues (nan_wavefront.hip), 1 sample per
+// texel, no emitter light sampling; same records, queries and payloads as the
+// per-texel kernels of nan_learned.hip bit for bit.
+//   train_paths: n training records (one plain path each) into io.tr_*.
+//   side: one side's texel samples: sun term and first bounce (or, at_x0, the
+//     texel itself) with their cache queries io.q_* at the side's path index;
+//     *count = paths of the side.
+//   finish: that side's estimates (plus max(0, io.q_pred)) into the payload.
+int wavefront_learned_train_paths(NanTracer *t, const LearnedIO &io, uint32_t iteration, uint32_t n,
+                                  uint32_t bounces, bool at_x0, hipStream_t stream);
+int wavefront_learned_side(NanTracer *t, const LearnedIO &io, uint32_t iteration, uint32_t side, bool at_x0,
+                           uint32_t *count, hipStream_t stream);
+int wavefront_learned_finish(NanTracer *t, const LearnedIO &io, float4 *payload, uint32_t side, uint32_t reset,
+                             bool at_x0, bool ramped, hipStream_t stream);
+
 
 }  // namespace nanx

@@ -38,22 +38,6 @@ struct LearnedBuffers {
 
 namespace {
 
-// 32-bit integer hash ("lowbias32", C. Wellons, hash-prospector, 2018).
-__device__ __forceinline__ uint32_t mix32(uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x7feb352du;
-    x ^= x >> 15;
-    x *= 0x846ca68bu;
-    x ^= x >> 16;
-    return x;
-}
-
-__device__ __forceinline__ void put3(float *a, uint64_t i, f3 v) {
-    a[3 * i + 0] = v.x;
-    a[3 * i + 1] = v.y;
-    a[3 * i + 2] = v.z;
-}
-
 // ---------------------------------------------------------------------------
 // light sampling of emissive triangles (NEE), combined with the cosine-sampled
 // bounce by the power heuristic (Veach, "Robust Monte Carlo Methods for Light
@@ -253,12 +237,11 @@ __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayo
     if (j < n) {
         bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && S.emitter_count > 0;
         bool at_x0 = (flags & NAN_LEARNED_TEXEL_QUERY) != 0;
-        uint32_t h = mix32(j ^ mix32(iteration * 0x9E3779B9u + 0x632BE5ABu));
-        uint64_t w = ((uint64_t)h * Ly.total_front) >> 32;
+        TrainStart st = train_start(Ly, j, iteration);
         Rng rng;
-        TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
-        bool back = (ts.side.y & 1u) && (mix32(h ^ 0x5bd1e995u) & 1u);
-        rng.state = mix32(h + 0x68E31DA4u);
+        TexelSurface ts = texel_surface(Ly, st.w, iteration, spp, rng);
+        bool back = (ts.side.y & 1u) && st.back;
+        rng.state = st.rng;
         f3 nf = world_flat_normal(S, ts.tri);
         Shading sd0, y1;
         shading_data(S, ts.tri, ts.b1, ts.b2, back ? -nf : nf, sd0);
@@ -407,19 +390,6 @@ __global__ __launch_bounds__(BLOCK) void learned_texel_direct(DevScene S, DevLay
     if (COUNT) flush_counters(pc, counters);
 }
 
-// Progressive mean with frame k (1-based since the reset) weighted by k:
-// m_k = m_{k-1} + 2 / (k + 1) * (e_k - m_{k-1}).  The cache improves over the
-// frames, so later estimates are the better ones.
-__device__ __forceinline__ void store_ramped(float4 *payload, uint32_t slot, f3 e, uint32_t reset) {
-    float4 p = payload[slot];
-    uint32_t prev_count = reset ? 0u : __float_as_uint(p.w);
-    f3 prev = prev_count == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(p.x, p.y, p.z);
-    uint32_t next = prev_count + 1 < kMaxTexelSamples ? prev_count + 1 : kMaxTexelSamples;
-    float w = 2.0f / (float)(prev_count + 2);
-    payload[slot] = make_float4(prev.x + w * (e.x - prev.x), prev.y + w * (e.y - prev.y), prev.z + w * (e.z - prev.z),
-                                __uint_as_float(next));
-}
-
 template <bool COUNT>
 __global__ void learned_finish(DevLayout Ly, LearnedDev B, float4 *payload, uint32_t spp, uint32_t reset,
                                uint32_t flags, unsigned long long *counters) {
@@ -482,6 +452,34 @@ int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query) {
     }
     if (!d.partial) {
         if (grow(&d.partial, std::max<uint64_t>(t->total_payload, 1) * 16)) return -1;
+    }
+    return 0;
+}
+
+LearnedIO learned_io(const LearnedDev &B) {
+    return LearnedIO{B.tr_pos, B.tr_nrm, B.tr_alb, B.tr_target, B.tr_valid, B.q_pos, B.q_nrm, B.q_alb, B.q_w, B.q_pred};
+}
+
+// The learned forms on the wavefront queues (same results as launch() below).
+int launch_wavefront(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t bounces, uint32_t reset,
+                     uint32_t n_train, uint32_t flags, hipStream_t s) {
+    LearnedIO io = learned_io(t->learned->d);
+    bool at_x0 = (flags & NAN_LEARNED_TEXEL_QUERY) != 0;
+    if (n_train) {
+        if (wavefront_learned_train_paths(t, io, iteration, n_train, bounces, at_x0, s)) return -1;
+        StageTimer tm(t, s, kStageCacheTrain);
+        if (nrc_train_step(c, io.tr_pos, io.tr_nrm, io.tr_alb, io.tr_target, io.tr_valid, n_train, s)) return -1;
+    }
+    for (uint32_t side = 0; side < 2; ++side) {
+        uint32_t count = 0;
+        if (wavefront_learned_side(t, io, iteration, side, at_x0, &count, s)) return -1;
+        if (count == 0) continue;
+        {
+            StageTimer tm(t, s, kStageCacheQuery);
+            if (nrc_predict(c, io.q_pos, io.q_nrm, io.q_alb, count, io.q_pred, s)) return -1;
+        }
+        if (wavefront_learned_finish(t, io, payload, side, reset, at_x0, (flags & NAN_LEARNED_RAMPED_MEAN) != 0, s))
+            return -1;
     }
     return 0;
 }
@@ -579,11 +577,58 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
         return -1;
     unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
     hipStream_t s = as_stream(stream);
+    // wavefront mode runs the forms on the tracer's queues when they need no
+    // per-texel extras: 1 sample per texel, no emitter light sampling, no
+    // counters, training paths within the tracer's path slots
+    bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && tracer->scene->dev.emitter_count > 0;
+    if (tracer->mode == NAN_TRACER_WAVEFRONT && nspp == 1 && !nee && !c && train_samples <= tracer->total_front)
+        return launch_wavefront(tracer, nrc, (float4 *)payload, iteration, max_bounces, reset, train_samples, flags, s);
     if (tracer->scene->stack_depth <= 32)
         return launch<32>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
                           s);
     return launch<64>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
                       s);
+}
+
+NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint32_t n, uint32_t max_bounces,
+                                    uint32_t flags, float *pos, float *nrm, float *alb, float *target,
+                                    uint32_t *valid, void *stream) {
+    if (!tracer || (n && (!pos || !nrm || !alb || !target || !valid)))
+        return fail(nullptr, 0, "nan_tst_learned_records: null argument");
+    if (tracer->scene->batch || tracer->scene_count != 1)
+        return fail(nullptr, 0, "nan_tst_learned_records: scene batches are not supported");
+    if (tracer->scene->stack_depth == 0)
+        return fail(nullptr, 0, "nan_tst_learned_records: BVH too deep for the per-texel kernels");
+    if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_learned_records: max_bounces must be at least 1");
+    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY))
+        return fail(nullptr, 0, "nan_tst_learned_records: unknown flags");
+    if (n == 0 || tracer->total_front == 0) return 0;
+    if (reserve(tracer, n, 0)) return -1;
+    hipStream_t s = as_stream(stream);
+    const LearnedDev &B = tracer->learned->d;
+    bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && tracer->scene->dev.emitter_count > 0;
+    if (tracer->mode == NAN_TRACER_WAVEFRONT && !nee && n <= tracer->total_front) {
+        if (wavefront_learned_train_paths(tracer, learned_io(B), iteration, n, max_bounces,
+                                          (flags & NAN_LEARNED_TEXEL_QUERY) != 0, s))
+            return -1;
+    } else {
+        const DevScene &S = tracer->scene->dev;
+        DevLayout Ly = dev_layout(tracer);
+        if (tracer->scene->stack_depth <= 32)
+            hipLaunchKernelGGL((learned_train_paths<32, kTraceBlock, false>), dim3(grid_for(n, kTraceBlock)),
+                               dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, max_bounces, n, flags, B, nullptr);
+        else
+            hipLaunchKernelGGL((learned_train_paths<64, kTraceBlock, false>), dim3(grid_for(n, kTraceBlock)),
+                               dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, max_bounces, n, flags, B, nullptr);
+        if (check_launch("learned training paths")) return -1;
+    }
+    size_t b3 = (size_t)n * 12;
+    HIP_TRY(hipMemcpyAsync(pos, B.tr_pos, b3, hipMemcpyDeviceToDevice, s), nullptr, 0);
+    HIP_TRY(hipMemcpyAsync(nrm, B.tr_nrm, b3, hipMemcpyDeviceToDevice, s), nullptr, 0);
+    HIP_TRY(hipMemcpyAsync(alb, B.tr_alb, b3, hipMemcpyDeviceToDevice, s), nullptr, 0);
+    HIP_TRY(hipMemcpyAsync(target, B.tr_target, b3, hipMemcpyDeviceToDevice, s), nullptr, 0);
+    HIP_TRY(hipMemcpyAsync(valid, B.tr_valid, (size_t)n * 4, hipMemcpyDeviceToDevice, s), nullptr, 0);
+    return 0;
 }
 
 }  // extern "C"

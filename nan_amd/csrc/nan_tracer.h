@@ -124,6 +124,51 @@ __device__ __forceinline__ void store_accumulated(float4 *payload, uint32_t slot
     payload[slot] = o;
 }
 
+// Progressive mean with frame k (1-based since the reset) weighted by k:
+// m_k = m_{k-1} + 2 / (k + 1) * (e_k - m_{k-1}) (learned forms: the cache
+// improves over the frames, so later estimates are the better ones).
+__device__ __forceinline__ void store_ramped(float4 *payload, uint32_t slot, f3 e, uint32_t reset) {
+    float4 p = payload[slot];
+    uint32_t prev_count = reset ? 0u : __float_as_uint(p.w);
+    f3 prev = prev_count == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(p.x, p.y, p.z);
+    uint32_t next = prev_count + 1 < kMaxTexelSamples ? prev_count + 1 : kMaxTexelSamples;
+    float w = 2.0f / (float)(prev_count + 2);
+    payload[slot] = make_float4(prev.x + w * (e.x - prev.x), prev.y + w * (e.y - prev.y), prev.z + w * (e.z - prev.z),
+                                __uint_as_float(next));
+}
+
+// 32-bit integer hash ("lowbias32", C. Wellons, hash-prospector, 2018).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ void put3(float *a, uint64_t i, f3 v) {
+    a[3 * i + 0] = v.x;
+    a[3 * i + 1] = v.y;
+    a[3 * i + 2] = v.z;
+}
+
+// Training record j of the learned forms: a uniformly drawn front texel and,
+// on a double-sided instance, a side; its RNG stream is its own.
+struct TrainStart {
+    uint64_t w;
+    bool back;
+    uint32_t rng;
+};
+__device__ __forceinline__ TrainStart train_start(const DevLayout &Ly, uint32_t j, uint32_t iteration) {
+    TrainStart ts;
+    uint32_t h = mix32(j ^ mix32(iteration * 0x9E3779B9u + 0x632BE5ABu));
+    ts.w = ((uint64_t)h * Ly.total_front) >> 32;
+    ts.back = (mix32(h ^ 0x5bd1e995u) & 1u) != 0;  // applies on double-sided instances only
+    ts.rng = mix32(h + 0x68E31DA4u);
+    return ts;
+}
+
 struct TexelSurface {
     uint32_t instance, prim, tri, global_texel, local, scene;
     float b1, b2;

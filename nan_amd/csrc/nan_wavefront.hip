@@ -65,6 +65,7 @@ struct WfBuffers {
     float4 *sh_d = nullptr;     // direction.xyz, target bits
     float4 *sh_c = nullptr;     // contribution.xyz, scene bits
     uint32_t qcap = 0;          // entries per partition of a ray queue
+    uint32_t *qidx = nullptr;   // [cap] learned forms: cache query index of each path
     uint32_t *counters = nullptr;   // [kCounterWords]: slot s, partition j at (s * kParts + j) * kCtrStride
     uint32_t *frame = nullptr;      // [iteration, reset] in device-frame mode
     uint32_t *spill = nullptr;      // [(kMaxStack - ring) * spill_lanes]
@@ -494,6 +495,11 @@ struct ShadeParams {
     uint32_t q_in, q_out;
     uint32_t in_slot, out_slot, sh_slot;
     uint32_t bounce, max_bounces;
+    // learned-form training paths (0: none): at the first hit y1, record 1 its
+    // position, normal and albedo as the record's features, record 2 its
+    // emission aside in W.sun; either way y1's emission stays out of rad
+    uint32_t train;
+    LearnedIO io;
 };
 
 // one iteration of trace_radiance's loop body (texture_space_path_tracer.slang:60-83)
@@ -524,6 +530,7 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W,
             float4 Tv = W.T[p], Rv = W.rad[p];
             f3 T = mk(Tv.x, Tv.y, Tv.z), rad = mk(Rv.x, Rv.y, Rv.z);
             uint32_t id = __float_as_uint(hv.w);
+            bool first_train = P.train != 0 && P.bounce == 1;
             if (id == kInvalid) {
                 rad = rad + T * miss_radiance(L, mk(rd.x, rd.y, rd.z));
             } else {
@@ -532,7 +539,16 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W,
                 f3 d = mk(rd.x, rd.y, rd.z);
                 Shading sd;
                 shading_data(S, id, hv.y, hv.z, -d, sd);
-                rad = rad + T * sd.emissive;
+                if (!first_train) {
+                    rad = rad + T * sd.emissive;
+                } else if (P.train == 1) {
+                    put3(P.io.tr_pos, p, sd.origin_front);
+                    put3(P.io.tr_nrm, p, sd.frame.n);
+                    put3(P.io.tr_alb, p, sd.diffuse);
+                    P.io.tr_valid[p] = 1u;
+                } else {
+                    W.sun[p] = make_float4(sd.emissive.x, sd.emissive.y, sd.emissive.z, 0.0f);
+                }
                 float ux = rng.next_1d(), uy = rng.next_1d();
                 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
                 float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
@@ -618,6 +634,230 @@ __global__ __launch_bounds__(kWfBlock) void wf_finish(DevLayout Ly, WfBuffers W,
 }
 
 // ---------------------------------------------------------------------------
+// learned forms (1 sample per texel, no emitter light sampling): the kernels of
+// nan_learned.hip restaged on the queues, with the same random draws and the
+// same floating-point operations in the same order
+// ---------------------------------------------------------------------------
+struct LearnedGenParams {
+    const uint32_t *list;  // null: path i is front texel i
+    uint32_t count, side, iteration;
+    uint32_t ext_slot, sh_slot;
+    uint32_t at_x0;        // 1: query the cache at the texel itself (no bounce ray)
+};
+
+// Texel samples of one side: the sun shadow ray, then the first bounce ray
+// (its query is filled at the hit) or, at_x0, the cache query at the texel.
+__global__ __launch_bounds__(kWfBlock, 6) void wf_learned_generate(DevScene S, DevLayout Ly, WfBuffers W,
+                                                                   LearnedIO io, LearnedGenParams P) {
+    uint32_t stride = gridDim.x * kWfBlock;
+    uint32_t *ext_count = queue_counter(W.counters, P.ext_slot, blockIdx.x % kParts);
+    uint32_t *sh_count = queue_counter(W.counters, P.sh_slot, blockIdx.x % kParts);
+    for (uint32_t base = blockIdx.x * kWfBlock; base < P.count; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < P.count;
+        bool want_shadow = false, want_ray = false;
+        f3 origin = mk(0.0f, 0.0f, 0.0f), ld = origin, dir = origin, contrib = origin;
+        uint32_t w = 0, scene = 0;
+        if (active) {
+            w = P.list ? P.list[i] : i;
+            Rng rng;
+            TexelSurface ts = texel_surface(Ly, w, P.iteration, 1u, rng);
+            scene = ts.scene;
+            const Light &L = S.scenes[scene].light;
+            if (P.side != 0) rng.state = W.rng[w];
+            f3 n = world_flat_normal(S, ts.tri);
+            Shading sd;
+            shading_data(S, ts.tri, ts.b1, ts.b2, P.side ? -n : n, sd);
+            float ux = rng.next_1d(), uy = rng.next_1d();
+            ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
+            float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
+            want_shadow = L.sun_on && ndl > 0.0f;
+            contrib = L.sun_illum * ndl;
+            origin = sd.origin_front;
+            W.sun[w] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (P.at_x0) {
+                put3(io.q_pos, i, sd.origin_front);
+                put3(io.q_nrm, i, sd.frame.n);
+                put3(io.q_alb, i, sd.diffuse);
+                io.q_w[i] = 1.0f;
+            } else {
+                float ex = rng.next_1d(), ey = rng.next_1d();
+                f3 sdir;
+                float pdf;
+                cosine_sample(ex, ey, sdir, pdf);
+                want_ray = pdf > 1e-6f;
+                dir = to_global(sd.frame, sdir);
+                W.T[w] = make_float4(1.0f, 1.0f, 1.0f, fabsf(sdir.z));
+                W.rad[w] = make_float4(0.0f, 0.0f, 0.0f, pdf);
+                W.qidx[w] = i;
+                put3(io.q_pos, i, mk(0.0f, 0.0f, 0.0f));
+                put3(io.q_nrm, i, mk(0.0f, 0.0f, 0.0f));
+                put3(io.q_alb, i, mk(0.0f, 0.0f, 0.0f));
+                io.q_w[i] = 0.0f;
+            }
+            W.rng[w] = rng.state;
+        }
+        uint32_t qs, qr;
+        block_append2(active && want_shadow, active && want_ray, sh_count, ext_count, 2 * W.qcap, W.qcap, qs, qr);
+        if (active && want_shadow && qs != 0xFFFFFFFFu) {
+            W.sh_o[qs] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
+            W.sh_d[qs] = make_float4(ld.x, ld.y, ld.z, __uint_as_float(kTargetSun));
+            W.sh_c[qs] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(scene));
+        }
+        if (active && want_ray && qr != 0xFFFFFFFFu) {
+            W.ray_o[0][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(w));
+            W.ray_d[0][qr] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scene));
+        }
+    }
+}
+
+// First hits of the learned bounce: the hit's emission (or the sky on a
+// miss) aside in rad.xyz, the hit's cache query at the path's query index.
+__global__ __launch_bounds__(kWfBlock, 5) void wf_learned_hit(DevScene S, WfBuffers W, LearnedIO io,
+                                                              uint32_t in_slot) {
+    uint32_t cnt[kParts];
+    queue_counts(W.counters, in_slot, W.qcap, cnt);
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kParts; ++j) count += cnt[j];
+    for (uint32_t kq = blockIdx.x * kWfBlock + threadIdx.x; kq < count; kq += gridDim.x * kWfBlock) {
+        uint32_t k = queue_entry(cnt, kq, W.qcap);
+        float4 ro = W.ray_o[0][k], rd = W.ray_d[0][k];
+        float4 hv = W.hits[k];
+        uint32_t p = __float_as_uint(ro.w);
+        const Light &L = S.scenes[__float_as_uint(rd.w)].light;
+        uint32_t i = W.qidx[p];
+        f3 d = mk(rd.x, rd.y, rd.z);
+        float4 Rv = W.rad[p];
+        uint32_t id = __float_as_uint(hv.w);
+        f3 term;
+        if (id == kInvalid) {
+            term = miss_radiance(L, d);
+        } else {
+            Shading y1;
+            shading_data(S, id, hv.y, hv.z, -d, y1);
+            term = y1.emissive;
+            put3(io.q_pos, i, y1.origin_front);
+            put3(io.q_nrm, i, y1.frame.n);
+            put3(io.q_alb, i, y1.diffuse);
+            io.q_w[i] = 1.0f;
+        }
+        W.rad[p] = make_float4(term.x, term.y, term.z, Rv.w);
+    }
+}
+
+struct LearnedFinishParams {
+    const uint32_t *list;
+    uint32_t count, side, reset, at_x0, ramped;
+};
+
+// E = sun + (az/pdf) * (Le(y1) or sky) + (az/pdf) * max(0, cache(y1)), or at
+// the texel E = sun + max(0, cache(x0)); into the side's payload slot.
+__global__ __launch_bounds__(kWfBlock) void wf_learned_finish(DevLayout Ly, WfBuffers W, LearnedIO io,
+                                                              float4 *payload, LearnedFinishParams P) {
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < P.count; i += gridDim.x * kWfBlock) {
+        uint32_t w = P.list ? P.list[i] : i;
+        float4 sv = W.sun[w];
+        f3 acc = mk(sv.x, sv.y, sv.z);
+        float4 c = io.q_pred[i];
+        f3 pred = mk(fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f));
+        if (P.at_x0) {
+            acc = acc + pred * 1.0f;
+        } else {
+            float4 Rv = W.rad[w];
+            float pdf = Rv.w;
+            if (pdf > 1e-6f) {
+                float weight = W.T[w].w / pdf;
+                acc = acc + mk(Rv.x, Rv.y, Rv.z) * weight;
+                if (io.q_w[i] > 0.0f) acc = acc + pred * weight;
+            }
+        }
+        uint32_t f = Ly.texel_face[w];
+        uint32_t inst = Ly.face_inst[f];
+        uint32_t local = w - Ly.front_prefix[inst];
+        uint32_t slot = P.side ? Ly.sides[inst].x + local : Ly.instances[inst].z + local;
+        if (P.ramped) store_ramped(payload, slot, acc, P.reset);
+        else store_accumulated(payload, slot, acc, P.reset);
+    }
+}
+
+// Training paths: record j starts like learned_train_paths in nan_learned.hip
+// (texel, side, own RNG stream, cosine bounce, no sun term at x0).
+__global__ __launch_bounds__(kWfBlock, 6) void wf_train_generate(DevScene S, DevLayout Ly, WfBuffers W,
+                                                                 LearnedIO io, uint32_t n, uint32_t iteration,
+                                                                 uint32_t at_x0, uint32_t ext_slot) {
+    uint32_t stride = gridDim.x * kWfBlock;
+    uint32_t *ext_count = queue_counter(W.counters, ext_slot, blockIdx.x % kParts);
+    uint32_t *unused = queue_counter(W.counters, ext_slot + 1, blockIdx.x % kParts);
+    for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += stride) {
+        uint32_t j = base + threadIdx.x;
+        bool active = j < n;
+        bool want_ray = false;
+        f3 origin = mk(0.0f, 0.0f, 0.0f), dir = origin;
+        uint32_t scene = 0;
+        if (active) {
+            TrainStart st = train_start(Ly, j, iteration);
+            Rng rng;
+            TexelSurface ts = texel_surface(Ly, st.w, iteration, 1u, rng);
+            bool back = (ts.side.y & 1u) && st.back;
+            rng.state = st.rng;
+            scene = ts.scene;
+            f3 nf = world_flat_normal(S, ts.tri);
+            Shading sd0;
+            shading_data(S, ts.tri, ts.b1, ts.b2, back ? -nf : nf, sd0);
+            float ex = rng.next_1d(), ey = rng.next_1d();
+            f3 sdir;
+            float pdf;
+            cosine_sample(ex, ey, sdir, pdf);
+            want_ray = pdf > 1e-6f;
+            origin = sd0.origin_front;
+            dir = to_global(sd0.frame, sdir);
+            W.T[j] = make_float4(1.0f, 1.0f, 1.0f, fabsf(sdir.z));
+            W.rad[j] = make_float4(0.0f, 0.0f, 0.0f, pdf);
+            W.sun[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            W.rng[j] = rng.state;
+            if (at_x0) {
+                put3(io.tr_pos, j, sd0.origin_front);
+                put3(io.tr_nrm, j, sd0.frame.n);
+                put3(io.tr_alb, j, sd0.diffuse);
+                io.tr_valid[j] = 1u;
+            } else {
+                put3(io.tr_pos, j, mk(0.0f, 0.0f, 0.0f));
+                put3(io.tr_nrm, j, mk(0.0f, 0.0f, 0.0f));
+                put3(io.tr_alb, j, mk(0.0f, 0.0f, 0.0f));
+                io.tr_valid[j] = 0u;
+            }
+        }
+        uint32_t qs, qr;
+        block_append2(false, active && want_ray, unused, ext_count, 2 * W.qcap, W.qcap, qs, qr);
+        if (active && want_ray && qr != 0xFFFFFFFFu) {
+            W.ray_o[0][qr] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(j));
+            W.ray_d[0][qr] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scene));
+        }
+    }
+}
+
+// Training targets: the suffix radiance of y1, or at the texel
+// (suffix + Le(y1)) * az/pdf (the sky * az/pdf on a miss).
+__global__ __launch_bounds__(kWfBlock) void wf_train_finish(WfBuffers W, LearnedIO io, uint32_t n, uint32_t at_x0) {
+    for (uint32_t j = blockIdx.x * kWfBlock + threadIdx.x; j < n; j += gridDim.x * kWfBlock) {
+        float4 Rv = W.rad[j];
+        f3 rad = mk(Rv.x, Rv.y, Rv.z);
+        f3 target = mk(0.0f, 0.0f, 0.0f);
+        if (at_x0) {
+            float pdf = Rv.w;
+            if (pdf > 1e-6f) {
+                float4 le = W.sun[j];
+                target = (rad + mk(le.x, le.y, le.z) * 1.0f) * (W.T[j].w / pdf);
+            }
+        } else if (io.tr_valid[j]) {
+            target = rad;
+        }
+        put3(io.tr_target, j, target);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 namespace nanx {
@@ -641,6 +881,7 @@ void wavefront_release(NanTracer *t) {
     hipFree(W->counters);
     hipFree(W->frame);
     hipFree(W->spill);
+    hipFree(W->qidx);
     delete W;
     t->wf = nullptr;
 }
@@ -809,6 +1050,107 @@ int wavefront_set_frame(NanTracer *t, uint32_t iteration, uint32_t reset) {
     uint32_t v[2] = {iteration, reset ? 1u : 0u};
     HIP_TRY(hipMemcpy(t->wf->frame, v, sizeof(v), hipMemcpyHostToDevice), nullptr, 0);
     return 0;
+}
+
+
+static int learned_checks(NanTracer *t) {
+    if (t->scene->info.max_stack > (uint32_t)kMaxStack)
+        return fail(nullptr, 0, "wavefront: BVH needs a %u-entry stack (max %d)", t->scene->info.max_stack,
+                    kMaxStack);
+    if (wavefront_alloc(t, 1)) return -1;
+    WfBuffers &W = *t->wf;
+    if (!W.qidx) HIP_TRY(hipMalloc((void **)&W.qidx, (size_t)W.capacity * 4), nullptr, 0);
+    return 0;
+}
+
+static void launch_extend_any(const WfBuffers &W, const DevScene &S, uint32_t q, uint32_t count_slot, uint32_t fetch,
+                              uint32_t steps, hipStream_t stream) {
+    if (W.ring == 16) launch_extend<16>(W, S, q, count_slot, fetch, steps, nullptr, stream);
+    else if (W.ring == 24) launch_extend<24>(W, S, q, count_slot, fetch, steps, nullptr, stream);
+    else launch_extend<32>(W, S, q, count_slot, fetch, steps, nullptr, stream);
+}
+
+static void launch_shadow_any(const WfBuffers &W, const DevScene &S, uint32_t count_slot, uint32_t fetch,
+                              uint32_t steps, hipStream_t stream) {
+    if (W.ring == 16) launch_shadow<16>(W, S, count_slot, fetch, steps, nullptr, stream);
+    else if (W.ring == 24) launch_shadow<24>(W, S, count_slot, fetch, steps, nullptr, stream);
+    else launch_shadow<32>(W, S, count_slot, fetch, steps, nullptr, stream);
+}
+
+int wavefront_learned_train_paths(NanTracer *t, const LearnedIO &io, uint32_t iteration, uint32_t n,
+                                  uint32_t bounces, bool at_x0, hipStream_t stream) {
+    if (learned_checks(t)) return -1;
+    WfBuffers &W = *t->wf;
+    if (n == 0) return 0;
+    if (n > W.capacity) return fail(nullptr, 0, "wavefront: %u training paths exceed the %u path slots", n, W.capacity);
+    if (4 * bounces + 8 > kCounterSlots) return fail(nullptr, 0, "wavefront: too many stages");
+    const DevScene &S = t->scene->dev;
+    DevLayout Ly = dev_layout(t);
+    uint32_t steps = std::max(1u, t->tuning.refill_steps);
+    uint32_t round8 = (grid_for(n, kWfBlock) + kParts - 1) / kParts * kParts;
+    uint32_t gen_blocks = std::min<uint32_t>(W.gen_blocks, round8);
+    uint32_t shade_blocks = std::min<uint32_t>(W.shade_blocks, round8);
+    StageTimer tm(t, stream, kStageLearnedPaths);
+    hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream, W.counters, kCounterWords);
+    uint32_t slot = 0;
+    uint32_t ext_slot = slot;
+    slot += 2;  // the generator's ray queue and an unused second queue
+    hipLaunchKernelGGL(wf_train_generate, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, Ly, W, io, n, iteration,
+                       at_x0 ? 1u : 0u, ext_slot);
+    uint32_t sh_slot = slot++;
+    uint32_t q = 0;
+    for (uint32_t b = 1; b <= bounces; ++b) {
+        uint32_t fetch = slot++;
+        launch_extend_any(W, S, q, ext_slot, fetch, steps, stream);
+        uint32_t next_ext = slot++;
+        ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces, at_x0 ? 2u : 1u, io};
+        hipLaunchKernelGGL(wf_shade, dim3(shade_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
+        uint32_t sfetch = slot++;
+        launch_shadow_any(W, S, sh_slot, sfetch, steps, stream);
+        sh_slot = slot++;
+        ext_slot = next_ext;
+        q = 1u - q;
+    }
+    hipLaunchKernelGGL(wf_train_finish, dim3(std::min<uint32_t>(W.finish_blocks, grid_for(n, kWfBlock))),
+                       dim3(kWfBlock), 0, stream, W, io, n, at_x0 ? 1u : 0u);
+    return check_launch("wavefront training paths");
+}
+
+int wavefront_learned_side(NanTracer *t, const LearnedIO &io, uint32_t iteration, uint32_t side, bool at_x0,
+                           uint32_t *count_out, hipStream_t stream) {
+    if (learned_checks(t)) return -1;
+    WfBuffers &W = *t->wf;
+    uint32_t count = side ? t->ds_count : (uint32_t)t->total_front;
+    *count_out = count;
+    if (count == 0) return 0;
+    const DevScene &S = t->scene->dev;
+    DevLayout Ly = dev_layout(t);
+    uint32_t steps = std::max(1u, t->tuning.refill_steps);
+    uint32_t round8 = (grid_for(count, kWfBlock) + kParts - 1) / kParts * kParts;
+    StageTimer tm(t, stream, kStageFirstBounce);
+    hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream, W.counters, kCounterWords);
+    LearnedGenParams gp{side ? t->ds_list : nullptr, count, side, iteration, 0u, 1u, at_x0 ? 1u : 0u};
+    hipLaunchKernelGGL(wf_learned_generate, dim3(std::min<uint32_t>(W.gen_blocks, round8)), dim3(kWfBlock), 0, stream,
+                       S, Ly, W, io, gp);
+    if (!at_x0) {
+        launch_extend_any(W, S, 0u, 0u, 2u, steps, stream);
+        hipLaunchKernelGGL(wf_learned_hit, dim3(std::min<uint32_t>(W.shade_blocks, round8)), dim3(kWfBlock), 0,
+                           stream, S, W, io, 0u);
+    }
+    launch_shadow_any(W, S, 1u, 3u, steps, stream);
+    return check_launch("wavefront learned samples");
+}
+
+int wavefront_learned_finish(NanTracer *t, const LearnedIO &io, float4 *payload, uint32_t side, uint32_t reset,
+                             bool at_x0, bool ramped, hipStream_t stream) {
+    WfBuffers &W = *t->wf;
+    uint32_t count = side ? t->ds_count : (uint32_t)t->total_front;
+    if (count == 0) return 0;
+    StageTimer tm(t, stream, kStageFinish);
+    LearnedFinishParams fp{side ? t->ds_list : nullptr, count, side, reset, at_x0 ? 1u : 0u, ramped ? 1u : 0u};
+    hipLaunchKernelGGL(wf_learned_finish, dim3(std::min<uint32_t>(W.finish_blocks, grid_for(count, kWfBlock))),
+                       dim3(kWfBlock), 0, stream, dev_layout(t), W, io, payload, fp);
+    return check_launch("wavefront learned finish");
 }
 
 }  // namespace nanx

@@ -101,6 +101,23 @@ class TextureSpacePathTracer:
                                                        stream)
         _native.check(rc, "nan_tst_execute")
 
+    def learned_training_records(self, iteration: int, n: int, *, emitter_nee: bool = True,
+                                 texel_query: bool = False) -> dict:
+        """The ``n`` training records ``execute_learned`` would build at
+        ``iteration`` (features ``pos``/``nrm``/``alb`` and ``target`` float32
+        [n, 3], ``valid`` int32[n]).  Test hook; no counterpart in the
+        reference."""
+        dev = self.device.torch_device
+        out = {k: torch.empty((n, 3), dtype=torch.float32, device=dev) for k in ("pos", "nrm", "alb", "target")}
+        out["valid"] = torch.empty((n,), dtype=torch.int32, device=dev)
+        flags = (_native.LEARNED_EMITTER_NEE if emitter_nee else 0) | (_native.LEARNED_TEXEL_QUERY if texel_query else 0)
+        rc = _native.lib().nan_tst_learned_records(
+            self.handle, max(0, int(iteration)), int(n), self.max_bounces, flags, out["pos"].data_ptr(),
+            out["nrm"].data_ptr(), out["alb"].data_ptr(), out["target"].data_ptr(), out["valid"].data_ptr(),
+            self.device.stream_handle)
+        _native.check(rc, "nan_tst_learned_records")
+        return out
+
     def execute_learned(self, command_encoder: CommandEncoder, output: torch.Tensor, iteration: int, cache, *,
                         reset: bool = False, train_samples: int = 65536, emitter_nee: bool = True,
                         ramped_mean: bool = True, texel_query: bool = False,

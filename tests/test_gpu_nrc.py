@@ -323,3 +323,40 @@ def test_texel_query_form_converges_faster_than_plain(device):
     _say(f"RMSE texel-query@32 {rmse_l:.4f}, plain@4 {rmse_p4:.4f}; means {lq.mean():.4f} {b.mean():.4f}")
     assert abs(lq.mean() - b.mean()) < 0.03 * b.mean()
     assert rmse_l < rmse_p4
+
+
+@pytest.mark.parametrize("texel_query", [False, True])
+def test_learned_forms_on_the_wavefront_match_the_per_texel_kernels(device, texel_query):
+    """In wavefront mode the learned forms run on the tracer's queues; their
+    training records and (with a fixed, untrained cache) their payloads are
+    bit-identical to the per-texel kernels of megakernel mode."""
+    from nan_amd.radiance_cache import RadianceCache
+    from nan_amd.scene import Scene
+    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
+    _say(f"learned wavefront vs per-texel (texel_query={texel_query})")
+    node, layout = _small_sunlit()
+    sc = Scene(device, node)
+    tracers = {m: TextureSpacePathTracer(device, sc, layout, samples_per_texel=1, max_bounces=3, mode=m)
+               for m in ("megakernel", "wavefront")}
+    n = min(60_000, layout.total_surface_texels)
+    recs = {m: t.learned_training_records(5, n, texel_query=texel_query) for m, t in tracers.items()}
+    for k in ("pos", "nrm", "alb", "target", "valid"):
+        a, b = recs["megakernel"][k].cpu().numpy(), recs["wavefront"][k].cpu().numpy()
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
+    valid = recs["wavefront"]["valid"].cpu().numpy()
+    assert valid.mean() > (0.99 if texel_query else 0.3)
+    cache = RadianceCache.for_scene(device, sc)
+    w, t = cache.get_params()
+    rng = np.random.default_rng(11)
+    cache.set_params((rng.standard_normal(w.shape) * 0.1).astype(w.dtype),
+                     (rng.standard_normal(t.shape) * 0.1).astype(t.dtype))
+    outs = {}
+    for m, tr in tracers.items():
+        out = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=device.torch_device)
+        enc = device.create_command_encoder()
+        for it in range(2):
+            tr.execute_learned(enc, out, it, cache, reset=(it == 0), train_samples=0, texel_query=texel_query)
+        outs[m] = out.cpu().numpy()
+    np.testing.assert_array_equal(outs["megakernel"].view(np.uint32), outs["wavefront"].view(np.uint32))
+    assert outs["wavefront"][:, :3].max() > 0.0
