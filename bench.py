@@ -116,6 +116,63 @@ def cpu_baseline(node, layout, threads: int, target_seconds: float = 12.0) -> di
     }
 
 
+def tinybvh_baseline(node, threads: int, rays: int = 4_000_000) -> dict | None:
+    """The reference's own CPU ray path, timed: the tinybvh it vendors
+    (vertex_baker/cpp/third_party/tinybvh, v1.8.0), binned-SAH build and
+    IsOccluded per ray on one worker per thread (tinybvh_visibility.cpp:
+    145-230), through oracle/_ref/libtinybvh_ref.so, on sun shadow rays from
+    uniformly drawn points of the scene's triangles.  None when the harness is
+    not built."""
+    import ctypes
+
+    import numpy as np
+
+    import oracle
+    from nan_amd.scene_arrays import flatten_scene
+
+    lib_path = ROOT / "oracle" / "_ref" / "libtinybvh_ref.so"
+    if not lib_path.exists():
+        return None
+    lib = ctypes.CDLL(str(lib_path))
+    lib.ref_bvh_create.restype = ctypes.c_void_p
+    lib.ref_bvh_create.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.ref_bvh_build_ms.restype = ctypes.c_double
+    lib.ref_bvh_build_ms.argtypes = [ctypes.c_void_p]
+    lib.ref_bvh_destroy.argtypes = [ctypes.c_void_p]
+    lib.ref_bvh_occluded.restype = ctypes.c_double
+    lib.ref_bvh_occluded.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int]
+    arrays = flatten_scene(node)
+    world = np.ascontiguousarray(oracle.OracleScene(arrays, use_bvh=False).world_triangles(), dtype=np.float32)
+    tris = world.reshape(-1, 3, 3).astype(np.float64)
+    rng = np.random.default_rng(0)
+    pick = rng.integers(0, len(tris), size=rays)
+    a, b = rng.random(rays), rng.random(rays)
+    flip = a + b > 1.0
+    a[flip], b[flip] = 1.0 - a[flip], 1.0 - b[flip]
+    t = tris[pick]
+    p = t[:, 0] + a[:, None] * (t[:, 1] - t[:, 0]) + b[:, None] * (t[:, 2] - t[:, 0])
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    sun = np.asarray(arrays["lighting"][0:3], dtype=np.float64)
+    n *= np.where((n @ sun) < 0.0, -1.0, 1.0)[:, None]
+    r = np.zeros((rays, 8), dtype=np.float32)
+    r[:, 0:3] = p + 1e-3 * n
+    r[:, 4:7] = sun
+    r[:, 7] = 1000.0
+    occ = np.zeros(rays, dtype=np.uint32)
+    h = lib.ref_bvh_create(world.ctypes.data, len(tris))
+    try:
+        build_ms = lib.ref_bvh_build_ms(h)
+        trace_ms = lib.ref_bvh_occluded(h, r.ctypes.data, rays, occ.ctypes.data, threads)
+    finally:
+        lib.ref_bvh_destroy(h)
+    return {"value": rays / (trace_ms * 1e-3) / 1e6, "unit": "Mrays/s (occlusion)", "cores": threads,
+            "kind": "reference", "build_ms": build_ms, "trace_ms": trace_ms,
+            "sample": (f"{rays} sun shadow rays from uniformly drawn triangle points (offset 1e-3 along the "
+                       f"normal), {float(occ.mean()):.3f} occluded; tinybvh v1.8.0 binned-SAH BVH over "
+                       f"{len(tris)} triangles")}
+
+
 MFMA_F16_PEAK_TFLOPS = 2500.0     # dense FP16 MFMA peak of MI355X
 MLP_FLOPS_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 16)   # executed (padded) forward FLOPs: 32,768
 
@@ -772,6 +829,10 @@ def main() -> int:
         threads = min(threads, 16)
         log(f"cpu baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(node, layout, threads, args.cpu_seconds)
+        ref = tinybvh_baseline(node, threads)
+        if ref is not None:
+            ref["gpu_shadow_mrays_per_s"] = shd / (stages["shadow"][0] / args.steps * 1e-3) / 1e6
+            out["cpu_baseline_reference_visibility"] = ref
     print(json.dumps(out), flush=True)
     return 0
 
