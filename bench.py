@@ -37,7 +37,7 @@ if str(ROOT) not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 # bumped whenever the tracer kernels change, so stale rocprof summaries are ignored
-KERNEL_VERSION = 3
+KERNEL_VERSION = 4
 METRIC = "texel-path samples/s (Mrays/s) at equal RMSE vs 4096-spp bake; learned vs plain"
 
 # algorithmic bytes (see the module docstring)
@@ -441,7 +441,7 @@ def bench_combined(args) -> int:
         f"built in {time.perf_counter() - t0:.1f} s")
     enc = device.create_command_encoder()
     stream = torch.cuda.current_stream(dev)
-    tune = autotune(batch, enc, space={"blocks_per_cu": (0, 4), "lds_stack": (16, 32), "refill_steps": (16, 32)},
+    tune = autotune(batch, enc, space={"blocks_per_cu": (0, 4), "lds_stack": (16, 32), "refill_steps": (8, 16)},
                     frames=1)
     log(f"autotuned {tune['best']}")
     batch.execute(enc, 0, reset=True)

@@ -208,6 +208,8 @@ void split_oversized_leaves(std::vector<BNode> &nodes, const std::vector<uint32_
     }
 }
 
+// (float)q * scale is exact (q < 256, scale a power of two), so this equals the
+// kernels' single-rounding fmaf(q, scale, origin) bit for bit.
 inline float decode(float origin, uint32_t q, float scale) { return origin + (float)q * scale; }
 
 inline float exp2i(int e) {

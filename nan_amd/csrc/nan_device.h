@@ -302,12 +302,12 @@ __device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &h
             int nhit = 0;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                float lox = origin.x + (float)((qlx >> (8 * c)) & 0xFFu) * sx;
-                float loy = origin.y + (float)((qly >> (8 * c)) & 0xFFu) * sy;
-                float loz = origin.z + (float)((qlz >> (8 * c)) & 0xFFu) * sz;
-                float hix = origin.x + (float)((qhx >> (8 * c)) & 0xFFu) * sx;
-                float hiy = origin.y + (float)((qhy >> (8 * c)) & 0xFFu) * sy;
-                float hiz = origin.z + (float)((qhz >> (8 * c)) & 0xFFu) * sz;
+                float lox = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), sx, origin.x);
+                float loy = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), sy, origin.y);
+                float loz = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), sz, origin.z);
+                float hix = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), sx, origin.x);
+                float hiy = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), sy, origin.y);
+                float hiz = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), sz, origin.z);
                 float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
                 float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
                 float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;

@@ -189,60 +189,70 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, float
         float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
         uint32_t valid = w0.w >> 24;
         uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
-        uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
         float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + slack;
+        // entry distance of each child (+inf when culled)
         float tn[4];
-        uint32_t ent[4];
-        int nhit = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            float lox = origin.x + (float)((qlx >> (8 * c)) & 0xFFu) * sx;
-            float loy = origin.y + (float)((qly >> (8 * c)) & 0xFFu) * sy;
-            float loz = origin.z + (float)((qlz >> (8 * c)) & 0xFFu) * sz;
-            float hix = origin.x + (float)((qhx >> (8 * c)) & 0xFFu) * sx;
-            float hiy = origin.y + (float)((qhy >> (8 * c)) & 0xFFu) * sy;
-            float hiz = origin.z + (float)((qhz >> (8 * c)) & 0xFFu) * sz;
+            float lox = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), sx, origin.x);
+            float loy = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), sy, origin.y);
+            float loz = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), sz, origin.z);
+            float hix = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), sx, origin.x);
+            float hiy = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), sy, origin.y);
+            float hiz = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), sz, origin.z);
             float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
             float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
             float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
             float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
             float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + slack;
             far = fminf(far, limit);
-            bool ok = ((valid >> c) & 1u) && near <= far;
-            if (ok) {
-                float key = near;
-                uint32_t e = child[c];
-                if (!ANY_HIT) {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (k < nhit && key < tn[k]) {
-                            float tk = tn[k];
-                            uint32_t ek = ent[k];
-                            tn[k] = key;
-                            ent[k] = e;
-                            key = tk;
-                            e = ek;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (k == nhit) {
-                        tn[k] = key;
-                        ent[k] = e;
-                    }
-                nhit++;
-            }
+            tn[c] = (((valid >> c) & 1u) && near <= far) ? near : INFINITY;
+        }
+        uint32_t e0 = w2.z, e1 = w2.w, e2 = w3.x, e3 = w3.y;
+        float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
+        int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
+        if (!ANY_HIT) {
+            // nearest first: a 4-input sorting network (culled children sort last)
+            auto cswap = [](float &ka, uint32_t &ea, float &kb, uint32_t &eb) {
+                bool sw = kb < ka;
+                float k = sw ? kb : ka;
+                uint32_t e = sw ? eb : ea;
+                kb = sw ? ka : kb;
+                eb = sw ? ea : eb;
+                ka = k;
+                ea = e;
+            };
+            cswap(k0, e0, k1, e1);
+            cswap(k2, e2, k3, e3);
+            cswap(k0, e0, k2, e2);
+            cswap(k1, e1, k3, e3);
+            cswap(k1, e1, k2, e2);
+        } else {
+            // any order: compact the surviving children to the front
+            auto take = [](float &ka, uint32_t &ea, float &kb, uint32_t &eb) {
+                bool sw = !(ka < INFINITY) && kb < INFINITY;
+                float k = sw ? kb : ka;
+                uint32_t e = sw ? eb : ea;
+                kb = sw ? ka : kb;
+                eb = sw ? ea : eb;
+                ka = k;
+                ea = e;
+            };
+            take(k0, e0, k1, e1);
+            take(k2, e2, k3, e3);
+            take(k0, e0, k2, e2);
+            take(k1, e1, k3, e3);
+            take(k1, e1, k2, e2);
         }
         if (nhit == 0) {
             if (ts.sp == 0) return true;
             ts.cur = pop();
             return false;
         }
-#pragma unroll
-        for (int k = 3; k >= 1; --k)
-            if (k < nhit) push(ent[k]);
-        ts.cur = ent[0];
+        if (nhit > 3) push(e3);
+        if (nhit > 2) push(e2);
+        if (nhit > 1) push(e1);
+        ts.cur = e0;
         return false;
     }
     uint32_t first = ts.cur & kLeafFirstMask;
@@ -281,8 +291,11 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counter
                                             uint32_t fetch_slot, uint32_t part_cap, uint32_t *lds_col,
                                             uint32_t *spill_col, uint32_t spill_stride, uint32_t steps, Load load,
                                             Store store, uint32_t &n_rays, uint32_t &n_nodes, uint32_t &n_tris) {
-    uint32_t cnt[kParts];
-    queue_counts(counters, count_slot, part_cap, cnt);
+    // partition sizes in LDS (registers are the traversal's)
+    __shared__ uint32_t s_cnt[kParts];
+    if (threadIdx.x < kParts)
+        s_cnt[threadIdx.x] = min(counters[((size_t)count_slot * kParts + threadIdx.x) * kCtrStride], part_cap);
+    __syncthreads();
     Ray r;
     Hit h;
     float slack = 0.0f;
@@ -298,7 +311,7 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counter
             unsigned long long mask = __ballot(!has);
             uint32_t n = (uint32_t)__popcll(mask);
             if (!n) break;
-            uint32_t c = pick(cnt, part);
+            uint32_t c = s_cnt[part];
             uint32_t base = 0;
             int leader = __ffsll((long long)mask) - 1;
             if ((int)lane_id() == leader) base = atomicAdd(queue_counter(counters, fetch_slot, part), n);
