@@ -441,7 +441,7 @@ def bench_combined(args) -> int:
         f"built in {time.perf_counter() - t0:.1f} s")
     enc = device.create_command_encoder()
     stream = torch.cuda.current_stream(dev)
-    tune = autotune(batch, enc, space={"blocks_per_cu": (0, 4), "lds_stack": (16, 32), "refill_steps": (8, 16)},
+    tune = autotune(batch, enc, space={"blocks_per_cu": (0, 4), "lds_stack": (16, 32), "refill_steps": (16, 32)},
                     frames=1)
     log(f"autotuned {tune['best']}")
     batch.execute(enc, 0, reset=True)

@@ -18,7 +18,7 @@ import torch
 DEFAULT_SPACE = {
     "blocks_per_cu": (0, 4),
     "lds_stack": (16, 24, 32),
-    "refill_steps": (4, 8, 16, 32),
+    "refill_steps": (8, 16, 32),
 }
 
 

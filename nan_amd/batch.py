@@ -82,7 +82,7 @@ class SceneBatch:
     def triangle_counts(self) -> list[int]:
         return [n.triangle_count for n in self.nodes]
 
-    def tune(self, blocks_per_cu: int = 0, lds_stack: int = 16, refill_steps: int = 8) -> None:
+    def tune(self, blocks_per_cu: int = 0, lds_stack: int = 16, refill_steps: int = 16) -> None:
         _native.check(_native.lib().nan_tracer_tune(self.handle, int(blocks_per_cu), int(lds_stack),
                                                     int(refill_steps)), "nan_tracer_tune")
         self.tuning = (int(blocks_per_cu), int(lds_stack), int(refill_steps))

@@ -277,6 +277,62 @@ struct TraversalStats {
     uint32_t nodes, tris;
 };
 
+// The children of a 4-wide node a ray enters: conservative slab tests on the
+// quantized boxes (decode origin + q * 2^e, one rounding), culled beyond
+// `limit`; returns how many survive, in e0..e3 nearest first (closest hit) or
+// in node order (any hit).  The visiting order never changes a result.
+template <bool ANY_HIT>
+__device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, const uint4 &w2, const uint4 &w3,
+                                             const Ray &r, float limit, float slack, uint32_t &e0, uint32_t &e1,
+                                             uint32_t &e2, uint32_t &e3) {
+    f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
+    float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
+    float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
+    float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
+    uint32_t valid = w0.w >> 24;
+    uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
+    float tn[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float lox = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), sx, origin.x);
+        float loy = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), sy, origin.y);
+        float loz = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), sz, origin.z);
+        float hix = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), sx, origin.x);
+        float hiy = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), sy, origin.y);
+        float hiz = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), sz, origin.z);
+        float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
+        float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
+        float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
+        float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+        float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + slack;
+        far = fminf(far, limit);
+        tn[c] = (((valid >> c) & 1u) && near <= far) ? near : INFINITY;  // +inf: culled
+    }
+    e0 = w2.z;
+    e1 = w2.w;
+    e2 = w3.x;
+    e3 = w3.y;
+    float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
+    int n = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
+    // a 4-input sorting network: ascending entry distance (closest hit) or
+    // survivors first (any hit); culled children go last either way
+    auto cswap = [](float &ka, uint32_t &ea, float &kb, uint32_t &eb) {
+        bool sw = ANY_HIT ? (!(ka < INFINITY) && kb < INFINITY) : kb < ka;
+        float k = sw ? kb : ka;
+        uint32_t e = sw ? eb : ea;
+        kb = sw ? ka : kb;
+        eb = sw ? ea : eb;
+        ka = k;
+        ea = e;
+    };
+    cswap(k0, e0, k1, e1);
+    cswap(k2, e2, k3, e3);
+    cswap(k0, e0, k2, e2);
+    cswap(k1, e1, k3, e3);
+    cswap(k1, e1, k2, e2);
+    return n;
+}
+
 // Stack of child entries in LDS: STACK rows of BLOCK lanes, lane-major so
 // the 64 lanes of a wave hit 64 different banks.
 template <int STACK, int BLOCK, bool ANY_HIT, bool COUNT>
@@ -289,63 +345,19 @@ __device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &h
             const uint4 *np = S.nodes + 4 * (size_t)cur;
             uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
             if (COUNT) st.nodes++;
-            f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
-            float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
-            float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
-            float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
-            uint32_t valid = w0.w >> 24;
-            uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
-            uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
-            float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack;
-            float tn[4];
-            uint32_t ent[4];
-            int nhit = 0;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float lox = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), sx, origin.x);
-                float loy = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), sy, origin.y);
-                float loz = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), sz, origin.z);
-                float hix = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), sx, origin.x);
-                float hiy = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), sy, origin.y);
-                float hiz = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), sz, origin.z);
-                float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
-                float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
-                float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
-                float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-                float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + S.slack;
-                far = fminf(far, limit);
-                bool ok = ((valid >> c) & 1u) && near <= far;
-                // insertion into a sorted list (ascending near), unrolled
-                if (ok) {
-                    float key = near;
-                    uint32_t e = child[c];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (k < nhit && key < tn[k]) {
-                            float tk = tn[k];
-                            uint32_t ek = ent[k];
-                            tn[k] = key;
-                            ent[k] = e;
-                            key = tk;
-                            e = ek;
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        if (k == nhit) { tn[k] = key; ent[k] = e; }
-                    nhit++;
-                }
-            }
+            uint32_t e0, e1, e2, e3;
+            int nhit = node_children<ANY_HIT>(w0, w1, w2, w3, r, (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack, S.slack,
+                                              e0, e1, e2, e3);
             if (nhit == 0) {
                 if (sp == 0) break;
                 cur = lds_col[(--sp) * BLOCK];
                 continue;
             }
             // push the far children (farthest first), continue with the nearest
-#pragma unroll
-            for (int k = 3; k >= 1; --k)
-                if (k < nhit) lds_col[(sp++) * BLOCK] = ent[k];
-            cur = ent[0];
+            if (nhit > 3) lds_col[(sp++) * BLOCK] = e3;
+            if (nhit > 2) lds_col[(sp++) * BLOCK] = e2;
+            if (nhit > 1) lds_col[(sp++) * BLOCK] = e1;
+            cur = e0;
             continue;
         }
         // leaf

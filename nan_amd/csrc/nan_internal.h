@@ -59,7 +59,7 @@ int wavefront_set_frame(NanTracer *t, uint32_t iteration, uint32_t reset);
 struct WfTuning {
     uint32_t blocks_per_cu = 0;  // persistent grid: 0 = occupancy limit
     uint32_t ring = 16;          // LDS stack ring entries per lane: 16, 24 or 32
-    uint32_t refill_steps = 8;   // traversal steps between lane refills
+    uint32_t refill_steps = 16;  // traversal steps between lane refills
 };
 
 }  // namespace nanx

@@ -94,13 +94,6 @@ __device__ __forceinline__ void queue_counts(const uint32_t *counters, uint32_t 
     for (uint32_t j = 0; j < kParts; ++j) c[j] = min(counters[((size_t)slot * kParts + j) * kCtrStride], part_cap);
 }
 
-__device__ __forceinline__ uint32_t pick(const uint32_t (&c)[kParts], uint32_t j) {
-    uint32_t v = c[0];
-#pragma unroll
-    for (uint32_t t = 1; t < kParts; ++t) v = j == t ? c[t] : v;
-    return v;
-}
-
 // Storage index of the k-th entry of a queue (partitions in order).
 __device__ __forceinline__ uint32_t queue_entry(const uint32_t (&c)[kParts], uint32_t k, uint32_t part_cap) {
     uint32_t j = 0, off = k;
@@ -152,102 +145,66 @@ __device__ __forceinline__ void block_append2(bool want_a, bool want_b, uint32_t
     __syncthreads();  // s_a / s_b are reused by the next call
 }
 
-// Resumable traversal with a RING-entry LDS ring per lane that spills
-// its oldest entries to HBM.  One call of trav_step() processes one node or
-// one leaf; the visiting rule and the result equal traverse() in
-// nan_device.h (closest hit = smallest (t, triangle id); any-hit stops at
-// the first hit in [tmin, tmax], in whatever order children are visited).
+// Resumable traversal with a RING-entry LDS ring per lane that spills its
+// oldest entries to HBM, in two phases a wave runs in step (after Aila and
+// Laine, "Understanding the Efficiency of Ray Traversal on GPUs", 2009):
+// in the node phase each lane visits one inner node, or parks the leaf it
+// reached and moves on to its next stacked entry; in the leaf phase every lane
+// with a parked leaf tests that leaf's triangles.  A wave thus never runs the
+// node and the triangle code in the same step for different lanes.  The
+// result equals traverse() in nan_device.h (closest hit = smallest
+// (t, triangle id); any-hit stops at the first hit in [tmin, tmax]), in
+// whatever order nodes and leaves are visited.
+constexpr uint32_t kNone = 0xFFFFFFFFu;  // not a node or leaf entry (leaves hold at most 8 triangles)
+
 struct TravState {
-    uint32_t cur;
+    uint32_t cur;   // next entry to visit, kNone once the stack is exhausted (then sp == 0)
     int sp;
+    uint32_t pend;  // leaf parked for the next leaf phase, or kNone
 };
 
-template <int RING, bool ANY_HIT, bool COUNT>
-__device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, float slack, Hit &hit, TravState &ts,
-                                          uint32_t *lds_col, uint32_t *spill_col, uint32_t spill_stride,
-                                          TraversalStats &st) {
-    auto push = [&](uint32_t e) {
+template <int RING>
+struct LaneStack {
+    uint32_t *lds_col, *spill_col;
+    uint32_t spill_stride;
+    __device__ __forceinline__ void push(TravState &ts, uint32_t e) const {
         uint32_t *slot = lds_col + (ts.sp % RING) * kTravBlock;
         if (ts.sp >= RING) spill_col[(size_t)(ts.sp - RING) * spill_stride] = *slot;
         *slot = e;
         ++ts.sp;
-    };
-    auto pop = [&]() -> uint32_t {
+    }
+    __device__ __forceinline__ uint32_t pop(TravState &ts) const {
         --ts.sp;
         uint32_t *slot = lds_col + (ts.sp % RING) * kTravBlock;
         uint32_t e = *slot;
         if (ts.sp >= RING) *slot = spill_col[(size_t)(ts.sp - RING) * spill_stride];
         return e;
-    };
-    if (!(ts.cur & kLeafBit)) {
+    }
+};
+
+// Node phase step; returns true when the ray has nothing left to visit.
+template <int RING, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ bool node_step(const DevScene &S, const Ray &r, float slack, const Hit &hit,
+                                          TravState &ts, const LaneStack<RING> &stk, TraversalStats &st) {
+    auto push = [&](uint32_t e) { stk.push(ts, e); };
+    auto pop = [&]() -> uint32_t { return stk.pop(ts); };
+    if (ts.cur == kNone) return ts.pend == kNone;
+    if (ts.cur & kLeafBit) {
+        if (ts.pend != kNone) return false;  // waits for the leaf phase
+        ts.pend = ts.cur;
+        ts.cur = ts.sp ? pop() : kNone;
+        return false;
+    }
+    {
         const uint4 *np = S.nodes + 4 * (size_t)ts.cur;
         uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
         if (COUNT) st.nodes++;
-        f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
-        float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
-        float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
-        float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
-        uint32_t valid = w0.w >> 24;
-        uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
-        float limit = (ANY_HIT ? r.tmax : hit.t) * kPad + slack;
-        // entry distance of each child (+inf when culled)
-        float tn[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            float lox = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), sx, origin.x);
-            float loy = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), sy, origin.y);
-            float loz = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), sz, origin.z);
-            float hix = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), sx, origin.x);
-            float hiy = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), sy, origin.y);
-            float hiz = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), sz, origin.z);
-            float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
-            float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
-            float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
-            float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-            float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + slack;
-            far = fminf(far, limit);
-            tn[c] = (((valid >> c) & 1u) && near <= far) ? near : INFINITY;
-        }
-        uint32_t e0 = w2.z, e1 = w2.w, e2 = w3.x, e3 = w3.y;
-        float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
-        int nhit = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
-        if (!ANY_HIT) {
-            // nearest first: a 4-input sorting network (culled children sort last)
-            auto cswap = [](float &ka, uint32_t &ea, float &kb, uint32_t &eb) {
-                bool sw = kb < ka;
-                float k = sw ? kb : ka;
-                uint32_t e = sw ? eb : ea;
-                kb = sw ? ka : kb;
-                eb = sw ? ea : eb;
-                ka = k;
-                ea = e;
-            };
-            cswap(k0, e0, k1, e1);
-            cswap(k2, e2, k3, e3);
-            cswap(k0, e0, k2, e2);
-            cswap(k1, e1, k3, e3);
-            cswap(k1, e1, k2, e2);
-        } else {
-            // any order: compact the surviving children to the front
-            auto take = [](float &ka, uint32_t &ea, float &kb, uint32_t &eb) {
-                bool sw = !(ka < INFINITY) && kb < INFINITY;
-                float k = sw ? kb : ka;
-                uint32_t e = sw ? eb : ea;
-                kb = sw ? ka : kb;
-                eb = sw ? ea : eb;
-                ka = k;
-                ea = e;
-            };
-            take(k0, e0, k1, e1);
-            take(k2, e2, k3, e3);
-            take(k0, e0, k2, e2);
-            take(k1, e1, k3, e3);
-            take(k1, e1, k2, e2);
-        }
+        uint32_t e0, e1, e2, e3;
+        int nhit = node_children<ANY_HIT>(w0, w1, w2, w3, r, (ANY_HIT ? r.tmax : hit.t) * kPad + slack, slack, e0,
+                                          e1, e2, e3);
         if (nhit == 0) {
-            if (ts.sp == 0) return true;
-            ts.cur = pop();
-            return false;
+            ts.cur = ts.sp ? pop() : kNone;
+            return ts.cur == kNone && ts.pend == kNone;
         }
         if (nhit > 3) push(e3);
         if (nhit > 2) push(e2);
@@ -255,31 +212,38 @@ __device__ __forceinline__ bool trav_step(const DevScene &S, const Ray &r, float
         ts.cur = e0;
         return false;
     }
-    uint32_t first = ts.cur & kLeafFirstMask;
-    uint32_t count = ((ts.cur >> kLeafCountShift) & 0xFu) + 1u;
-    for (uint32_t i = 0; i < count; ++i) {
-        const float4 *tp = S.tris + 3 * (size_t)(first + i);
-        float4 a = tp[0], b = tp[1], c = tp[2];
-        if (COUNT) st.tris++;
-        float t, b1, b2;
-        if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
-            uint32_t id = __float_as_uint(c.y);
-            if (ANY_HIT) {
-                hit.t = t;
-                hit.id = id;
-                return true;
-            }
-            if (t < hit.t || (t == hit.t && id < hit.id)) {
-                hit.t = t;
-                hit.b1 = b1;
-                hit.b2 = b2;
-                hit.id = id;
+}
+
+// Leaf phase step: tests the parked leaf; returns true when the ray is done.
+template <bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ bool leaf_step(const DevScene &S, const Ray &r, Hit &hit, TravState &ts,
+                                          TraversalStats &st) {
+    if (ts.pend != kNone) {
+        uint32_t first = ts.pend & kLeafFirstMask;
+        uint32_t count = ((ts.pend >> kLeafCountShift) & 0xFu) + 1u;
+        ts.pend = kNone;
+        for (uint32_t i = 0; i < count; ++i) {
+            const float4 *tp = S.tris + 3 * (size_t)(first + i);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            if (COUNT) st.tris++;
+            float t, b1, b2;
+            if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
+                uint32_t id = __float_as_uint(c.y);
+                if (ANY_HIT) {
+                    hit.t = t;
+                    hit.id = id;
+                    return true;
+                }
+                if (t < hit.t || (t == hit.t && id < hit.id)) {
+                    hit.t = t;
+                    hit.b1 = b1;
+                    hit.b2 = b2;
+                    hit.id = id;
+                }
             }
         }
     }
-    if (ts.sp == 0) return true;
-    ts.cur = pop();
-    return false;
+    return ts.cur == kNone;
 }
 
 // Persistent traversal over a ray queue with per-lane refill: a lane whose
@@ -296,10 +260,11 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counter
     if (threadIdx.x < kParts)
         s_cnt[threadIdx.x] = min(counters[((size_t)count_slot * kParts + threadIdx.x) * kCtrStride], part_cap);
     __syncthreads();
+    const LaneStack<RING> stk{lds_col, spill_col, spill_stride};
     Ray r;
     Hit h;
     float slack = 0.0f;
-    TravState ts{0, 0};
+    TravState ts{0, 0, kNone};
     TraversalStats st{0, 0};
     uint32_t k = 0;
     bool has = false;
@@ -325,7 +290,7 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counter
                     load(k, r, root, slack);
                     ray_setup(r);
                     h = Hit{r.tmax, 0.0f, 0.0f, kInvalid};
-                    ts = TravState{root, 0};
+                    ts = TravState{root, 0, kNone};
                     st = TraversalStats{0, 0};
                     has = true;
                 }
@@ -336,16 +301,28 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counter
             }
         }
         if (__ballot(has) == 0ull) break;
-        for (uint32_t s = 0; s < steps && has; ++s) {
-            if (trav_step<RING, ANY_HIT, COUNT>(S, r, slack, h, ts, lds_col, spill_col, spill_stride, st)) {
-                store(k, h);
-                has = false;
-                if (COUNT) {
-                    n_rays++;
-                    n_nodes += st.nodes;
-                    n_tris += st.tris;
+        for (uint32_t s = 0; s < steps; ++s) {
+            // leaf phase once the lanes waiting on a parked leaf are at least
+            // half as many as those that can still visit nodes (measured
+            // against 1/8, 1/4, 1, 2 and "all waiting": this is the fastest)
+            bool blocked = has && ts.pend != kNone && (ts.cur == kNone || (ts.cur & kLeafBit));
+            bool movable = has && ts.cur != kNone && (!(ts.cur & kLeafBit) || ts.pend == kNone);
+            unsigned long long mb = __ballot(blocked), mm = __ballot(movable);
+            bool leaf_phase = mb && (!mm || 2 * __popcll(mb) >= __popcll(mm));
+            if (has) {
+                bool done = leaf_phase ? leaf_step<ANY_HIT, COUNT>(S, r, h, ts, st)
+                                       : node_step<RING, ANY_HIT, COUNT>(S, r, slack, h, ts, stk, st);
+                if (done) {
+                    store(k, h);
+                    has = false;
+                    if (COUNT) {
+                        n_rays++;
+                        n_nodes += st.nodes;
+                        n_tris += st.tris;
+                    }
                 }
             }
+            if (__ballot(has) == 0ull) break;
         }
     }
 }

@@ -146,7 +146,7 @@ class TextureSpacePathTracer:
             None if counters is None else counters.data_ptr(), command_encoder.stream_handle)
         _native.check(rc, "nan_tst_execute_learned")
 
-    def tune(self, blocks_per_cu: int = 0, lds_stack: int = 16, refill_steps: int = 8) -> None:
+    def tune(self, blocks_per_cu: int = 0, lds_stack: int = 16, refill_steps: int = 16) -> None:
         """Launch shape of the traversal kernels (results do not depend on it)."""
         _native.check(_native.lib().nan_tracer_tune(self.handle, int(blocks_per_cu), int(lds_stack),
                                                     int(refill_steps)), "nan_tracer_tune")
