@@ -103,16 +103,17 @@ __device__ __forceinline__ Corners level_corners(const NrcGrid &E, uint32_t leve
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-// One thread per (sample, pair of levels) over np (a multiple of 64) samples:
-// 16 corner gathers of 4 bytes (both features of an entry) and one 8-byte
-// store of the four features in fragment order.  Thread 'pairs' of a sample
-// writes normal, albedo and zero padding.  Samples >= n get zeros.
+// One thread per (sample, pair of levels) over np (a multiple of 64) samples,
+// level-major (blockIdx.y = the pair, so a wave gathers 64 neighbouring
+// samples from one level's table): 16 corner gathers of 4 bytes (both
+// features of an entry) and one 8-byte store of the four features in
+// fragment order.  Pair 'pairs' writes normal, albedo and zero padding.
+// Samples >= n get zeros.
 __global__ void nrc_encode_fwd(NrcGrid E, const float *pos, const float *nrm, const float *alb, uint32_t n,
                                uint32_t np, _Float16 *enc, float *enc_f32 /* nullable [n][48] */) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t pairs = (E.levels + 1u) / 2u, per = pairs + 1u;
-    if (t >= (uint64_t)np * per) return;
-    uint32_t s = (uint32_t)(t / per), p = (uint32_t)(t % per);
+    const uint32_t pairs = (E.levels + 1u) / 2u;
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x, p = blockIdx.y;
+    if (s >= np) return;
     if (p == pairs) {
         float v[kNrcIn - 32];
         for (int k = 0; k < kNrcIn - 32; ++k) v[k] = 0.0f;
@@ -555,9 +556,8 @@ uint32_t bwd_waves(uint32_t groups) { return std::max<uint32_t>(1, std::min<uint
 void encode(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float *enc_f32,
             hipStream_t s) {
     uint32_t np = groups_for(n) * 64;
-    uint64_t threads = (uint64_t)np * ((c->cfg.levels + 1) / 2 + 1);
-    hipLaunchKernelGGL(nrc_encode_fwd, dim3(grid_for(threads, 256)), dim3(256), 0, s, grid_of(c), pos, nrm, alb, n,
-                       np, c->enc, enc_f32);
+    dim3 grid(grid_for(np, 256), (c->cfg.levels + 1) / 2 + 1);
+    hipLaunchKernelGGL(nrc_encode_fwd, grid, dim3(256), 0, s, grid_of(c), pos, nrm, alb, n, np, c->enc, enc_f32);
 }
 
 // Forward (saving activations), loss and backward into c->wpart / c->denc.
