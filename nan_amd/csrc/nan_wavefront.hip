@@ -941,26 +941,36 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     return 0;
 }
 
+// Persistent grid of a traversal launch: the occupancy-sized grid, or fewer
+// blocks (a multiple of kParts) when the queue holds at most max_rays rays.
+static uint32_t trav_grid(const WfBuffers &W, uint64_t max_rays) {
+    uint64_t need = (grid_for(max_rays, kTravBlock) + kParts - 1) / kParts * kParts;
+    return (uint32_t)std::min<uint64_t>(W.trav_blocks, std::max<uint64_t>(need, kParts));
+}
+
 template <int RING>
 static void launch_extend(const WfBuffers &W, const DevScene &S, uint32_t q, uint32_t count_slot, uint32_t fetch,
-                          uint32_t steps, unsigned long long *counters, hipStream_t stream) {
+                          uint32_t steps, unsigned long long *counters, hipStream_t stream,
+                          uint64_t max_rays = ~0ull) {
+    uint32_t blocks = trav_grid(W, max_rays);
     if (counters)
-        hipLaunchKernelGGL((wf_extend<RING, true>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
-                           count_slot, fetch, steps, counters);
+        hipLaunchKernelGGL((wf_extend<RING, true>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, q, count_slot,
+                           fetch, steps, counters);
     else
-        hipLaunchKernelGGL((wf_extend<RING, false>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W, q,
-                           count_slot, fetch, steps, counters);
+        hipLaunchKernelGGL((wf_extend<RING, false>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, q, count_slot,
+                           fetch, steps, counters);
 }
 
 template <int RING>
 static void launch_shadow(const WfBuffers &W, const DevScene &S, uint32_t count_slot, uint32_t fetch, uint32_t steps,
-                          unsigned long long *counters, hipStream_t stream) {
+                          unsigned long long *counters, hipStream_t stream, uint64_t max_rays = ~0ull) {
+    uint32_t blocks = trav_grid(W, max_rays);
     if (counters)
-        hipLaunchKernelGGL((wf_shadow<RING, true>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
-                           count_slot, fetch, steps, counters);
+        hipLaunchKernelGGL((wf_shadow<RING, true>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, count_slot,
+                           fetch, steps, counters);
     else
-        hipLaunchKernelGGL((wf_shadow<RING, false>), dim3(W.trav_blocks), dim3(kTravBlock), 0, stream, S, W,
-                           count_slot, fetch, steps, counters);
+        hipLaunchKernelGGL((wf_shadow<RING, false>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, count_slot,
+                           fetch, steps, counters);
 }
 
 int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
@@ -999,9 +1009,9 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                 uint32_t fetch = slot++;
                 {
                     StageTimer tm(t, stream, kStageExtend);
-                    if (W.ring == 16) launch_extend<16>(W, S, q, ext_slot, fetch, steps, counters, stream);
-                    else if (W.ring == 24) launch_extend<24>(W, S, q, ext_slot, fetch, steps, counters, stream);
-                    else launch_extend<32>(W, S, q, ext_slot, fetch, steps, counters, stream);
+                    if (W.ring == 16) launch_extend<16>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
+                    else if (W.ring == 24) launch_extend<24>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
+                    else launch_extend<32>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
                 }
                 uint32_t next_ext = slot++;
                 ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces};
@@ -1012,9 +1022,11 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                 uint32_t sfetch = slot++;
                 {
                     StageTimer tm(t, stream, kStageShadow);
-                    if (W.ring == 16) launch_shadow<16>(W, S, sh_slot, sfetch, steps, counters, stream);
-                    else if (W.ring == 24) launch_shadow<24>(W, S, sh_slot, sfetch, steps, counters, stream);
-                    else launch_shadow<32>(W, S, sh_slot, sfetch, steps, counters, stream);
+                    // at most one sun ray per path from generate (bounce 1) and one from shade
+                    uint64_t max_sh = 2ull * count;
+                    if (W.ring == 16) launch_shadow<16>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
+                    else if (W.ring == 24) launch_shadow<24>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
+                    else launch_shadow<32>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
                 }
                 sh_slot = slot++;
                 ext_slot = next_ext;
@@ -1054,17 +1066,17 @@ static int learned_checks(NanTracer *t) {
 }
 
 static void launch_extend_any(const WfBuffers &W, const DevScene &S, uint32_t q, uint32_t count_slot, uint32_t fetch,
-                              uint32_t steps, hipStream_t stream) {
-    if (W.ring == 16) launch_extend<16>(W, S, q, count_slot, fetch, steps, nullptr, stream);
-    else if (W.ring == 24) launch_extend<24>(W, S, q, count_slot, fetch, steps, nullptr, stream);
-    else launch_extend<32>(W, S, q, count_slot, fetch, steps, nullptr, stream);
+                              uint32_t steps, hipStream_t stream, uint64_t max_rays) {
+    if (W.ring == 16) launch_extend<16>(W, S, q, count_slot, fetch, steps, nullptr, stream, max_rays);
+    else if (W.ring == 24) launch_extend<24>(W, S, q, count_slot, fetch, steps, nullptr, stream, max_rays);
+    else launch_extend<32>(W, S, q, count_slot, fetch, steps, nullptr, stream, max_rays);
 }
 
 static void launch_shadow_any(const WfBuffers &W, const DevScene &S, uint32_t count_slot, uint32_t fetch,
-                              uint32_t steps, hipStream_t stream) {
-    if (W.ring == 16) launch_shadow<16>(W, S, count_slot, fetch, steps, nullptr, stream);
-    else if (W.ring == 24) launch_shadow<24>(W, S, count_slot, fetch, steps, nullptr, stream);
-    else launch_shadow<32>(W, S, count_slot, fetch, steps, nullptr, stream);
+                              uint32_t steps, hipStream_t stream, uint64_t max_rays) {
+    if (W.ring == 16) launch_shadow<16>(W, S, count_slot, fetch, steps, nullptr, stream, max_rays);
+    else if (W.ring == 24) launch_shadow<24>(W, S, count_slot, fetch, steps, nullptr, stream, max_rays);
+    else launch_shadow<32>(W, S, count_slot, fetch, steps, nullptr, stream, max_rays);
 }
 
 int wavefront_learned_train_paths(NanTracer *t, const LearnedIO &io, uint32_t iteration, uint32_t n,
@@ -1091,12 +1103,12 @@ int wavefront_learned_train_paths(NanTracer *t, const LearnedIO &io, uint32_t it
     uint32_t q = 0;
     for (uint32_t b = 1; b <= bounces; ++b) {
         uint32_t fetch = slot++;
-        launch_extend_any(W, S, q, ext_slot, fetch, steps, stream);
+        launch_extend_any(W, S, q, ext_slot, fetch, steps, stream, n);
         uint32_t next_ext = slot++;
         ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces, at_x0 ? 2u : 1u, io};
         hipLaunchKernelGGL(wf_shade, dim3(shade_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
         uint32_t sfetch = slot++;
-        launch_shadow_any(W, S, sh_slot, sfetch, steps, stream);
+        launch_shadow_any(W, S, sh_slot, sfetch, steps, stream, n);
         sh_slot = slot++;
         ext_slot = next_ext;
         q = 1u - q;
@@ -1123,11 +1135,11 @@ int wavefront_learned_side(NanTracer *t, const LearnedIO &io, uint32_t iteration
     hipLaunchKernelGGL(wf_learned_generate, dim3(std::min<uint32_t>(W.gen_blocks, round8)), dim3(kWfBlock), 0, stream,
                        S, Ly, W, io, gp);
     if (!at_x0) {
-        launch_extend_any(W, S, 0u, 0u, 2u, steps, stream);
+        launch_extend_any(W, S, 0u, 0u, 2u, steps, stream, count);
         hipLaunchKernelGGL(wf_learned_hit, dim3(std::min<uint32_t>(W.shade_blocks, round8)), dim3(kWfBlock), 0,
                            stream, S, W, io, 0u);
     }
-    launch_shadow_any(W, S, 1u, 3u, steps, stream);
+    launch_shadow_any(W, S, 1u, 3u, steps, stream, count);
     return check_launch("wavefront learned samples");
 }
 
