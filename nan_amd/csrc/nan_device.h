@@ -279,8 +279,9 @@ struct TraversalStats {
 
 // The children of a 4-wide node a ray enters: conservative slab tests on the
 // quantized boxes (decode origin + q * 2^e, one rounding), culled beyond
-// `limit`; returns how many survive, in e0..e3 nearest first (closest hit) or
-// in node order (any hit).  The visiting order never changes a result.
+// `limit`; returns how many survive, in e0..e3 nearest first (also for any
+// hit: a sun ray's occluders tend to be near, measured 1% faster than node
+// order).  The visiting order never changes a result.
 template <bool ANY_HIT>
 __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, const uint4 &w2, const uint4 &w3,
                                              const Ray &r, float limit, float slack, uint32_t &e0, uint32_t &e1,
@@ -314,10 +315,9 @@ __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, c
     e3 = w3.y;
     float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
     int n = (k0 < INFINITY) + (k1 < INFINITY) + (k2 < INFINITY) + (k3 < INFINITY);
-    // a 4-input sorting network: ascending entry distance (closest hit) or
-    // survivors first (any hit); culled children go last either way
+    // a 4-input sorting network on the entry distance; culled children go last
     auto cswap = [](float &ka, uint32_t &ea, float &kb, uint32_t &eb) {
-        bool sw = ANY_HIT ? (!(ka < INFINITY) && kb < INFINITY) : kb < ka;
+        bool sw = kb < ka;
         float k = sw ? kb : ka;
         uint32_t e = sw ? eb : ea;
         kb = sw ? ka : kb;
