@@ -161,10 +161,11 @@ __global__ void nrc_encode_fwd(NrcGrid E, const float *pos, const float *nrm, co
 // sums of many contributions inside FP16 range.
 constexpr float kTableGradScale = 1.0f / 16.0f;
 
+// One thread per (sample, level), level-major (blockIdx.y = level): the
+// atomics of a wave all go to one level's table.
 __global__ void nrc_encode_bwd(NrcGrid E, const float *pos, uint32_t n, const float *denc, __half2 *grad_table) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)n * E.levels) return;
-    uint32_t s = (uint32_t)(t / E.levels), level = (uint32_t)(t % E.levels);
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x, level = blockIdx.y;
+    if (s >= n) return;
     float g0 = denc[(size_t)s * kNrcIn + 2 * level], g1 = denc[(size_t)s * kNrcIn + 2 * level + 1];
     if (g0 == 0.0f && g1 == 0.0f) return;
     g0 *= kTableGradScale;
@@ -655,7 +656,7 @@ int nrc_train_one(NanNrc *c, const float *pos, const float *nrm, const float *al
                   const uint32_t *valid, uint32_t n, hipStream_t s) {
     if (nrc_reserve_train(c, n)) return -1;
     forward_backward(c, pos, nrm, alb, target, valid, n, s);
-    hipLaunchKernelGGL(nrc_encode_bwd, dim3(grid_for((uint64_t)n * c->cfg.levels, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(nrc_encode_bwd, dim3(grid_for(n, 256), c->cfg.levels), dim3(256), 0, s,
                        grid_of(c), pos, n, c->denc, c->tg);
     uint32_t reduce_blocks = grid_for(kNrcWeights, kReduceWeights);
     c->step++;
