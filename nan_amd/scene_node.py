@@ -116,6 +116,34 @@ class SceneNode:
     def triangle_count(self) -> int:
         return sum(self.meshes[m].triangle_count for m, _, _ in self.instances)
 
+    def add_scene_node(self, node: "SceneNode", transform: Transform) -> list[int]:
+        """Copies ``node``'s meshes and materials in and instances every one of
+        its instances under the one new ``transform`` (scene_node.py:682-718;
+        the node's own instance transforms are not composed, as there)."""
+        tid = self.add_transform(transform)
+        mesh_base = len(self.meshes)
+        mat_base = len(self.materials)
+        self.meshes.extend(node.meshes)
+        self.materials.extend(node.materials)
+        return [self.add_instance(mesh_base + m, mat_base + k, tid) for m, k, _ in node.instances]
+
+    @classmethod
+    def load_asset(cls, path: str, default_material: Material | None = None, scale: float = 1.0,
+                   auto_center: bool = False) -> "SceneNode":
+        """A scene from an OBJ, glTF/GLB, STL or PLY file
+        (scene_node.py:476-647); see :mod:`nan_amd.asset_import`."""
+        from . import asset_import
+
+        return asset_import.load_asset(cls, path, default_material, scale, auto_center)
+
+    @classmethod
+    def load_json(cls, path: str, axis_conversion: str | None = None) -> "SceneNode":
+        """A scene composed of asset files (called at app.py:151); see
+        :func:`nan_amd.asset_import.load_json` for the format."""
+        from . import asset_import
+
+        return asset_import.load_json(cls, path, axis_conversion)
+
     @classmethod
     def demo(cls) -> "SceneNode":
         """The reference's demo Cornell box structure (scene_node.py:721-830):
