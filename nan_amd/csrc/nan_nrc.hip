@@ -179,7 +179,19 @@ __global__ void nrc_encode_bwd(NrcGrid E, const float *pos, uint32_t n, const fl
 // ---------------------------------------------------------------------------
 // fused MLP forward
 // ---------------------------------------------------------------------------
-// acc[t][ot] = W^T block (ot, .) x X^T tile t, over IN/16 input blocks.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+// One K=32 MFMA over two K=16 fragments: hardware K index 8q+i of lane
+// quarter q stands for fragment 0's 4q+i (i < 4) and fragment 1's 4q+i-4,
+// the same map in both operands, so the product sums the same 32 terms.
+__device__ __forceinline__ floatx4_t mfma32(half4_t a0, half4_t a1, half4_t b0, half4_t b1, floatx4_t c) {
+    half8_t a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    half8_t b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// acc[t][ot] = W^T block (ot, .) x X^T tile t, over IN/16 input blocks taken
+// in pairs on the K=32 MFMA (an odd last block on K=16).
 template <int IN, int OUT>
 __device__ __forceinline__ void layer_fwd(const _Float16 *wl, const half4_t (&x)[4][4], floatx4_t (&acc)[4][4]) {
     const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
@@ -187,47 +199,68 @@ __device__ __forceinline__ void layer_fwd(const _Float16 *wl, const half4_t (&x)
     for (int ot = 0; ot < OUT / 16; ++ot) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t][ot] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        const _Float16 *row = &wl[(ot * 16 + r) * (IN + kWtPad) + 4 * q];
 #pragma unroll
-        for (int kb = 0; kb < IN / 16; ++kb) {
-            half4_t a = *(const half4_t *)&wl[(ot * 16 + r) * (IN + kWtPad) + kb * 16 + 4 * q];
+        for (int kb = 0; kb + 1 < IN / 16; kb += 2) {
+            half4_t a0 = *(const half4_t *)&row[kb * 16], a1 = *(const half4_t *)&row[kb * 16 + 16];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][ot] = mfma16(a, x[t][kb], acc[t][ot]);
+            for (int t = 0; t < 4; ++t) acc[t][ot] = mfma32(a0, a1, x[t][kb], x[t][kb + 1], acc[t][ot]);
+        }
+        if ((IN / 16) % 2) {
+            half4_t a = *(const half4_t *)&row[(IN / 16 - 1) * 16];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t][ot] = mfma16(a, x[t][IN / 16 - 1], acc[t][ot]);
         }
     }
 }
 
+// ReLU after the rounding to FP16, on packed halves (rounding is monotone and
+// keeps zero, so max(round(a), 0) = round(max(a, 0)) up to the sign of zero).
 __device__ __forceinline__ void relu_to_half(const floatx4_t (&acc)[4][4], half4_t (&x)[4][4]) {
+    const half4_t zero = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) x[t][kb][i] = (_Float16)fmaxf(acc[t][kb][i], 0.0f);
+            x[t][kb] = __builtin_elementwise_max(__builtin_convertvector(acc[t][kb], half4_t), zero);
+}
+
+// Copies the per-layer [out][in] FP16 weights into LDS rows padded by kWtPad,
+// 16 bytes at a time (every layer width and offset is a multiple of 8 halves).
+__device__ __forceinline__ void stage_weights(const _Float16 *wt, _Float16 *wl, uint32_t tid, uint32_t threads) {
+    static_assert(kNrcIn % 8 == 0 && kNrcWidth % 8 == 0 && kWtPad % 8 == 0, "16-byte weight rows");
+    for (int l = 0; l < kNrcLayers; ++l) {
+        const int in = nrc_layer_in(l), chunks = in * nrc_layer_out(l) / 8;
+        const uint4 *src = reinterpret_cast<const uint4 *>(wt + nrc_layer_offset(l));
+        _Float16 *dst = wl + wt_lds_offset(l);
+        for (int i = (int)tid; i < chunks; i += (int)threads) {
+            const int e = 8 * i;
+            *reinterpret_cast<uint4 *>(&dst[(e / in) * (in + kWtPad) + e % in]) = src[i];
+        }
+    }
 }
 
 // wt: fp16 weights, per layer [out][in].  enc: encoded input (fragment order,
 // KB = 3).  pred[s] = first four outputs of sample s < n.  save (SAVE): the
 // inputs of layers 1..4, [layer-1][group][4096] fragment order.
 template <bool SAVE>
-__global__ __launch_bounds__(kFwdBlock) void nrc_mlp_fwd(const _Float16 *wt, const _Float16 *enc, uint32_t groups,
+__global__ __launch_bounds__(kFwdBlock) __attribute__((amdgpu_waves_per_eu(SAVE ? 3 : 4, 4))) void nrc_mlp_fwd(const _Float16 *wt, const _Float16 *enc, uint32_t groups,
                                                           uint32_t n, float4 *pred, _Float16 *save,
                                                           uint32_t cap_groups, float *stats) {
-    __shared__ _Float16 wl[kWtLds];
+    __shared__ __align__(16) _Float16 wl[kWtLds];
     if (SAVE && blockIdx.x == 0 && threadIdx.x == 0) {  // training statistics of this step
         stats[0] = 0.0f;
         stats[1] = (float)n;
         stats[2] = 0.0f;
     }
-    for (int l = 0; l < kNrcLayers; ++l) {
-        const int in = nrc_layer_in(l), count = in * nrc_layer_out(l);
-        const _Float16 *src = wt + nrc_layer_offset(l);
-        _Float16 *dst = wl + wt_lds_offset(l);
-        for (int i = threadIdx.x; i < count; i += kFwdBlock) dst[(i / in) * (in + kWtPad) + i % in] = src[i];
-    }
+    stage_weights(wt, wl, threadIdx.x, kFwdBlock);
     __syncthreads();
     const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
     const uint32_t waves = gridDim.x * (kFwdBlock / 64);
     for (uint32_t g = blockIdx.x * (kFwdBlock / 64) + threadIdx.x / 64; g < groups; g += waves) {
+        // weights are read from LDS per group: kept in registers across groups
+        // they would leave room for two waves per SIMD
+        __asm__ volatile("" ::: "memory");
         half4_t x[4][4];
         floatx4_t acc[4][4];
         const _Float16 *e = enc + (size_t)g * kEncGroup;
@@ -550,7 +583,9 @@ NrcGrid grid_of(const NanNrc *c) {
 
 uint32_t groups_for(uint32_t n) { return (n + 63u) / 64u; }
 
-uint32_t fwd_blocks(uint32_t groups) { return std::max<uint32_t>(1, std::min<uint32_t>((groups + 3) / 4, 4096)); }
+// Four resident blocks of four waves per CU: 1024 blocks fill the chip once,
+// and each block stages the weights once for all the groups it takes.
+uint32_t fwd_blocks(uint32_t groups) { return std::max<uint32_t>(1, std::min<uint32_t>((groups + 3) / 4, 1024)); }
 
 uint32_t bwd_waves(uint32_t groups) { return std::max<uint32_t>(1, std::min<uint32_t>(groups, kNrcBwdWaves)); }
 
