@@ -301,22 +301,65 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
 
 LEARNED_FORMS = {
     # name: (texel_query, training records per frame, Adam minibatches)
-    "one_bounce": (False, 65536, 4),
-    "texel_query": (True, 262144, 8),
+    "one_bounce": (False, 65536, 8),
+    "texel_query": (True, 131072, 16),
 }
+LEARNED_FRAMES = (1, 2, 4, 8, 16, 64)
 
 
-def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 512, frames=(1, 4, 16, 64)) -> dict:
-    """Both learned forms against the plain tracer on the workload's own
-    scene: RMSE against a plain bake at each frame count, the plain frames
-    that reach it and the equal-RMSE speedup (plain frames x plain ms over
-    learned frames x learned ms).  ``one_bounce`` is the configured form
-    (65,536 suffixes per frame, paths end in the cache after one bounce);
-    ``texel_query`` queries the cache at the texel itself (no bounce ray;
-    262,144 training paths per frame in 8 Adam steps)."""
+def learned_forms_vs_plain(device, scene, tracer, rmse, curve: dict, plain_ms: float, frames=LEARNED_FRAMES) -> dict:
+    """Each form of LEARNED_FORMS from a fresh cache: its RMSE (``rmse``
+    against the plain bake) after each frame count in ``frames``, the plain
+    frames that reach it (``curve``: plain RMSE per frame count) and the
+    equal-RMSE speedup (plain frames x ``plain_ms`` over the learned time),
+    plus the learned frame's stage times."""
     import torch
 
     from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+
+    enc = device.create_command_encoder()
+    stream = torch.cuda.current_stream(device.torch_device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    slots = tracer.layout.total_payload_count
+    forms = {}
+    for name, (texel_query, n_train, mb) in LEARNED_FORMS.items():
+        cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=mb))
+        learned = torch.zeros((slots, 4), dtype=torch.float32, device=device.torch_device)
+        res = {}
+        elapsed = 0.0
+        for it in range(max(frames)):
+            ev0.record(stream)
+            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=n_train,
+                                   texel_query=texel_query)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            elapsed += ev0.elapsed_time(ev1)
+            if it + 1 in frames:
+                r = rmse(learned)
+                m = _plain_frames_for(curve, r)
+                res[str(it + 1)] = {"rmse": r, "plain_frames_for_equal_rmse": m,
+                                    "learned_ms_per_frame": elapsed / (it + 1),
+                                    "speedup_vs_plain": m * plain_ms / elapsed}
+        tracer.profile(True)
+        for it in range(2):
+            tracer.execute_learned(enc, learned, max(frames) + it, cache, train_samples=n_train,
+                                   texel_query=texel_query)
+        stages = {k: v[0] / 2 for k, v in tracer.stage_times().items()}
+        tracer.profile(False)
+        forms[name] = {"train_samples": n_train, "minibatches": mb, "frames": res, "stage_ms_per_frame": stages}
+        log(f"learned {name}: " + ", ".join(f"{k}: x{v['speedup_vs_plain']:.2f}" for k, v in res.items()))
+        del cache, learned
+    return forms
+
+
+def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 512, frames=LEARNED_FRAMES) -> dict:
+    """Both learned forms against the plain tracer on the workload's own
+    scene at equal RMSE against a plain bake (``learned_forms_vs_plain``).
+    ``one_bounce`` is the configured form (65,536 suffixes per frame, paths
+    end in the cache after one bounce); ``texel_query`` queries the cache at
+    the texel itself (no bounce ray; 131,072 training paths per frame in 16
+    Adam steps)."""
+    import torch
 
     enc = device.create_command_encoder()
     stream = torch.cuda.current_stream(device.torch_device)
@@ -347,34 +390,7 @@ def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 512, fram
             curve[it + 1] = rmse(plain)
     plain_ms /= max(frames)
     del plain
-    forms = {}
-    for name, (texel_query, n_train, mb) in LEARNED_FORMS.items():
-        cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=mb))
-        learned = torch.zeros_like(bake)
-        res = {}
-        elapsed = 0.0
-        for it in range(max(frames)):
-            ev0.record(stream)
-            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=n_train,
-                                   texel_query=texel_query)
-            ev1.record(stream)
-            torch.cuda.synchronize()
-            elapsed += ev0.elapsed_time(ev1)
-            if it + 1 in frames:
-                r = rmse(learned)
-                m = _plain_frames_for(curve, r)
-                res[str(it + 1)] = {"rmse": r, "plain_frames_for_equal_rmse": m,
-                                    "learned_ms_per_frame": elapsed / (it + 1),
-                                    "speedup_vs_plain": m * plain_ms / elapsed}
-        tracer.profile(True)
-        for it in range(2):
-            tracer.execute_learned(enc, learned, max(frames) + it, cache, train_samples=n_train,
-                                   texel_query=texel_query)
-        stages = {k: v[0] / 2 for k, v in tracer.stage_times().items()}
-        tracer.profile(False)
-        forms[name] = {"train_samples": n_train, "minibatches": mb, "frames": res, "stage_ms_per_frame": stages}
-        log(f"learned at scale, {name}: " + ", ".join(f"{k}: x{v['speedup_vs_plain']:.2f}" for k, v in res.items()))
-        del cache, learned
+    forms = learned_forms_vs_plain(device, scene, tracer, rmse, curve, plain_ms, frames)
     return {"bake_frames": bake_frames, "plain_ms_per_frame": plain_ms,
             "plain_rmse": {str(k): v for k, v in curve.items()}, "forms": forms}
 
@@ -503,24 +519,8 @@ def bench_combined(args) -> int:
         tracer.execute(enc, plain, it, reset=(it == 0))
         if (it + 1) & it == 0:
             curve[it + 1] = rmse(plain)
-    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=4))
-    learned = torch.zeros_like(bake)
-    learned_frames = {}
-    elapsed = 0.0
-    done = 0
-    for target in (16, 64):
-        r0.record(stream)
-        for it in range(done, target):
-            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0))
-        r1.record(stream)
-        torch.cuda.synchronize()
-        elapsed += r0.elapsed_time(r1)
-        done = target
-        r = rmse(learned)
-        m = _plain_frames_for(curve, r)
-        learned_frames[str(target)] = {"rmse": r, "plain_frames_for_equal_rmse": m,
-                                       "learned_ms_per_frame": elapsed / target,
-                                       "speedup_vs_plain": m * mean_ms / elapsed}
+    forms = learned_forms_vs_plain(device, scene, tracer, rmse, curve, mean_ms)
+    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig())
     q = slots  # one cache query per texel sample
     rng = np.random.default_rng(3)
     lo, hi = scene.bounds()
@@ -561,7 +561,7 @@ def bench_combined(args) -> int:
         "autotune": tune["best"],
         "resolve_3840x2160_ms": resolve_ms,
         "learned_vs_plain": {"bake_frames": args.bake_frames, "plain_rmse": {str(k): v for k, v in curve.items()},
-                             "learned": learned_frames},
+                             "forms": forms},
         "roofline": {"bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
                      "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / MFMA_F16_PEAK_TFLOPS},
