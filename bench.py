@@ -643,7 +643,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["synthetic", "cornell", "batch", "learned", "combined"],
                     default="synthetic")
-    ap.add_argument("--bake-frames", type=int, default=1024, help="plain bake length of --workload combined")
+    ap.add_argument("--bake-frames", type=int, default=4096, help="plain bake length of --workload combined")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-cache comparison")
     ap.add_argument("--scale-bake-frames", type=int, default=512,
                     help="plain bake length of the learned comparison on the workload's own scene")
