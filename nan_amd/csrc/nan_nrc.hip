@@ -508,8 +508,10 @@ __global__ __launch_bounds__(256) void nrc_adam_mlp(float *w, float *m, float *v
 
 // Hash table, one entry (2 features) per thread: dense, or lazy (entries with
 // zero gradient keep their state).  Non-finite FP16 gradient sums are dropped.
-__global__ void nrc_adam_table(float2 *w, float2 *m, float2 *v, __half2 *grad, __half2 *w16, uint64_t n,
-                               const float *stats, AdamArgs a, int dense) {
+// An entry's state is one 32-byte record, so a touched entry costs one read
+// and one write of a single sector wherever it lies in the table.
+__global__ void nrc_adam_table(TableState *st, __half2 *grad, __half2 *w16, uint64_t n, const float *stats, AdamArgs a,
+                               int dense) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     __half2 g2 = grad[k];
@@ -517,18 +519,32 @@ __global__ void nrc_adam_table(float2 *w, float2 *m, float2 *v, __half2 *grad, _
     if (g[0] == 0.0f && g[1] == 0.0f && !dense) return;
     grad[k] = __halves2half2(__float2half(0.0f), __float2half(0.0f));
     float sc = grad_scale(stats) / kTableGradScale;
-    float2 wk = w[k], mk2 = m[k], vk = v[k];
-    float wv[2] = {wk.x, wk.y}, mv[2] = {mk2.x, mk2.y}, vv[2] = {vk.x, vk.y};
+    float4 *rec = reinterpret_cast<float4 *>(st + k);
+    float4 wm = rec[0], vp = rec[1];
+    float wv[2] = {wm.x, wm.y}, mv[2] = {wm.z, wm.w}, vv[2] = {vp.x, vp.y};
     for (int c = 0; c < 2; ++c) {
         float gc = isfinite(g[c]) ? g[c] * sc : 0.0f;
         mv[c] = a.b1 * mv[c] + (1.0f - a.b1) * gc;
         vv[c] = a.b2 * vv[c] + (1.0f - a.b2) * gc * gc;
         wv[c] = wv[c] - a.lr * (mv[c] / a.c1) / (sqrtf(vv[c] / a.c2) + a.eps);
     }
-    w[k] = make_float2(wv[0], wv[1]);
-    m[k] = make_float2(mv[0], mv[1]);
-    v[k] = make_float2(vv[0], vv[1]);
+    rec[0] = make_float4(wv[0], wv[1], mv[0], mv[1]);
+    rec[1] = make_float4(vv[0], vv[1], 0.0f, 0.0f);
     w16[k] = __halves2half2(__float2half(wv[0]), __float2half(wv[1]));
+}
+
+// FP32 master weights of the table in and out of the state records (one
+// float2 per entry), for the contiguous host copies of create/get/set.
+__global__ void nrc_table_scatter(TableState *st, const float2 *w, uint64_t n) {
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    *reinterpret_cast<float2 *>(st[k].w) = w[k];
+}
+
+__global__ void nrc_table_gather(const TableState *st, float2 *w, uint64_t n) {
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    w[k] = *reinterpret_cast<const float2 *>(st[k].w);
 }
 
 __global__ __launch_bounds__(256) void nrc_sum_partials(const float *partial, uint32_t parts, const float *stats,
@@ -612,8 +628,8 @@ void forward_backward(NanNrc *c, const float *pos, const float *nrm, const float
 }  // namespace
 
 NanNrc::~NanNrc() {
-    for (void *p : {(void *)w32, (void *)wm, (void *)wv, (void *)wpart, (void *)w16t, (void *)w16n, (void *)t32,
-                    (void *)tm, (void *)tv, (void *)tg, (void *)t16, (void *)enc, (void *)save, (void *)dy,
+    for (void *p : {(void *)w32, (void *)wm, (void *)wv, (void *)wpart, (void *)w16t, (void *)w16n, (void *)ts,
+                    (void *)tg, (void *)t16, (void *)enc, (void *)save, (void *)dy,
                     (void *)denc, (void *)pred, (void *)stats})
         hipFree(p);
 }
@@ -706,8 +722,8 @@ int nrc_train_one(NanNrc *c, const float *pos, const float *nrm, const float *al
     hipLaunchKernelGGL(nrc_adam_mlp, dim3(reduce_blocks), dim3(256), 0, s, c->w32, c->wm, c->wv, c->wpart,
                        bwd_waves(groups_for(n)), c->w16t, c->w16n, c->stats, a);
     uint64_t entries = c->table_params / 2;
-    hipLaunchKernelGGL(nrc_adam_table, dim3(grid_for(entries, 256)), dim3(256), 0, s, (float2 *)c->t32,
-                       (float2 *)c->tm, (float2 *)c->tv, c->tg, (__half2 *)c->t16, entries, c->stats, a,
+    hipLaunchKernelGGL(nrc_adam_table, dim3(grid_for(entries, 256)), dim3(256), 0, s, c->ts, c->tg,
+                       (__half2 *)c->t16, entries, c->stats, a,
                        (int)cf.dense_table_adam);
     return check_launch("nrc train");
 }
@@ -734,6 +750,36 @@ void weight_copies(const float *w, std::vector<_Float16> &wt, std::vector<_Float
         for (int o = 0; o < out; ++o)
             for (int i = 0; i < in; ++i) wn[off + i * out + o] = wt[off + o * in + i];
     }
+}
+
+
+// Host <-> table master weights (table_params floats, entry-major pairs)
+// through a temporary device buffer: contiguous copies, then a scatter into
+// or a gather out of the state records.
+int table_weights_in(NanNrc *c, const float *host) {
+    uint64_t entries = c->table_params / 2;
+    float2 *tmp = nullptr;
+    HIP_TRY(hipMalloc((void **)&tmp, entries * sizeof(float2)), nullptr, 0);
+    hipError_t e = hipMemcpy(tmp, host, entries * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(nrc_table_scatter, dim3(grid_for(entries, 256)), dim3(256), 0, 0, c->ts, tmp, entries);
+        e = hipDeviceSynchronize();
+    }
+    hipFree(tmp);
+    HIP_TRY(e, nullptr, 0);
+    return 0;
+}
+
+int table_weights_out(const NanNrc *c, float *host) {
+    uint64_t entries = c->table_params / 2;
+    float2 *tmp = nullptr;
+    HIP_TRY(hipMalloc((void **)&tmp, entries * sizeof(float2)), nullptr, 0);
+    hipLaunchKernelGGL(nrc_table_gather, dim3(grid_for(entries, 256)), dim3(256), 0, 0, c->ts, tmp, entries);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(host, tmp, entries * sizeof(float2), hipMemcpyDeviceToHost);
+    hipFree(tmp);
+    HIP_TRY(e, nullptr, 0);
+    return 0;
 }
 
 }  // namespace
@@ -793,13 +839,12 @@ NAN_API int nan_nrc_create(const NanNrcConfig *cfg, NanNrc **out, char *err, siz
               upload(&c->wm, zw.data(), zw.size() * 4, err, cap) == 0 &&
               upload(&c->wv, zw.data(), zw.size() * 4, err, cap) == 0 &&
               upload(&c->wpart, nullptr, (size_t)kNrcBwdWaves * kNrcWeights * 4, err, cap) == 0 &&
-              upload(&c->t32, t.data(), t.size() * 4, err, cap) == 0 &&
+              upload(&c->ts, nullptr, t.size() / 2 * sizeof(TableState), err, cap) == 0 &&
+              hipMemset(c->ts, 0, t.size() / 2 * sizeof(TableState)) == hipSuccess &&
+              table_weights_in(c, t.data()) == 0 &&
               upload(&c->t16, t16.data(), t16.size() * 2, err, cap) == 0 &&
-              upload(&c->tm, nullptr, t.size() * 4, err, cap) == 0 &&
-              upload(&c->tv, nullptr, t.size() * 4, err, cap) == 0 &&
               upload(&c->tg, nullptr, t.size() * 2, err, cap) == 0 &&
               upload(&c->stats, stats.data(), stats.size() * 4, err, cap) == 0 &&
-              hipMemset(c->tm, 0, t.size() * 4) == hipSuccess && hipMemset(c->tv, 0, t.size() * 4) == hipSuccess &&
               hipMemset(c->tg, 0, t.size() * 2) == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
               nrc_reserve_train(c, 65536) == 0;
     if (!ok) {
@@ -824,7 +869,7 @@ NAN_API int nan_nrc_get_params(const NanNrc *nrc, float *weights, float *table) 
     if (!nrc) return fail(nullptr, 0, "nan_nrc_get_params: null cache");
     HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
     if (weights) HIP_TRY(hipMemcpy(weights, nrc->w32, (size_t)kNrcWeights * 4, hipMemcpyDeviceToHost), nullptr, 0);
-    if (table) HIP_TRY(hipMemcpy(table, nrc->t32, nrc->table_params * 4, hipMemcpyDeviceToHost), nullptr, 0);
+    if (table && table_weights_out(nrc, table)) return -1;
     return 0;
 }
 
@@ -841,7 +886,7 @@ NAN_API int nan_nrc_set_params(NanNrc *nrc, const float *weights, const float *t
     if (table) {
         std::vector<_Float16> h;
         to_half(table, nrc->table_params, h);
-        HIP_TRY(hipMemcpy(nrc->t32, table, nrc->table_params * 4, hipMemcpyHostToDevice), nullptr, 0);
+        if (table_weights_in(nrc, table)) return -1;
         HIP_TRY(hipMemcpy(nrc->t16, h.data(), h.size() * 2, hipMemcpyHostToDevice), nullptr, 0);
     }
     return 0;

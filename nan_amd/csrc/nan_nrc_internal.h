@@ -36,6 +36,12 @@ struct NrcGrid {
 
 }  // namespace nanx
 
+// Optimiser state of one hash-table entry (both features): FP32 master
+// weights, Adam first and second moments, padding to 32 bytes.
+struct alignas(32) TableState {
+    float w[2], m[2], v[2], pad[2];
+};
+
 struct NanNrc {
     NanNrcConfig cfg{};
     uint32_t table_entries = 0;
@@ -46,8 +52,9 @@ struct NanNrc {
     // plain ([in][out]), Adam moments, per-wave gradient partials
     float *w32 = nullptr, *wm = nullptr, *wv = nullptr, *wpart = nullptr;
     _Float16 *w16t = nullptr, *w16n = nullptr;
-    // hash table: fp32 master, fp16 copy, Adam moments, packed fp16 gradient pairs
-    float *t32 = nullptr, *tm = nullptr, *tv = nullptr;
+    // hash table: optimiser state records, packed fp16 gradient pairs, fp16
+    // copy read by the encoding
+    TableState *ts = nullptr;
     __half2 *tg = nullptr;
     _Float16 *t16 = nullptr;
     // batch scratch (capacity in 64-sample groups): prediction (enc, pred)
