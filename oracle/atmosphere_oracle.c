@@ -18,7 +18,11 @@
  * geometric-series sum of the exponential densities; the horizon boundary of
  * the transmittance table; monotonicity in sun angle and height; the
  * azimuthal mirror symmetry of the sky-view table.  The multiple-scattering
- * table is pinned by invariants only (positive, finite, f_ms < 1). */
+ * table is pinned by invariants: finite and non-negative, exactly linear in
+ * the transmittance table (doubling it doubles every value bit for bit; a
+ * zero table gives zero), zero where the sun is under the ground; the
+ * sky-view table is exactly linear in its two input tables jointly and the
+ * sum of its single- and multiple-scattering parts. */
 #include <math.h>
 #include <stdint.h>
 

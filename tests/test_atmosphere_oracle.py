@@ -180,3 +180,37 @@ def test_view_transmittance_reads_the_bottom_row(tables):
     # the view point (0.2 km) lies below the first row's centre: clamp to row 0
     want = t[0, x0, :3] * (1 - w) + t[0, x0 + 1, :3] * w
     np.testing.assert_allclose(got, want, rtol=1e-5)
+
+
+def test_multiscattering_table_is_linear_in_the_transmittance_table(tables):
+    # every sun term and the ground bounce carry one transmittance factor and
+    # f_ms carries none: doubling the table doubles the result, bit for bit
+    # (a factor of two commutes with FP32 rounding and with the bilinear read)
+    t, m = tables
+    m2 = A.multiscattering_lut(2.0 * t)
+    np.testing.assert_array_equal(m2[..., :3], 2.0 * m[..., :3])
+    # a table of zeros leaves nothing: no term without a sun transmittance
+    assert (A.multiscattering_lut(np.zeros_like(t))[..., :3] == 0.0).all()
+
+
+def test_multiscattering_is_zero_with_the_sun_under_the_ground(tables):
+    # ground level, sun straight down: every sample point has the planet
+    # between it and the sun, and the ground bounce needs dot(pos, sun) > 0
+    _, m = tables
+    assert (m[0, 0, :3] == 0.0).all()
+    assert (m[0, -1, :3] > 0.0).all()
+
+
+@pytest.mark.parametrize("sun", [(0.0, 0.5, 0.866), (0.3, 0.2, -0.9)])
+def test_sky_view_table_is_linear_in_its_two_tables(tables, sun):
+    t, m = tables
+    s = A.sky_view_lut(t, m, sun)
+    # both tables doubled: the radiance doubles bit for bit
+    np.testing.assert_array_equal(A.sky_view_lut(2.0 * t, 2.0 * m, sun)[..., :3], 2.0 * s[..., :3])
+    # single scattering (sun transmittance) plus multiple scattering (psi); the
+    # in-segment integral (in - in * T) / ext cancels in FP32 on optically thin
+    # steps (relative error ~1e-7 / (1 - T)), so the split sums agree to 1e-3
+    single = A.sky_view_lut(t, np.zeros_like(m), sun)[..., :3].astype(np.float64)
+    multi = A.sky_view_lut(np.zeros_like(t), m, sun)[..., :3].astype(np.float64)
+    np.testing.assert_allclose(single + multi, s[..., :3], rtol=1e-3, atol=1e-12)
+    assert (single > 0.0).mean() > 0.4 and (multi > 0.0).all()
