@@ -292,18 +292,23 @@ __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, c
     float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
     uint32_t valid = w0.w >> 24;
     uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
+    // slab distance of plane q: (origin + q*s - o) * inv = q * (s*inv) + (origin - o) * inv,
+    // one FMA per plane.  s is a power of two, so s*inv is exact; the rest
+    // rounds (origin - o), the product and the FMA: |error| <= 2eps|(origin - o)*inv| + eps|t|,
+    // no larger than the 3 roundings of (origin + q*s - o) * inv it replaces and
+    // well inside kPad and the slack
+    const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+    const float bx = (origin.x - r.o.x) * r.inv.x, by = (origin.y - r.o.y) * r.inv.y,
+                bz = (origin.z - r.o.z) * r.inv.z;
     float tn[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        float lox = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), sx, origin.x);
-        float loy = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), sy, origin.y);
-        float loz = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), sz, origin.z);
-        float hix = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), sx, origin.x);
-        float hiy = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), sy, origin.y);
-        float hiz = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), sz, origin.z);
-        float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
-        float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
-        float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
+        float t0x = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), ax, bx);
+        float t1x = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), ax, bx);
+        float t0y = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), ay, by);
+        float t1y = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), ay, by);
+        float t0z = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), az, bz);
+        float t1z = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), az, bz);
         float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
         float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + slack;
         far = fminf(far, limit);
