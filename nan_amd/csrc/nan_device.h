@@ -22,7 +22,8 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kLeafCountShift = 27;
 constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
 // Ray-box slack.  A child box is visited when its entry distance is within
-// far * (1 + 2^-10) + slack and within best_t * (1 + 2^-10) + slack, where
+// far * (1 + 2^-10) + slack and within best_t * (1 + 2^-10) + slack (each one
+// FMA), where
 // slack = 2^-18 * max(1, largest world coordinate) (DevScene::slack).  The
 // relative term covers the slab test's rounding (2*gamma_3 of Ize, "Robust
 // BVH Ray Traversal", JCGT 2013); the absolute term covers the FP32 triangle
@@ -300,17 +301,24 @@ __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, c
     const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
     const float bx = (origin.x - r.o.x) * r.inv.x, by = (origin.y - r.o.y) * r.inv.y,
                 bz = (origin.z - r.o.z) * r.inv.z;
+    // the entry plane of each axis follows the sign of the direction (s*inv has
+    // the sign of inv and the FMA is monotone in q), so the near and far
+    // planes of all four children are picked once per node instead of
+    // ordering every pair: the same values as min/max of the two planes
+    const uint32_t nqx = r.inv.x >= 0.0f ? qlx : qhx, fqx = r.inv.x >= 0.0f ? qhx : qlx;
+    const uint32_t nqy = r.inv.y >= 0.0f ? qly : qhy, fqy = r.inv.y >= 0.0f ? qhy : qly;
+    const uint32_t nqz = r.inv.z >= 0.0f ? qlz : qhz, fqz = r.inv.z >= 0.0f ? qhz : qlz;
     float tn[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        float t0x = __builtin_fmaf((float)((qlx >> (8 * c)) & 0xFFu), ax, bx);
-        float t1x = __builtin_fmaf((float)((qhx >> (8 * c)) & 0xFFu), ax, bx);
-        float t0y = __builtin_fmaf((float)((qly >> (8 * c)) & 0xFFu), ay, by);
-        float t1y = __builtin_fmaf((float)((qhy >> (8 * c)) & 0xFFu), ay, by);
-        float t0z = __builtin_fmaf((float)((qlz >> (8 * c)) & 0xFFu), az, bz);
-        float t1z = __builtin_fmaf((float)((qhz >> (8 * c)) & 0xFFu), az, bz);
-        float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-        float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kPad + slack;
+        float tnx = __builtin_fmaf((float)((nqx >> (8 * c)) & 0xFFu), ax, bx);
+        float tfx = __builtin_fmaf((float)((fqx >> (8 * c)) & 0xFFu), ax, bx);
+        float tny = __builtin_fmaf((float)((nqy >> (8 * c)) & 0xFFu), ay, by);
+        float tfy = __builtin_fmaf((float)((fqy >> (8 * c)) & 0xFFu), ay, by);
+        float tnz = __builtin_fmaf((float)((nqz >> (8 * c)) & 0xFFu), az, bz);
+        float tfz = __builtin_fmaf((float)((fqz >> (8 * c)) & 0xFFu), az, bz);
+        float near = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+        float far = __builtin_fmaf(fminf(fminf(tfx, tfy), tfz), kPad, slack);
         far = fminf(far, limit);
         tn[c] = (((valid >> c) & 1u) && near <= far) ? near : INFINITY;  // +inf: culled
     }
@@ -351,7 +359,7 @@ __device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &h
             uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
             if (COUNT) st.nodes++;
             uint32_t e0, e1, e2, e3;
-            int nhit = node_children<ANY_HIT>(w0, w1, w2, w3, r, (ANY_HIT ? r.tmax : hit.t) * kPad + S.slack, S.slack,
+            int nhit = node_children<ANY_HIT>(w0, w1, w2, w3, r, __builtin_fmaf(ANY_HIT ? r.tmax : hit.t, kPad, S.slack), S.slack,
                                               e0, e1, e2, e3);
             if (nhit == 0) {
                 if (sp == 0) break;

@@ -200,7 +200,7 @@ __device__ __forceinline__ bool node_step(const DevScene &S, const Ray &r, float
         uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
         if (COUNT) st.nodes++;
         uint32_t e0, e1, e2, e3;
-        int nhit = node_children<ANY_HIT>(w0, w1, w2, w3, r, (ANY_HIT ? r.tmax : hit.t) * kPad + slack, slack, e0,
+        int nhit = node_children<ANY_HIT>(w0, w1, w2, w3, r, __builtin_fmaf(ANY_HIT ? r.tmax : hit.t, kPad, slack), slack, e0,
                                           e1, e2, e3);
         if (nhit == 0) {
             ts.cur = ts.sp ? pop() : kNone;
