@@ -288,17 +288,17 @@ __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, c
                                              const Ray &r, float limit, float slack, uint32_t &e0, uint32_t &e1,
                                              uint32_t &e2, uint32_t &e3) {
     f3 origin = mk(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z));
-    float sx = exp2i((int)(int8_t)(w0.w & 0xFFu));
-    float sy = exp2i((int)(int8_t)((w0.w >> 8) & 0xFFu));
-    float sz = exp2i((int)(int8_t)((w0.w >> 16) & 0xFFu));
     uint32_t valid = w0.w >> 24;
     uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
     // slab distance of plane q: (origin + q*s - o) * inv = q * (s*inv) + (origin - o) * inv,
-    // one FMA per plane.  s is a power of two, so s*inv is exact; the rest
+    // one FMA per plane.  s = 2^e (e: the signed exponent byte), so s*inv =
+    // ldexp(inv, e) is exact; the rest
     // rounds (origin - o), the product and the FMA: |error| <= 2eps|(origin - o)*inv| + eps|t|,
     // no larger than the 3 roundings of (origin + q*s - o) * inv it replaces and
     // well inside kPad and the slack
-    const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+    const float ax = __builtin_ldexpf(r.inv.x, (int)(int8_t)(w0.w & 0xFFu)),
+                ay = __builtin_ldexpf(r.inv.y, (int)(int8_t)((w0.w >> 8) & 0xFFu)),
+                az = __builtin_ldexpf(r.inv.z, (int)(int8_t)((w0.w >> 16) & 0xFFu));
     const float bx = (origin.x - r.o.x) * r.inv.x, by = (origin.y - r.o.y) * r.inv.y,
                 bz = (origin.z - r.o.z) * r.inv.z;
     // the entry plane of each axis follows the sign of the direction (s*inv has
