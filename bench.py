@@ -37,7 +37,7 @@ if str(ROOT) not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 # bumped whenever the tracer kernels change, so stale rocprof summaries are ignored
-KERNEL_VERSION = 4
+KERNEL_VERSION = 5
 METRIC = "texel-path samples/s (Mrays/s) at equal RMSE vs 4096-spp bake; learned vs plain"
 
 # algorithmic bytes (see the module docstring)
@@ -812,6 +812,9 @@ def main() -> int:
         rec = json.loads(prof.read_text())
         if str(layout.total_payload_count) in rec.get("workload", "") and rec.get("kernel_version") == KERNEL_VERSION:
             out["roofline"]["traffic"] = rec["traffic_bytes_per_launch"]
+            # the algorithmic bytes count every node and triangle read, most of
+            # them served by L2 and the MALL: the HBM rate is the traffic's
+            out["roofline"]["hbm_gbs"] = rec["traffic_bytes_per_launch"] / (avg_launch_ms * 1e-3) / 1e9
             out["roofline"]["l2_hit_rate"] = rec.get("l2_hit_rate")
             out["roofline"]["traffic_source"] = rec.get("source")
     if not args.no_learned and args.workload == "synthetic":
