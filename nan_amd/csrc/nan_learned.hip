@@ -461,12 +461,33 @@ LearnedIO learned_io(const LearnedDev &B) {
 }
 
 // The learned forms on the wavefront queues (same results as launch() below).
+// Training batches up to this many paths trace faster as one thread per path
+// (one launch) than through the queues (a dozen launches per bounce whose
+// fixed cost dominates small batches); both write the same records.
+constexpr uint32_t kQueuedTrainPaths = 524288;
+
 int launch_wavefront(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t bounces, uint32_t reset,
                      uint32_t n_train, uint32_t flags, hipStream_t s) {
     LearnedIO io = learned_io(t->learned->d);
     bool at_x0 = (flags & NAN_LEARNED_TEXEL_QUERY) != 0;
     if (n_train) {
-        if (wavefront_learned_train_paths(t, io, iteration, n_train, bounces, at_x0, s)) return -1;
+        if (n_train < kQueuedTrainPaths && t->scene->stack_depth != 0) {
+            StageTimer tm(t, s, kStageLearnedPaths);
+            const DevScene &S = t->scene->dev;
+            DevLayout Ly = dev_layout(t);
+            const LearnedDev &B = t->learned->d;
+            if (t->scene->stack_depth <= 32)
+                hipLaunchKernelGGL((learned_train_paths<32, kTraceBlock, false>), dim3(grid_for(n_train, kTraceBlock)),
+                                   dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, bounces, n_train, flags, B,
+                                   (unsigned long long *)nullptr);
+            else
+                hipLaunchKernelGGL((learned_train_paths<64, kTraceBlock, false>), dim3(grid_for(n_train, kTraceBlock)),
+                                   dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, bounces, n_train, flags, B,
+                                   (unsigned long long *)nullptr);
+            if (check_launch("learned training paths")) return -1;
+        } else if (wavefront_learned_train_paths(t, io, iteration, n_train, bounces, at_x0, s)) {
+            return -1;
+        }
         StageTimer tm(t, s, kStageCacheTrain);
         if (nrc_train_step(c, io.tr_pos, io.tr_nrm, io.tr_alb, io.tr_target, io.tr_valid, n_train, s)) return -1;
     }
