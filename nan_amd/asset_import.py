@@ -834,8 +834,12 @@ def load_json(cls, path: str, axis_conversion: str | None = None):
          "lighting": {"sun_direction": [...], "sun_intensity": pi,
                       "sun_color": [...], "sky_radiance": [...], "atmosphere": true}}
 
-    Each object is loaded with :func:`load_asset` and added through
-    ``add_scene_node`` under ``T * R * S`` (rotation X then Y then Z).
+    Each object is loaded with :func:`load_asset` and placed by its object
+    transform ``T * R * S`` (rotation X then Y then Z) composed with each of
+    its own instance transforms, so an instanced asset keeps its layout (the
+    reference's ``add_scene_node``, scene_node.py:682-718, gives every
+    instance the one object transform instead; that method keeps that
+    behaviour).
     With ``axis_conversion="z_up_to_y_up"`` the vectors and rotations of the
     description (not the asset files, which carry their own convention) are
     converted with the reference's helpers (scene_node.py:648-679).
@@ -864,7 +868,13 @@ def load_json(cls, path: str, axis_conversion: str | None = None):
         s = _vec3(obj, "scale", [1.0, 1.0, 1.0], where)
         if conv:
             t, r = z_up_to_y_up(t), rotation_z_up_to_y_up(r)
-        node.add_scene_node(child, Transform(t, [math.radians(a) for a in r], s))
+        obj = Transform(t, [math.radians(a) for a in r], s).matrix.astype(np.float64)
+        mesh_base, mat_base = len(node.meshes), len(node.materials)
+        node.meshes.extend(child.meshes)
+        node.materials.extend(child.materials)
+        for m, k, ti in child.instances:
+            placed = (obj @ child.transforms[ti].matrix.astype(np.float64)).astype(np.float32)
+            node.add_instance(mesh_base + m, mat_base + k, node.add_transform(Transform.from_matrix(placed)))
     cam = doc.get("camera")
     if cam is not None:
         pos = _vec3(cam, "position", list(node.camera.position), "camera")

@@ -331,8 +331,9 @@ def test_add_scene_node_and_load_json(tmp_path):
     (tmp_path / "s.json").write_text(json.dumps(desc))
     node = SceneNode.load_json(str(tmp_path / "s.json"), axis_conversion="z_up_to_y_up")
     assert len(node.instances) == 9 and len(node.meshes) == 8
-    # the demo's instances share the one object transform (identity here)
-    assert len({t for _, _, t in node.instances[:8]}) == 1
+    # the demo keeps its own instance transforms (object transform identity)
+    for (_, _, t), (_, _, dt) in zip(node.instances[:8], demo.instances):
+        np.testing.assert_array_equal(node.transforms[t].matrix, demo.transforms[dt].matrix)
     m = node.transforms[node.instances[8][2]].matrix
     # z-up (5, 6, 7) -> y-up (5, 7, -6); rotation (0, 0, 90) -> (90, 0, 0) degrees about x
     np.testing.assert_allclose(m[:3, 3], [5, 7, -6])
@@ -341,10 +342,12 @@ def test_add_scene_node_and_load_json(tmp_path):
     np.testing.assert_allclose(node.camera.position, [1, 3, -2])
     assert node.camera.fov == 45.0
     assert node.lighting.atmosphere and node.lighting.sun_direction == (0.0, 1.0, -0.0)
-    # add_scene_node directly
+    # add_scene_node (the reference's behaviour): every added instance gets the one transform
     base = SceneNode.demo()
     ids = base.add_scene_node(SceneNode.load_asset(str(tmp_path / "cube.stl")), Transform((0, 3, 0)))
     assert ids == [8] and base.instances[8][0] == len(base.meshes) - 1
+    ids = base.add_scene_node(SceneNode.demo(), Transform((0, 0, 5)))
+    assert len(ids) == 8 and len({base.instances[i][2] for i in ids}) == 1
     with pytest.raises(ValueError, match="objects"):
         (tmp_path / "bad.json").write_text("{}")
         SceneNode.load_json(str(tmp_path / "bad.json"))
