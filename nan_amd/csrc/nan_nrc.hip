@@ -78,11 +78,10 @@ struct Corners {
     float w[8];
 };
 
-__device__ __forceinline__ Corners level_corners(const NrcGrid &E, uint32_t level, const float *pos) {
-    Corners c;
+// The cell of a position in one level: lower corner and fractions.
+__device__ __forceinline__ void level_cell(const NrcGrid &E, uint32_t level, const float *pos, uint32_t (&base)[3],
+                                           float (&f)[3]) {
     float fr = (float)E.res[level];
-    float f[3];
-    uint32_t base[3];
     for (int a = 0; a < 3; ++a) {
         float u = fminf(fmaxf((pos[a] - E.bmin[a]) * E.bscale[a], 0.0f), 1.0f);
         float x = u * fr;
@@ -90,14 +89,26 @@ __device__ __forceinline__ Corners level_corners(const NrcGrid &E, uint32_t leve
         base[a] = (uint32_t)fl;
         f[a] = x - fl;
     }
+}
+
+// Corner k (bit 0: +x, bit 1: +y, bit 2: +z) of a cell: table entry (dense
+// indexing where (res+1)^3 fits, spatial hash otherwise) and trilinear weight.
+__device__ __forceinline__ void cell_corner(const NrcGrid &E, uint32_t level, const uint32_t (&base)[3],
+                                            const float (&f)[3], uint32_t k, uint32_t &idx, float &w) {
     uint32_t r1 = E.res[level] + 1u;
-    for (int k = 0; k < 8; ++k) {
-        uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
-        uint32_t x = base[0] + dx, y = base[1] + dy, z = base[2] + dz;
-        c.idx[k] = E.dense[level] ? x + r1 * (y + r1 * z)
-                                  : ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & (E.entries - 1u);
-        c.w[k] = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
-    }
+    uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    uint32_t x = base[0] + dx, y = base[1] + dy, z = base[2] + dz;
+    idx = E.dense[level] ? x + r1 * (y + r1 * z)
+                         : ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & (E.entries - 1u);
+    w = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
+}
+
+__device__ __forceinline__ Corners level_corners(const NrcGrid &E, uint32_t level, const float *pos) {
+    Corners c;
+    uint32_t base[3];
+    float f[3];
+    level_cell(E, level, pos, base, f);
+    for (uint32_t k = 0; k < 8; ++k) cell_corner(E, level, base, f, k, c.idx[k], c.w[k]);
     return c;
 }
 
@@ -161,19 +172,25 @@ __global__ void nrc_encode_fwd(NrcGrid E, const float *pos, const float *nrm, co
 // sums of many contributions inside FP16 range.
 constexpr float kTableGradScale = 1.0f / 16.0f;
 
-// One thread per (sample, level), level-major (blockIdx.y = level): the
-// atomics of a wave all go to one level's table.
+// One thread per (sample, corner) of one level (blockIdx.y = level): eight
+// consecutive lanes take the eight corners of a sample, one packed atomic
+// each.  The atomics are the cost; eight times the threads of one per
+// (sample, level) keep more of them in flight (73 -> 43 us for 8,192
+// samples).
 __global__ void nrc_encode_bwd(NrcGrid E, const float *pos, uint32_t n, const float *denc, __half2 *grad_table) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x, level = blockIdx.y;
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t level = blockIdx.y, s = (uint32_t)(gid >> 3), k = (uint32_t)(gid & 7u);
     if (s >= n) return;
     float g0 = denc[(size_t)s * kNrcIn + 2 * level], g1 = denc[(size_t)s * kNrcIn + 2 * level + 1];
     if (g0 == 0.0f && g1 == 0.0f) return;
     g0 *= kTableGradScale;
     g1 *= kTableGradScale;
-    Corners c = level_corners(E, level, pos + 3 * (size_t)s);
+    uint32_t base[3], idx;
+    float f[3], w;
+    level_cell(E, level, pos + 3 * (size_t)s, base, f);
+    cell_corner(E, level, base, f, k, idx, w);
     __half2 *g = grad_table + (size_t)level * E.entries;
-    for (int k = 0; k < 8; ++k)
-        unsafeAtomicAdd(&g[c.idx[k]], __halves2half2(__float2half(c.w[k] * g0), __float2half(c.w[k] * g1)));
+    unsafeAtomicAdd(&g[idx], __halves2half2(__float2half(w * g0), __float2half(w * g1)));
 }
 
 // ---------------------------------------------------------------------------
@@ -707,7 +724,7 @@ int nrc_train_one(NanNrc *c, const float *pos, const float *nrm, const float *al
                   const uint32_t *valid, uint32_t n, hipStream_t s) {
     if (nrc_reserve_train(c, n)) return -1;
     forward_backward(c, pos, nrm, alb, target, valid, n, s);
-    hipLaunchKernelGGL(nrc_encode_bwd, dim3(grid_for(n, 256), c->cfg.levels), dim3(256), 0, s,
+    hipLaunchKernelGGL(nrc_encode_bwd, dim3(grid_for((uint64_t)n * 8, 256), c->cfg.levels), dim3(256), 0, s,
                        grid_of(c), pos, n, c->denc, c->tg);
     uint32_t reduce_blocks = grid_for(kNrcWeights, kReduceWeights);
     c->step++;
