@@ -572,9 +572,10 @@ def bench_combined(args) -> int:
 
 
 def bench_batch(args) -> int:
-    """64 seeded scenes (1k-200k triangles, 16k-2M slot budgets, 1-8 bounces)
-    traced by one set of wavefront launches captured in a hipGraph, after the
-    autotuner picked the traversal launch shape."""
+    """64 seeded scenes (1k-200k triangles, 16k-2M slot budgets, Russian
+    roulette paths of 1-8 bounces) traced by one set of wavefront launches
+    captured in a hipGraph, after the autotuner picked the traversal launch
+    shape."""
     import numpy as np
     import torch
 
@@ -623,8 +624,9 @@ def bench_batch(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": "64-scene batch (seed 0): 1k-200k triangles and 16k-2M slot budgets log-uniform, "
-                        "1-8 bounces, one hipGraph per frame",
+                        "Russian roulette paths of 1-8 bounces, one hipGraph per frame",
             "scenes": len(nodes),
+            "russian_roulette": True,
             "triangles": int(sum(batch.triangle_counts)),
             "payload_slots": int(slots),
             "bounces": bounces,

@@ -103,6 +103,14 @@ NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16);
  * constant sky of the lighting record.  Batches copy the table of each scene at
  * nan_batch_create. */
 NAN_API int nan_scene_set_sky_view_lut(NanScene *scene, const float *sky_view_lut);
+/* Russian roulette on the scene's paths (enable != 0); no counterpart in the
+ * reference, whose paths always run to max_bounces (texture_space_path_tracer.
+ * slang:57-84).  A path that would take another bounce survives with
+ * probability min(1, max component of its throughput) and divides its
+ * throughput by it, drawing one random number; the estimate stays unbiased
+ * and paths end after 1 to max_bounces bounces.  Batches copy the flag of each
+ * scene at nan_batch_create.  The learned forms refuse such scenes. */
+NAN_API int nan_scene_set_russian_roulette(NanScene *scene, uint32_t enable);
 
 /* Closest hit and occlusion for rays float[n][8] (origin, tmin, dir, tmax) in device
  * memory; writes hits float[n][4] (t, b1, b2, 0; t < 0 on a miss), ids uint32[n][2]

@@ -95,7 +95,7 @@ EXPORTED_SYMBOLS = (
     "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned", "nan_surface_filter",
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
-    "nan_tst_learned_records",
+    "nan_tst_learned_records", "nan_scene_set_russian_roulette",
 )
 
 _LIB = None
@@ -114,6 +114,7 @@ def _declare(lib) -> None:
         "nan_scene_info": (I, [vp, ctypes.POINTER(NanSceneInfo)]),
         "nan_scene_set_lighting": (I, [vp, f32p]),
         "nan_scene_set_sky_view_lut": (I, [vp, vp]),
+        "nan_scene_set_russian_roulette": (I, [vp, ctypes.c_uint32]),
         "nan_atmosphere_transmittance_lut": (I, [vp, vp]),
         "nan_atmosphere_multiscattering_lut": (I, [vp, vp, vp]),
         "nan_atmosphere_sky_view_lut": (I, [vp, vp, f32p, vp, vp]),

@@ -131,9 +131,13 @@ class SceneBatch:
 
 
 def batch_workload(seed: int = 0, count: int = 64, *, tri_range=(1_000, 200_000),
-                   budget_range=(16_384, 2_097_152), bounce_range=(1, 8)):
+                   budget_range=(16_384, 2_097_152), bounce_range=(1, 8), roulette: bool = True):
     """Seeded many-scene workload: triangle counts and texel budgets
-    log-uniform in the given ranges, path lengths uniform in bounce_range.
+    log-uniform in the given ranges.  With ``roulette`` every scene runs
+    Russian roulette under a cap of bounce_range[1] bounces, so its paths end
+    after bounce_range[0]..bounce_range[1] bounces; without it each scene's
+    fixed path length is drawn uniformly from bounce_range (the draw is made
+    either way, so the scenes themselves do not depend on the choice).
     Per scene the layout uses texels_per_unit 8 (halved until the budget can
     hold the scene), max_resolution 16 and the largest min_resolution in
     {4, 2, 1} whose unhalved demand fits the budget.  A drawn budget below
@@ -169,7 +173,9 @@ def batch_workload(seed: int = 0, count: int = 64, *, tri_range=(1_000, 200_000)
                               max_total_texels=budget)
                     break
             tpu *= 0.5
+        drawn = int(rng.integers(bounce_range[0], bounce_range[1] + 1))
+        node.russian_roulette = roulette
         nodes.append(node)
         kws.append(kw)
-        bounces.append(int(rng.integers(bounce_range[0], bounce_range[1] + 1)))
+        bounces.append(int(bounce_range[1]) if roulette else drawn)
     return nodes, kws, bounces

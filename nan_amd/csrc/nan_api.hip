@@ -40,12 +40,14 @@ __global__ __launch_bounds__(BLOCK) void tst_megakernel(DevScene S, DevLayout Ly
         TexelSurface ts = texel_surface(Ly, w, iteration, spp, rng);
         uint32_t nspp = spp > 1u ? spp : 1u;
         f3 n = world_flat_normal(S, ts.tri);
-        f3 e = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.tri, ts.b1, ts.b2, n, nspp, max_bounces, rng, col, pc);
+        const bool roulette = S.scenes[ts.scene].roulette != 0u;
+        f3 e = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.tri, ts.b1, ts.b2, n, nspp, max_bounces, rng, col, pc,
+                                                        roulette);
         store_accumulated(payload, ts.global_texel, e, reset);
         if (COUNT) pc.slots++;
         if (ts.side.y & 1u) {
             f3 eb = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.tri, ts.b1, ts.b2, -n, nspp, max_bounces, rng, col,
-                                                             pc);
+                                                             pc, roulette);
             store_accumulated(payload, ts.side.x + ts.local, eb, reset);
             if (COUNT) pc.slots++;
         }
@@ -540,6 +542,7 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
         rec.root = s->dev.root;
         rec.slack = s->dev.slack;
         rec.max_bounces = 0;
+        rec.roulette = 0;
         rec.light = s->dev.light;
         SceneRec *drec;
         if (up(&drec, &rec, sizeof(rec))) {
@@ -577,6 +580,7 @@ static int refresh_light(NanScene *scene) {
     SceneRec rec{};
     rec.root = scene->dev.root;
     rec.slack = scene->dev.slack;
+    rec.roulette = scene->roulette;
     rec.light = scene->dev.light;
     HIP_TRY(hipMemcpy((void *)scene->dev.scenes, &rec, sizeof(rec), hipMemcpyHostToDevice), nullptr, 0);
     return 0;
@@ -585,6 +589,13 @@ static int refresh_light(NanScene *scene) {
 NAN_API int nan_scene_set_lighting(NanScene *scene, const float *lighting16) {
     if (!scene || !lighting16) return fail(nullptr, 0, "nan_scene_set_lighting: null argument");
     std::memcpy(scene->lighting, lighting16, sizeof(scene->lighting));
+    return refresh_light(scene);
+}
+
+NAN_API int nan_scene_set_russian_roulette(NanScene *scene, uint32_t enable) {
+    if (!scene) return fail(nullptr, 0, "nan_scene_set_russian_roulette: null scene");
+    if (scene->batch) return fail(nullptr, 0, "nan_scene_set_russian_roulette: set it on each scene of a batch");
+    scene->roulette = enable ? 1u : 0u;
     return refresh_light(scene);
 }
 
@@ -911,6 +922,7 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
         recs[k].root = rebase_entry(sc->dev.root);
         recs[k].slack = sc->dev.slack;
         recs[k].max_bounces = 0;
+        recs[k].roulette = sc->roulette;
         recs[k].light = sc->dev.light;
         // layout
         h_faces.insert(h_faces.end(), tr->h_faces.begin(), tr->h_faces.end());

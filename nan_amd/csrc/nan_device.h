@@ -180,7 +180,7 @@ struct SceneRec {
     uint32_t root;         // BVH root entry (rebased into the shared node array)
     float slack;           // absolute box-test slack of the scene
     uint32_t max_bounces;  // 0: use the launch's value
-    uint32_t pad;
+    uint32_t roulette;     // 1: Russian roulette on every bounce that would continue
     Light light;
 };
 

@@ -76,6 +76,7 @@ struct NanScene {
     std::vector<uint32_t> h_mats;
     bool batch = false;  // assembled from several scenes (wavefront only)
     float lighting[16] = {};            // the lighting record last set
+    uint32_t roulette = 0;              // Russian roulette on this scene's paths (1: on)
     const float4 *sky_lut = nullptr;    // sky-view table (caller-owned device memory) or null
     ~NanScene() {
         for (void *p : allocations) hipFree(p);

@@ -585,6 +585,8 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
         return fail(nullptr, 0, "nan_tst_execute_learned: scene batches are not supported");
     if (tracer->device_frame)
         return fail(nullptr, 0, "nan_tst_execute_learned: device-frame mode is not supported");
+    if (tracer->scene->roulette)
+        return fail(nullptr, 0, "nan_tst_execute_learned: Russian roulette is not supported by the learned forms");
     if (tracer->scene->stack_depth == 0)
         return fail(nullptr, 0, "nan_tst_execute_learned: BVH too deep for the per-texel kernels");
     if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_execute_learned: max_bounces must be at least 1");
@@ -620,6 +622,8 @@ NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint3
         return fail(nullptr, 0, "nan_tst_learned_records: scene batches are not supported");
     if (tracer->scene->stack_depth == 0)
         return fail(nullptr, 0, "nan_tst_learned_records: BVH too deep for the per-texel kernels");
+    if (tracer->scene->roulette)
+        return fail(nullptr, 0, "nan_tst_learned_records: Russian roulette is not supported by the learned forms");
     if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_learned_records: max_bounces must be at least 1");
     if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY))
         return fail(nullptr, 0, "nan_tst_learned_records: unknown flags");

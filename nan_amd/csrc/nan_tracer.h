@@ -31,10 +31,23 @@ __device__ __forceinline__ bool shadow_visible(const DevScene &S, f3 origin, f3 
     return !occluded;
 }
 
+// Russian roulette (an extension; the reference's paths always run to
+// max_bounces): a path that would take another bounce survives with
+// probability p = min(1, max component of its throughput) and carries T / p,
+// so the estimate stays unbiased.  One draw, taken only when the path would
+// continue, in the oracle's order.
+__device__ __forceinline__ bool roulette_survives(f3 &T, Rng &rng) {
+    float p = fminf(1.0f, fmaxf(fmaxf(T.x, T.y), T.z));
+    float u = rng.next_1d();
+    if (!(u < p)) return false;
+    T = mk(T.x / p, T.y / p, T.z / p);
+    return true;
+}
+
 // trace_radiance (texture_space_path_tracer.slang:57-84)
 template <int STACK, int BLOCK, bool COUNT>
 __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_bounces, Rng &rng, uint32_t *col,
-                             PathCounters &pc) {
+                             PathCounters &pc, bool roulette = false) {
     const Light &L = S.light;
     f3 T = mk(1.0f, 1.0f, 1.0f), rad = mk(0.0f, 0.0f, 0.0f);
     for (uint32_t bounce = 0; bounce < max_bounces; ++bounce) {
@@ -72,6 +85,7 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
         f3 f = sd.diffuse * kInvPi;
         float az = fabsf(sdir.z);
         T = T * mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf);
+        if (roulette && bounce + 1 < max_bounces && !roulette_survives(T, rng)) return rad;
         origin = sd.origin_front;
         dir = to_global(sd.frame, sdir);
     }
@@ -81,7 +95,8 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
 // estimate_surface_irradiance (texture_space_path_tracer.slang:86-117)
 template <int STACK, int BLOCK, bool COUNT>
 __device__ f3 estimate_irradiance(const DevScene &S, uint32_t tri, float b1, float b2, f3 view,
-                                  uint32_t spp, uint32_t max_bounces, Rng &rng, uint32_t *col, PathCounters &pc) {
+                                  uint32_t spp, uint32_t max_bounces, Rng &rng, uint32_t *col, PathCounters &pc,
+                                  bool roulette = false) {
     const Light &L = S.light;
     Shading sd;
     shading_data(S, tri, b1, b2, view, sd);
@@ -99,7 +114,7 @@ __device__ f3 estimate_irradiance(const DevScene &S, uint32_t tri, float b1, flo
         cosine_sample(ex, ey, sdir, pdf);
         if (pdf > 1e-6f) {
             f3 rad = trace_radiance<STACK, BLOCK, COUNT>(S, sd.origin_front, to_global(sd.frame, sdir), max_bounces,
-                                                         rng, col, pc);
+                                                         rng, col, pc, roulette);
             float az = fabsf(sdir.z);
             sample = sample + mk(rad.x * az / pdf, rad.y * az / pdf, rad.z * az / pdf);
         }

@@ -559,6 +559,7 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W,
                     origin = sd.origin_front;
                     dir = to_global(sd.frame, sdir);
                     want_ray = P.bounce < max_bounces;
+                    if (want_ray && sc.roulette) want_ray = roulette_survives(T, rng);
                 }
                 W.rng[p] = rng.state;
                 W.T[p] = make_float4(T.x, T.y, T.z, Tv.w);

@@ -61,6 +61,8 @@ class Scene:
         _native.check(lib.nan_scene_info(self.handle, ctypes.byref(self.info)), "nan_scene_info")
         self.frame_index = 0
         self._attach_sky()
+        if a["russian_roulette"]:
+            self.set_russian_roulette(True)
 
     def _apply_atmosphere(self, lighting) -> None:
         """With ``lighting.atmosphere``, (re)generates the tables for its sun
@@ -113,6 +115,17 @@ class Scene:
         _native.check(_native.lib().nan_scene_set_lighting(self.handle, _f32(self.arrays["lighting"])),
                       "nan_scene_set_lighting")
         self._attach_sky()
+
+    def set_russian_roulette(self, enable: bool) -> None:
+        """Russian roulette on this scene's texture-space paths: a path that
+        would take another bounce survives with probability min(1, max
+        throughput component) and is reweighted by its inverse.  An extension:
+        the reference's paths always run to max_bounces
+        (texture_space_path_tracer.slang:57-84)."""
+        self.scene_node.russian_roulette = bool(enable)
+        self.arrays["russian_roulette"] = bool(enable)
+        _native.check(_native.lib().nan_scene_set_russian_roulette(self.handle, 1 if enable else 0),
+                      "nan_scene_set_russian_roulette")
 
     def trace_rays(self, rays: torch.Tensor, *, occlusion: bool = True):
         """Closest hit (and occlusion) for ``rays`` float32[n, 8] on the

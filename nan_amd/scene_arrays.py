@@ -54,6 +54,7 @@ def flatten_scene(scene_node: SceneNode) -> dict:
         "inv_transposes": np.ascontiguousarray(inv_t, dtype=np.float32),
         "materials": materials,
         "lighting": scene_node.lighting.pack(),
+        "russian_roulette": bool(getattr(scene_node, "russian_roulette", False)),
     }
 
 

@@ -89,6 +89,9 @@ class SceneNode:
         self.instances: list[tuple[int, int, int]] = []
         self.lighting = Lighting()
         self.asset_path: str | None = None
+        # Russian roulette on the texture-space paths (an extension: the
+        # reference's paths always run to max_bounces)
+        self.russian_roulette = False
 
     def add_material(self, material: Material) -> int:
         self.materials.append(material)

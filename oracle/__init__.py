@@ -84,6 +84,7 @@ def liboracle():
     lib.orc_crosscheck_result.restype = ctypes.c_int
     lib.orc_crosscheck_result.argtypes = [f32p]
     lib.orc_scene_set_sky_view_lut.argtypes = [ctypes.c_void_p, f32p]
+    lib.orc_scene_set_russian_roulette.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_sky_view_lookup.argtypes = [f32p, f32p, f32p, ctypes.c_uint64, f32p]
     lib.orc_atmo_scattering.argtypes = [f32p, f32p]
     lib.orc_atmo_sphere_hit.restype = ctypes.c_float
@@ -179,6 +180,12 @@ class OracleScene:
             _p(a["lighting"], f32p),
         )
         self.handle = self.lib.orc_scene_create(ctypes.byref(self.desc), 1 if use_bvh else 0)
+        if bool(arrays.get("russian_roulette", False)):
+            self.set_russian_roulette(True)
+
+    def set_russian_roulette(self, enable: bool) -> None:
+        """Russian roulette on the scene's paths (the rule of trace_radiance)."""
+        self.lib.orc_scene_set_russian_roulette(self.handle, 1 if enable else 0)
 
     def set_sky_view_lut(self, lut) -> None:
         """Attaches a sky-view table float32[200, 200, 4] (None detaches)."""

@@ -23,7 +23,10 @@
  *     (test_mesh_colors_rgb9e5.py:15-50) over exhaustive channel sweeps;
  *   - ray/triangle: an FP64 brute-force nearest hit (orc_intersect_fp64)
  *     and tinybvh's occlusion query built from the reference tree;
- *   - shading/paths: white-furnace and closed-form emitter invariants.
+ *   - shading/paths: white-furnace and closed-form emitter invariants;
+ *   - Russian roulette (an extension): the emitting closed box, whose
+ *     samples are pi * Le * (integer hit count) with a geometric tail and
+ *     the no-roulette mean (tests/test_roulette_oracle.py).
  */
 #include <float.h>
 #include <math.h>
@@ -329,6 +332,7 @@ typedef struct {
     int use_bvh;
     const float *sky_lut; /* sky-view table [200][200][4] (caller-owned) or NULL */
     float slack;         /* absolute box-test slack, see box_hit */
+    int roulette;        /* Russian roulette on paths (extension), see trace_radiance */
 } orc_scene;
 
 static void build_world_triangles(orc_scene *s) {
@@ -474,6 +478,7 @@ ORC_API orc_scene *orc_scene_create(const orc_scene_desc *desc, int use_bvh) {
 }
 
 ORC_API void orc_scene_set_sky_view_lut(orc_scene *s, const float *sky_lut) { s->sky_lut = sky_lut; }
+ORC_API void orc_scene_set_russian_roulette(orc_scene *s, int enable) { s->roulette = enable != 0; }
 
 ORC_API void orc_scene_destroy(orc_scene *s) {
     if (!s) return;
@@ -1031,6 +1036,16 @@ static f3 trace_radiance(const orc_scene *s, const orc_light *L, f3 origin, f3 d
         f3 f = muls(sd.diffuse, ORC_INV_PI);
         float az = fabsf(sdir.z);
         T = mul(T, mk(f.x * az / pdf, f.y * az / pdf, f.z * az / pdf));
+        /* Russian roulette (no counterpart in the reference, whose paths run
+         * to max_bounces): only a path that would take another bounce draws;
+         * it survives with probability p = min(1, max component of T) and
+         * carries T / p on, which keeps the estimate unbiased. */
+        if (s->roulette && bounce + 1 < max_bounces) {
+            float p = fminf(1.0f, fmaxf(fmaxf(T.x, T.y), T.z));
+            float u = rng_1d(rng);
+            if (!(u < p)) return rad;
+            T = mk(T.x / p, T.y / p, T.z / p);
+        }
         origin = sd.origin_front;
         dir = to_global(&sd.frame, sdir);
     }

@@ -64,6 +64,21 @@ def test_batch_equals_one_at_a_time(device):
     _assert_same(batch, _single_runs(batch, 3), "eager")
 
 
+def test_batch_with_russian_roulette_equals_one_at_a_time(device):
+    from nan_amd.batch import SceneBatch
+
+    _say("eager batch with russian roulette on three of five scenes vs single scenes")
+    nodes, kws, bounces = _small_batch()
+    for k in (0, 1, 3):
+        nodes[k].russian_roulette = True
+    bounces = [8, 8, 3, 6, 2]
+    batch = SceneBatch(device, nodes, kws, bounces)
+    enc = device.create_command_encoder()
+    for it in range(2):
+        batch.execute(enc, it, reset=(it == 0))
+    _assert_same(batch, _single_runs(batch, 2), "roulette")
+
+
 def test_batch_graph_replays_equal_eager(device):
     from nan_amd.batch import SceneBatch
 

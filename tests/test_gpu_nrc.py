@@ -289,6 +289,29 @@ def test_texel_query_form_with_zero_cache_traces_no_bounce(device):
     assert (v[:, :3].max(axis=1) > 0.0).mean() > 0.3  # the lit part of the scene
 
 
+def test_learned_forms_refuse_russian_roulette_scenes(device):
+    """The learned forms' suffix paths have no Russian roulette: a scene with
+    it on is refused with an error, and leaves the output untouched."""
+    from nan_amd import _native
+    from nan_amd.radiance_cache import RadianceCache
+    from nan_amd.scene import Scene
+    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
+    _say("learned forms refuse roulette scenes")
+    node, layout = _small_sunlit()
+    node.russian_roulette = True
+    sc = Scene(device, node)
+    tracer = TextureSpacePathTracer(device, sc, layout, samples_per_texel=1, max_bounces=3, mode="megakernel")
+    cache = RadianceCache.for_scene(device, sc)
+    out = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=device.torch_device)
+    for texel_query in (False, True):
+        with pytest.raises(_native.NanError, match="Russian roulette"):
+            tracer.execute_learned(device.create_command_encoder(), out, 0, cache, reset=True, train_samples=0,
+                                   texel_query=texel_query)
+    torch.cuda.synchronize()
+    assert not out.any()
+
+
 def test_texel_query_form_converges_faster_than_plain(device):
     """On a sunlit scene the texel-query form (cache queried at the texel, no
     bounce ray) stays unbiased in the mean (within 3% of a 256-frame plain

@@ -230,6 +230,36 @@ def test_tracer_sunlit_scene_bit_exact(device, nan, which, mode):
     assert (c[0], c[1], c[4]) == (cnt[0], cnt[1], cnt[3])
 
 
+@pytest.mark.parametrize("mode", ["wavefront", "megakernel"])
+def test_tracer_russian_roulette_bit_exact(device, nan, mode):
+    # roulette on a sunlit scene with paths of up to 6 bounces: the HIP path
+    # and the oracle draw the same survival numbers and end the same paths
+    _say(f"tracer russian roulette, 6 bounces ({mode})")
+    node = batch_scene(23, 5000)
+    node.russian_roulette = True
+    kw = dict(texels_per_unit=6.0, min_resolution=1, max_resolution=8, max_total_texels=2_000_000)
+    scene_mod, tst_mod, _ = nan
+    layout = MeshColorsLayout.build(node, **kw)
+    sc = scene_mod.Scene(device, node)
+    tracer = tst_mod.TextureSpacePathTracer(device, sc, layout, mode=mode, max_bounces=6)
+    dev = device.torch_device
+    payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=dev)
+    counters = torch.zeros(8, dtype=torch.int64, device=dev)
+    enc = device.create_command_encoder()
+    for it in range(2):
+        tracer.execute(enc, payload, it, reset=(it == 0), counters=counters if it == 1 else None)
+    got = payload.cpu().numpy()
+    ref, _ = _oracle_payload(node, layout, 1, bounces=6)
+    cnt = oracle.OracleScene(flatten_scene(node)).trace_texels(
+        layout.face_table, layout.instance_table, layout.side_table, ref, 1, 1, 6, False, threads=oracle.cpu_count())
+    diff = np.any(got.view(np.uint32) != ref.view(np.uint32), axis=1)
+    assert diff.sum() == 0, f"{diff.sum()} of {len(diff)} slots differ"
+    c = counters.cpu().numpy()
+    assert (c[0], c[1], c[4]) == (cnt[0], cnt[1], cnt[3])
+    # roulette ends paths early: fewer extension rays than 6 per path
+    assert c[0] < 6 * c[4]
+
+
 def test_wavefront_two_samples_per_texel_bit_exact(device, nan):
     _say("tracer spp=2 wavefront vs megakernel vs oracle")
     node = batch_scene(22, 4000)
