@@ -575,6 +575,25 @@ NAN_API int nan_scene_info(const NanScene *scene, NanSceneInfo *out) {
     return 0;
 }
 
+// The front texels in [0, total_front) that `ds` (ascending) does not hold:
+// the paths of the single-sided instances, which the wavefront traces apart
+// from the double-sided ones.
+static int upload_single_sided(NanTracer *t, const std::vector<uint32_t> &ds, char *err, size_t cap) {
+    std::vector<uint32_t> ss;
+    ss.reserve((size_t)t->total_front - ds.size());
+    size_t j = 0;
+    for (uint64_t w = 0; w < t->total_front; ++w) {
+        if (j < ds.size() && ds[j] == w) {
+            ++j;
+            continue;
+        }
+        ss.push_back((uint32_t)w);
+    }
+    if (j != ds.size()) return fail(err, cap, "double-sided texel list is not ascending");
+    t->ss_count = (uint32_t)ss.size();
+    return upload(&t->ss_list, ss.data(), ss.size() * 4, err, cap);
+}
+
 static int refresh_light(NanScene *scene) {
     scene->dev.light = make_light(scene->lighting, scene->sky_lut);
     SceneRec rec{};
@@ -690,6 +709,7 @@ NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint3
             for (uint32_t k = 0; k < instance_table[4 * (size_t)i + 3]; ++k) ds.push_back(prefix[i] + k);
     t->ds_count = (uint32_t)ds.size();
     rc |= upload(&t->ds_list, ds.data(), ds.size() * 4, err, cap);
+    rc |= upload_single_sided(t, ds, err, cap);
     t->h_faces.assign(face_table, face_table + (size_t)face_count * 2);
     t->h_instances.assign(instance_table, instance_table + (size_t)instance_count * 4);
     t->h_sides.assign(side_table, side_table + (size_t)instance_count * 2);
@@ -1003,6 +1023,7 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
     rc |= upload(&t->face_inst, h_fi.data(), h_fi.size() * 4, err, cap);
     rc |= upload(&t->front_prefix, h_prefix.data(), h_prefix.size() * 4, err, cap);
     rc |= upload(&t->ds_list, h_ds.data(), h_ds.size() * 4, err, cap);
+    rc |= upload_single_sided(t, h_ds, err, cap);
     rc |= upload(&t->face_scene, h_face_scene.data(), h_face_scene.size() * 4, err, cap);
     rc |= upload(&t->scene_inst_base, h_inst_base.data(), h_inst_base.size() * 4, err, cap);
     rc |= upload(&t->payload_table, nullptr, (size_t)count * sizeof(float4 *), err, cap);

@@ -93,6 +93,8 @@ struct NanTracer {
     uint32_t *front_prefix = nullptr;
     uint32_t *ds_list = nullptr;      // front texels of double-sided instances
     uint32_t ds_count = 0;
+    uint32_t *ss_list = nullptr;      // the other front texels (ascending)
+    uint32_t ss_count = 0;
     // batches of scenes (nan_batch_create)
     uint32_t scene_count = 1;
     uint32_t *face_scene = nullptr;
@@ -132,6 +134,7 @@ inline NanTracer::~NanTracer() {
     delete owned_scene;
     {
         hipFree(ds_list);
+        hipFree(ss_list);
         hipFree(faces);
         hipFree(instances);
         hipFree(sides);

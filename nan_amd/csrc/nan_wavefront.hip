@@ -47,6 +47,13 @@ constexpr uint32_t kCounterSlots = 1024;
 constexpr uint32_t kParts = 8;       // queue partitions, one counter each
 constexpr uint32_t kCtrStride = 16;  // uint32 words between counters (one 64-byte line each)
 constexpr uint32_t kCounterWords = kCounterSlots * kParts * kCtrStride;
+// The double-sided texels' sweeps run on a second stream with grids of
+// 1/kSideDiv of the main ones, so their short, latency-bound launches hold
+// few CUs while the single-sided sweep fills the rest (0: one stream).
+#ifndef NAN_WF_SIDE_DIV
+#define NAN_WF_SIDE_DIV 4
+#endif
+constexpr uint32_t kSideDiv = NAN_WF_SIDE_DIV;
 
 }  // namespace
 
@@ -76,6 +83,13 @@ struct WfBuffers {
     uint32_t gen_blocks = 0;        // grids of the stage kernels: resident capacity (CUs x occupancy),
     uint32_t shade_blocks = 0;      // so a grid-stride loop never waits for a second round of blocks
     uint32_t finish_blocks = 0;
+    // Host only.  The double-sided texels' paths (front sweep, then back
+    // sweep) run on a second stream beside the single-sided texels' front
+    // sweep: `ds` is their queue set (own queues and spill area, this set's
+    // per-path arrays and counters).  Null when the tracer has no such split.
+    WfBuffers *ds = nullptr;
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -856,6 +870,21 @@ namespace nanx {
 void wavefront_release(NanTracer *t) {
     WfBuffers *W = t->wf;
     if (!W) return;
+    if (WfBuffers *D = W->ds) {
+        hipFree(D->ray_o[0]);
+        hipFree(D->ray_o[1]);
+        hipFree(D->ray_d[0]);
+        hipFree(D->ray_d[1]);
+        hipFree(D->hits);
+        hipFree(D->sh_o);
+        hipFree(D->sh_d);
+        hipFree(D->sh_c);
+        hipFree(D->spill);
+        delete D;
+    }
+    if (W->ev_fork) hipEventDestroy(W->ev_fork);
+    if (W->ev_join) hipEventDestroy(W->ev_join);
+    if (W->side_stream) hipStreamDestroy(W->side_stream);
     hipFree(W->rng);
     hipFree(W->T);
     hipFree(W->rad);
@@ -891,6 +920,7 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     if (t->wf && (t->wf->ring != tu.ring || t->wf->blocks_per_cu_req != tu.blocks_per_cu)) wavefront_release(t);
     if (t->wf) {
         if (spp > 1 && !t->wf->acc) HIP_TRY(hipMalloc((void **)&t->wf->acc, (size_t)t->wf->capacity * 16), nullptr, 0);
+        if (t->wf->ds) t->wf->ds->acc = t->wf->acc;
         return 0;
     }
     WfBuffers *W = new WfBuffers();
@@ -939,6 +969,37 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     W->finish_blocks = (uint32_t)(cus * std::max(1, of));
     W->spill_lanes = W->trav_blocks * kTravBlock;
     HIP_TRY(hipMalloc((void **)&W->spill, (size_t)(kMaxStack - W->ring) * W->spill_lanes * 4), nullptr, 0);
+    if (kSideDiv != 0 && t->ds_count != 0 && t->ss_count != 0) {
+        WfBuffers *D = new WfBuffers(*W);
+        W->ds = D;
+        D->ds = nullptr;
+        D->side_stream = nullptr;
+        D->ev_fork = D->ev_join = nullptr;
+        D->ray_o[0] = D->ray_o[1] = D->ray_d[0] = D->ray_d[1] = nullptr;
+        D->hits = D->sh_o = D->sh_d = D->sh_c = nullptr;
+        D->spill = nullptr;
+        D->qidx = nullptr;
+        auto share = [](uint32_t blocks) { return std::max<uint32_t>(kParts, blocks / kSideDiv / kParts * kParts); };
+        D->trav_blocks = share(W->trav_blocks);
+        D->gen_blocks = share(W->gen_blocks);
+        D->shade_blocks = share(W->shade_blocks);
+        D->finish_blocks = std::max<uint32_t>(1u, W->finish_blocks / kSideDiv);
+        D->spill_lanes = D->trav_blocks * kTravBlock;
+        D->qcap = (uint32_t)(((t->ds_count + kParts - 1) / kParts + 256 + 255) / 256 * 256);
+        size_t d4 = (size_t)D->qcap * kParts * 16;
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(hipMalloc((void **)&D->ray_o[k], d4), nullptr, 0);
+            HIP_TRY(hipMalloc((void **)&D->ray_d[k], d4), nullptr, 0);
+        }
+        HIP_TRY(hipMalloc((void **)&D->hits, d4), nullptr, 0);
+        HIP_TRY(hipMalloc((void **)&D->sh_o, 2 * d4), nullptr, 0);
+        HIP_TRY(hipMalloc((void **)&D->sh_d, 2 * d4), nullptr, 0);
+        HIP_TRY(hipMalloc((void **)&D->sh_c, 2 * d4), nullptr, 0);
+        HIP_TRY(hipMalloc((void **)&D->spill, (size_t)(kMaxStack - D->ring) * D->spill_lanes * 4), nullptr, 0);
+        HIP_TRY(hipStreamCreateWithFlags(&W->side_stream, hipStreamNonBlocking), nullptr, 0);
+        HIP_TRY(hipEventCreateWithFlags(&W->ev_fork, hipEventDisableTiming), nullptr, 0);
+        HIP_TRY(hipEventCreateWithFlags(&W->ev_join, hipEventDisableTiming), nullptr, 0);
+    }
     return 0;
 }
 
@@ -974,6 +1035,80 @@ static void launch_shadow(const WfBuffers &W, const DevScene &S, uint32_t count_
                            fetch, steps, counters);
 }
 
+// The sweeps of one set of paths (`list` null: front texel i is path i) on
+// one queue set and stream: per sample, generate, `bounces` rounds of extend /
+// shade / shadow, finish.  Counter slots come from [slot, slot_end).
+static int run_sweeps(NanTracer *t, WfBuffers &W, float4 *payload, const uint32_t *list, uint32_t count,
+                      uint32_t side, uint32_t iteration, uint32_t spp, uint32_t bounces, uint32_t reset,
+                      unsigned long long *counters, hipStream_t stream, uint32_t &slot, uint32_t slot_end) {
+    if (count == 0) return 0;
+    const DevScene &S = t->scene->dev;
+    DevLayout Ly = dev_layout(t);
+    uint32_t nspp = spp > 1 ? spp : 1;
+    uint32_t steps = std::max(1u, t->tuning.refill_steps);
+    uint32_t round8 = (grid_for(count, kWfBlock) + kParts - 1) / kParts * kParts;
+    uint32_t gen_blocks = std::min<uint32_t>(W.gen_blocks, round8);
+    uint32_t shade_blocks = std::min<uint32_t>(W.shade_blocks, round8);
+    uint32_t finish_blocks = std::min<uint32_t>(W.finish_blocks, grid_for(count, kWfBlock));
+    for (uint32_t k = 0; k < nspp; ++k) {
+        if (slot + 4 * bounces + 8 > slot_end) return fail(nullptr, 0, "wavefront: too many stages");
+        uint32_t ext_slot = slot++, sh_slot = slot++;
+        GenParams gp{t->device_frame ? W.frame : nullptr, list, count, side, k, iteration, spp, ext_slot, sh_slot};
+        {
+            StageTimer tm(t, stream, kStageGenerate);
+            hipLaunchKernelGGL(wf_generate, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, Ly, W, gp);
+        }
+        uint32_t q = 0;
+        for (uint32_t b = 1; b <= bounces; ++b) {
+            uint32_t fetch = slot++;
+            {
+                StageTimer tm(t, stream, kStageExtend);
+                if (W.ring == 16) launch_extend<16>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
+                else if (W.ring == 24) launch_extend<24>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
+                else launch_extend<32>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
+            }
+            uint32_t next_ext = slot++;
+            ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces};
+            {
+                StageTimer tm(t, stream, kStageShade);
+                hipLaunchKernelGGL(wf_shade, dim3(shade_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
+            }
+            uint32_t sfetch = slot++;
+            {
+                StageTimer tm(t, stream, kStageShadow);
+                // at most one sun ray per path from generate (bounce 1) and one from shade
+                uint64_t max_sh = 2ull * count;
+                if (W.ring == 16) launch_shadow<16>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
+                else if (W.ring == 24) launch_shadow<24>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
+                else launch_shadow<32>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
+            }
+            sh_slot = slot++;
+            ext_slot = next_ext;
+            q = 1u - q;
+        }
+        FinishParams fp{t->device_frame ? W.frame : nullptr, list, count, side, k, nspp, reset,
+                        t->scene->batch ? t->payload_table : nullptr};
+        StageTimer tm(t, stream, kStageFinish);
+        if (counters)
+            hipLaunchKernelGGL(wf_finish<true>, dim3(finish_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
+                               counters);
+        else
+            hipLaunchKernelGGL(wf_finish<false>, dim3(finish_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
+                               counters);
+    }
+    return 0;
+}
+
+// One frame.  A double-sided texel's back path starts from the random state
+// its front path ended with (texture_space_path_tracer.slang:204-214), so its
+// back sweep waits for its front sweep; the back sweep is small and made of
+// short, latency-bound launches.  With both kinds of texels present the
+// double-sided texels' two sweeps therefore run on a second stream, forked
+// from and joined back into the caller's stream, beside the single-sided
+// texels' front sweep, which fills the GPU meanwhile.  Every path touches
+// only its own per-path entries and payload slots, and each stream has its
+// own queues, spill area and counter slots, so the results are those of the
+// one-stream order.  Stage timing (profiling) uses the one-stream order.
 int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                       uint32_t reset, unsigned long long *counters, hipStream_t stream) {
     if (t->scene->info.max_stack > (uint32_t)kMaxStack)
@@ -983,66 +1118,37 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
         return fail(nullptr, 0, "wavefront: batch payloads are not set (nan_batch_set_payloads)");
     if (wavefront_alloc(t, spp)) return -1;
     WfBuffers &W = *t->wf;
-    const DevScene &S = t->scene->dev;
-    DevLayout Ly = dev_layout(t);
     uint32_t nspp = spp > 1 ? spp : 1;
-    uint32_t steps = std::max(1u, t->tuning.refill_steps);
-    uint32_t slot = 0;  // next free counter slot
+    uint64_t per_sweep = (uint64_t)nspp * (4ull * bounces + 2);
+    bool split = W.ds != nullptr && !t->profile && 2 * per_sweep + 8 <= kCounterSlots / 2;
     hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream, W.counters, kCounterWords);
-    for (uint32_t side = 0; side < 2; ++side) {
-        uint32_t count = side ? t->ds_count : (uint32_t)t->total_front;
-        const uint32_t *list = side ? t->ds_list : nullptr;
-        if (count == 0) continue;
-        uint32_t round8 = (grid_for(count, kWfBlock) + kParts - 1) / kParts * kParts;
-        uint32_t gen_blocks = std::min<uint32_t>(W.gen_blocks, round8);
-        uint32_t shade_blocks = std::min<uint32_t>(W.shade_blocks, round8);
-        uint32_t finish_blocks = std::min<uint32_t>(W.finish_blocks, grid_for(count, kWfBlock));
-        for (uint32_t k = 0; k < nspp; ++k) {
-            if (slot + 4 * bounces + 8 > kCounterSlots) return fail(nullptr, 0, "wavefront: too many stages");
-            uint32_t ext_slot = slot++, sh_slot = slot++;
-            GenParams gp{t->device_frame ? W.frame : nullptr, list, count, side, k, iteration, spp, ext_slot, sh_slot};
-            {
-                StageTimer tm(t, stream, kStageGenerate);
-                hipLaunchKernelGGL(wf_generate, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, Ly, W, gp);
-            }
-            uint32_t q = 0;
-            for (uint32_t b = 1; b <= bounces; ++b) {
-                uint32_t fetch = slot++;
-                {
-                    StageTimer tm(t, stream, kStageExtend);
-                    if (W.ring == 16) launch_extend<16>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
-                    else if (W.ring == 24) launch_extend<24>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
-                    else launch_extend<32>(W, S, q, ext_slot, fetch, steps, counters, stream, count);
-                }
-                uint32_t next_ext = slot++;
-                ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces};
-                {
-                    StageTimer tm(t, stream, kStageShade);
-                    hipLaunchKernelGGL(wf_shade, dim3(shade_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
-                }
-                uint32_t sfetch = slot++;
-                {
-                    StageTimer tm(t, stream, kStageShadow);
-                    // at most one sun ray per path from generate (bounce 1) and one from shade
-                    uint64_t max_sh = 2ull * count;
-                    if (W.ring == 16) launch_shadow<16>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
-                    else if (W.ring == 24) launch_shadow<24>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
-                    else launch_shadow<32>(W, S, sh_slot, sfetch, steps, counters, stream, max_sh);
-                }
-                sh_slot = slot++;
-                ext_slot = next_ext;
-                q = 1u - q;
-            }
-            FinishParams fp{t->device_frame ? W.frame : nullptr, list, count, side, k, nspp, reset,
-                            t->scene->batch ? t->payload_table : nullptr};
-            StageTimer tm(t, stream, kStageFinish);
-            if (counters)
-                hipLaunchKernelGGL(wf_finish<true>, dim3(finish_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
-                                   counters);
-            else
-                hipLaunchKernelGGL(wf_finish<false>, dim3(finish_blocks), dim3(kWfBlock), 0, stream, Ly, W, payload, fp,
-                                   counters);
-        }
+    if (!split) {
+        uint32_t slot = 0;
+        if (run_sweeps(t, W, payload, nullptr, (uint32_t)t->total_front, 0, iteration, spp, bounces, reset,
+                       counters, stream, slot, kCounterSlots) ||
+            run_sweeps(t, W, payload, t->ds_list, t->ds_count, 1, iteration, spp, bounces, reset, counters, stream,
+                       slot, kCounterSlots))
+            return -1;
+    } else {
+        WfBuffers &D = *W.ds;
+        hipStream_t side_stream = W.side_stream;
+        HIP_TRY(hipEventRecord(W.ev_fork, stream), nullptr, 0);
+        HIP_TRY(hipStreamWaitEvent(side_stream, W.ev_fork, 0), nullptr, 0);
+        uint32_t dslot = kCounterSlots / 2, sslot = 0;
+        int rc = run_sweeps(t, D, payload, t->ds_list, t->ds_count, 0, iteration, spp, bounces, reset, counters,
+                            side_stream, dslot, kCounterSlots);
+        if (!rc)
+            rc = run_sweeps(t, D, payload, t->ds_list, t->ds_count, 1, iteration, spp, bounces, reset, counters,
+                            side_stream, dslot, kCounterSlots);
+        if (!rc)
+            rc = run_sweeps(t, W, payload, t->ss_list, t->ss_count, 0, iteration, spp, bounces, reset, counters,
+                            stream, sslot, kCounterSlots / 2);
+        // the join is recorded whatever happened, so a capture never ends with the side stream forked
+        hipError_t e1 = hipEventRecord(W.ev_join, side_stream);
+        hipError_t e2 = hipStreamWaitEvent(stream, W.ev_join, 0);
+        if (rc) return -1;
+        HIP_TRY(e1, nullptr, 0);
+        HIP_TRY(e2, nullptr, 0);
     }
     if (t->device_frame) hipLaunchKernelGGL(wf_advance_frame, dim3(1), dim3(64), 0, stream, W.frame);
     return check_launch("wavefront");

@@ -277,6 +277,33 @@ def test_wavefront_two_samples_per_texel_bit_exact(device, nan):
     np.testing.assert_array_equal(outs[0].view(np.uint32), ref.view(np.uint32))
 
 
+def test_two_stream_sweeps_equal_one_stream_order(device, nan):
+    # the double-sided texels' sweeps run on a second stream unless stage
+    # timing is on; both orders leave the same bits and trace the same rays
+    # (node visits and triangle tests depend on which rays share a wave, as
+    # for any launch shape, so they are not compared)
+    _say("wavefront two-stream vs one-stream order")
+    node = synthetic_scene(3, terrain_quads=48, box_grid=8, spheres=24, extent=20.0)
+    kw = dict(texels_per_unit=8.0, min_resolution=2, max_resolution=8, max_total_texels=4_000_000)
+    layout, sc, tracer = _make(device, node, nan, **kw)
+    assert layout.total_payload_count > layout.total_surface_texels  # some double-sided instances
+    dev = device.torch_device
+    outs, counts = [], []
+    for profiled in (False, True):
+        tracer.profile(profiled)
+        payload = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=dev)
+        counters = torch.zeros(8, dtype=torch.int64, device=dev)
+        enc = device.create_command_encoder()
+        for it in range(3):
+            tracer.execute(enc, payload, it, reset=(it == 0), counters=counters if it == 2 else None)
+        outs.append(payload.cpu().numpy())
+        counts.append(counters.cpu().numpy())
+    tracer.profile(False)
+    np.testing.assert_array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    for k in (0, 1, 4):  # extend rays, shadow rays, slot updates
+        assert counts[0][k] == counts[1][k], (k, counts[0][k], counts[1][k])
+
+
 def test_stage_timing_reports_every_stage(device, nan):
     node = cornell_box()
     layout, sc, tracer = _make(device, node, nan)
