@@ -135,7 +135,12 @@ NAN_API int nan_tracer_tables(const NanTracer *tracer, const void **face_table, 
 /* Kernel organisation of nan_tst_execute; both give bit-identical payloads.
  * NAN_TRACER_WAVEFRONT (default): staged ray-gen / extend / shade / shadow /
  * finish kernels with persistent traversal; NAN_TRACER_MEGAKERNEL: one thread
- * per texel runs the whole path (the reference's dispatch shape). */
+ * per texel runs the whole path (the reference's dispatch shape).  When at
+ * most 1/8 of the front texels are double-sided, the wavefront traces those
+ * texels' paths on a second stream of the tracer's own, forked from the
+ * caller's stream by an event and joined back before the call returns (also
+ * under graph capture); work queued afterwards on the caller's stream sees
+ * the whole frame.  Stage timing (nan_tracer_profile) keeps one stream. */
 enum { NAN_TRACER_MEGAKERNEL = 0, NAN_TRACER_WAVEFRONT = 1 };
 NAN_API int nan_tracer_set_mode(NanTracer *tracer, int mode);
 
