@@ -54,6 +54,7 @@ constexpr uint32_t kCounterWords = kCounterSlots * kParts * kCtrStride;
 #define NAN_WF_SIDE_DIV 4
 #endif
 constexpr uint32_t kSideDiv = NAN_WF_SIDE_DIV;
+constexpr uint64_t kSideLimit = 8;  // split when at most 1/kSideLimit of front texels are double-sided
 
 }  // namespace
 
@@ -971,10 +972,11 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     HIP_TRY(hipMalloc((void **)&W->spill, (size_t)(kMaxStack - W->ring) * W->spill_lanes * 4), nullptr, 0);
     // The side stream carries two sweeps of the double-sided texels on a
     // quarter of the GPU: with at most 1/8 of the front texels double-sided
-    // its work stays within its share.  More (26% in the 4.0M-triangle
-    // combined scene, where a quarter-width side stream took 30.2 ms per
-    // frame against 23.8 ms) runs in one-stream order.
-    if (kSideDiv != 0 && t->ds_count != 0 && t->ss_count != 0 && (uint64_t)t->ds_count * 8 <= t->total_front) {
+    // its work stays within its share.  More runs in one-stream order: with
+    // 26% (the 4.0M-triangle combined scene) a split frame took 30.2 ms with
+    // quarter-width, 27.2 ms with third-width and 24.9 ms with half-width
+    // side grids, against 23.8 ms in one-stream order.
+    if (kSideDiv != 0 && t->ds_count != 0 && t->ss_count != 0 && (uint64_t)t->ds_count * kSideLimit <= t->total_front) {
         WfBuffers *D = new WfBuffers(*W);
         W->ds = D;
         D->ds = nullptr;
