@@ -121,7 +121,10 @@ NAN_API int nan_trace_rays(NanScene *scene, const float *rays, uint64_t n, float
 
 /* Uploads a Mesh Colors layout (mesh_colors.py:299-337: face uint2, instance uint4,
  * side uint2 tables) and creates a tracer over scene (TextureSpacePathTracer.__init__,
- * texture_space_path_tracer.py:8-31). */
+ * texture_space_path_tracer.py:8-31).  The wavefront working set (per-path state,
+ * queues, counters, spill area, second stream) is allocated here, sized from the
+ * layout, so the tracer's first nan_tst_execute allocates nothing and may be
+ * captured into a graph. */
 NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint32_t face_count,
                               const uint32_t *instance_table, const uint32_t *side_table,
                               uint32_t instance_count, uint64_t total_surface_texels,
@@ -161,7 +164,10 @@ NAN_API int nan_tracer_stage_times(NanTracer *tracer, double *ms, uint32_t *laun
 /* Launch shape of the wavefront traversal kernels: persistent blocks per CU
  * (0 = occupancy limit), LDS stack ring entries per lane (16, 24 or 32; deeper
  * entries spill to HBM) and traversal steps between per-lane ray refills.
- * Results do not depend on it.  No reference counterpart (autotuner hook). */
+ * Results do not depend on it.  A new launch shape resizes the tracer's
+ * working set here (allocation, device synchronisation), so this is not
+ * called during graph capture; execute calls never allocate.  No reference
+ * counterpart (autotuner hook). */
 NAN_API int nan_tracer_tune(NanTracer *tracer, uint32_t blocks_per_cu, uint32_t lds_stack, uint32_t refill_steps);
 
 /* Many independent scenes traced by one set of launches: concatenates the
@@ -188,7 +194,9 @@ NAN_API int nan_tracer_device_frame(NanTracer *tracer, int enable, uint32_t iter
 /* One progressive iteration over every texel of every instance
  * (TextureSpacePathTracer.execute, texture_space_path_tracer.py:33-59; kernel
  * texture_space_path_tracer.slang:140-216).  payload: 16-byte slots
- * {f32 r,g,b; u32 count}[total_payload_count] in device memory. */
+ * {f32 r,g,b; u32 count}[total_payload_count] in device memory.  Any
+ * samples_per_texel; in wavefront mode max_bounces up to 127 (checked before
+ * the first launch, so a refused call leaves the payload untouched). */
 NAN_API int nan_tst_execute(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,
                             uint32_t max_bounces, uint32_t reset, void *stream);
 /* Same, adding per-launch work counters (NAN_COUNTER_*) into counters
@@ -340,6 +348,15 @@ enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2, NAN_LEARNED_TEX
 NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payload, uint32_t iteration,
                                     uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
                                     uint32_t train_samples, uint32_t flags, uint64_t *counters, void *stream);
+/* Sizes the learned forms' working set (training records, cache queries and
+ * the network's batch scratch) for train_samples training paths and
+ * samples_per_texel samples per texel; growing them synchronises the device,
+ * so this must not be called during graph capture.  nan_tst_execute_learned grows the working
+ * set itself when called outside a capture; inside one it allocates nothing
+ * and fails if the working set is too small, so a caller capturing a first
+ * learned iteration reserves it here beforehand.  No reference counterpart. */
+NAN_API int nan_tst_learned_reserve(NanTracer *tracer, NanNrc *nrc, uint32_t samples_per_texel,
+                                    uint32_t train_samples);
 /* The training records of one iteration of nan_tst_execute_learned (n plain
  * paths from uniformly drawn texels): features pos/nrm/alb float[n][3], target
  * float[n][3], valid uint32[n], device memory; built by the wavefront queues

@@ -95,7 +95,7 @@ EXPORTED_SYMBOLS = (
     "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned", "nan_surface_filter",
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
-    "nan_tst_learned_records", "nan_scene_set_russian_roulette",
+    "nan_tst_learned_records", "nan_tst_learned_reserve", "nan_scene_set_russian_roulette",
 )
 
 _LIB = None
@@ -156,6 +156,7 @@ def _declare(lib) -> None:
         "nan_nrc_stats": (I, [vp, f32p]),
         "nan_nrc_time_forward": (I, [vp, vp, vp, vp, U32, U32, f32p, vp]),
         "nan_tst_execute_learned": (I, [vp, vp, vp, U32, U32, U32, U32, U32, U32, vp, vp]),
+        "nan_tst_learned_reserve": (I, [vp, vp, U32, U32]),
         "nan_surface_filter": (I, [vp, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp]),
     }
     for name, (res, args) in sig.items():

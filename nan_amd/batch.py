@@ -118,11 +118,14 @@ class SceneBatch:
     def clear_device_frame(self) -> None:
         _native.check(_native.lib().nan_tracer_device_frame(self.handle, 0, 0, 0), "nan_tracer_device_frame")
 
-    def capture(self, command_encoder: CommandEncoder, next_iteration: int) -> "torch.cuda.CUDAGraph":
+    def capture(self, command_encoder: CommandEncoder, next_iteration: int,
+                reset: bool = False) -> "torch.cuda.CUDAGraph":
         """Captures one batched frame into a hipGraph.  Replays run iterations
-        next_iteration, next_iteration + 1, ...  (needs one eager frame first,
-        so that the working buffers exist)."""
-        self.set_device_frame(next_iteration, False)
+        next_iteration, next_iteration + 1, ..., the first of them resetting
+        the caches when ``reset``.  The working buffers exist from the
+        batch's creation (and from the last ``tune``), so this may be the
+        batch's first frame."""
+        self.set_device_frame(next_iteration, reset)
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             self.execute(command_encoder, 0)

@@ -717,6 +717,7 @@ NAN_API int nan_tracer_create(NanScene *scene, const uint32_t *face_table, uint3
     t->h_face_inst = std::move(face_inst);
     t->h_prefix = std::move(prefix);
     t->h_ds = std::move(ds);
+    if (!rc && wavefront_prepare(t)) rc = fail(err, cap, "nan_tracer_create: %s", g_last_error.c_str());
     if (rc) {
         delete t;
         return -1;
@@ -870,7 +871,8 @@ NAN_API int nan_tracer_tune(NanTracer *tracer, uint32_t blocks_per_cu, uint32_t 
     tracer->tuning.blocks_per_cu = blocks_per_cu;
     tracer->tuning.ring = lds_stack;
     tracer->tuning.refill_steps = refill_steps;
-    return 0;
+    // a new launch shape resizes the working set here, not in the next execute
+    return wavefront_prepare(tracer);
 }
 
 NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers, uint32_t count, NanTracer **out,
@@ -1027,6 +1029,7 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
     rc |= upload(&t->face_scene, h_face_scene.data(), h_face_scene.size() * 4, err, cap);
     rc |= upload(&t->scene_inst_base, h_inst_base.data(), h_inst_base.size() * 4, err, cap);
     rc |= upload(&t->payload_table, nullptr, (size_t)count * sizeof(float4 *), err, cap);
+    if (!rc && wavefront_prepare(t)) rc = fail(err, cap, "nan_batch_create: %s", g_last_error.c_str());
     if (rc) {
         delete t;
         return -1;

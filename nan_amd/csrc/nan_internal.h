@@ -50,6 +50,17 @@ inline int upload(T **dst, const void *src, size_t bytes, char *err, size_t cap)
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// True while `s` is being captured into a graph, when nothing may allocate
+// or synchronise the device.  A failed query counts as capturing.
+inline bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
 constexpr int kTraceBlock = 128;
 constexpr int kFlatBlock = 256;
 
@@ -108,8 +119,8 @@ struct NanTracer {
     uint32_t face_count = 0, instance_count = 0;
     uint64_t total_front = 0, total_payload = 0;
     int mode = 1;                     // 0 megakernel, 1 wavefront
-    struct WfBuffers *wf = nullptr;   // wavefront working set, allocated on first use
-    struct LearnedBuffers *learned = nullptr;  // learned-cache working set, allocated on first use
+    struct WfBuffers *wf = nullptr;   // wavefront working set (wavefront_prepare)
+    struct LearnedBuffers *learned = nullptr;  // learned-cache working set (nan_tst_learned_reserve)
     nanx::WfTuning tuning;
     // stage timing: (stage, start, end) event triples recorded while profiling
     bool profile = false;
@@ -226,7 +237,10 @@ inline int check_launch(const char *what) {
     return 0;
 }
 
-// Wavefront tracer (nan_wavefront.hip).
+// Wavefront tracer (nan_wavefront.hip).  wavefront_prepare allocates the
+// working set for the tracer's layout and launch shape (again after a launch
+// shape change); the execute paths only use it.
+int wavefront_prepare(NanTracer *t);
 int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                       uint32_t reset, unsigned long long *counters, hipStream_t stream);
 

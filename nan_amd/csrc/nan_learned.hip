@@ -429,11 +429,24 @@ int grow(T **p, size_t bytes) {
     return 0;
 }
 
-int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query) {
-    if (!t->learned) t->learned = new LearnedBuffers();
+// Grows the learned forms' record and query buffers; refused while `s` is
+// being captured into a graph (growing allocates and synchronises).
+int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query, hipStream_t s) {
+    if (!t->learned) {
+        if (stream_capturing(s))
+            return fail(nullptr, 0, "learned forms: buffers not reserved; call nan_tst_learned_reserve before "
+                                    "graph capture");
+        t->learned = new LearnedBuffers();
+    }
     LearnedBuffers &B = *t->learned;
     LearnedDev &d = B.d;
-    if (n_train > B.cap_train || n_query > B.cap_query) HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
+    if (n_train > B.cap_train || n_query > B.cap_query) {
+        if (stream_capturing(s))
+            return fail(nullptr, 0, "learned forms: %u training paths / %llu queries exceed the reserved %u / %llu; "
+                                    "call nan_tst_learned_reserve before graph capture", n_train,
+                        (unsigned long long)n_query, B.cap_train, (unsigned long long)B.cap_query);
+        HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
+    }
     if (n_train > B.cap_train) {
         size_t n = n_train;
         B.cap_train = 0;
@@ -451,6 +464,7 @@ int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query) {
         B.cap_query = n_query;
     }
     if (!d.partial) {
+        if (stream_capturing(s)) return fail(nullptr, 0, "learned forms: buffers not reserved before graph capture");
         if (grow(&d.partial, std::max<uint64_t>(t->total_payload, 1) * 16)) return -1;
     }
     return 0;
@@ -596,10 +610,11 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
     uint32_t nspp = samples_per_texel > 1u ? samples_per_texel : 1u;
     uint64_t nq = tracer->total_payload * nspp;
     if (nq >= (1ull << 31)) return fail(nullptr, 0, "nan_tst_execute_learned: too many cache queries per frame");
-    if (reserve(tracer, train_samples, nq) || nrc_reserve(nrc, (uint32_t)nq) || nrc_reserve_train(nrc, train_samples))
+    hipStream_t s = as_stream(stream);
+    if (reserve(tracer, train_samples, nq, s) || nrc_reserve(nrc, (uint32_t)nq, s) ||
+        nrc_reserve_train(nrc, train_samples, s))
         return -1;
     unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
-    hipStream_t s = as_stream(stream);
     // wavefront mode runs the forms on the tracer's queues when they need no
     // per-texel extras: 1 sample per texel, no emitter light sampling, no
     // counters, training paths within the tracer's path slots
@@ -611,6 +626,20 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
                           s);
     return launch<64>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
                       s);
+}
+
+NAN_API int nan_tst_learned_reserve(NanTracer *tracer, NanNrc *nrc, uint32_t samples_per_texel,
+                                    uint32_t train_samples) {
+    if (!tracer || !nrc) return fail(nullptr, 0, "nan_tst_learned_reserve: null argument");
+    uint32_t nspp = samples_per_texel > 1u ? samples_per_texel : 1u;
+    uint64_t nq = tracer->total_payload * nspp;
+    if (nq >= (1ull << 31)) return fail(nullptr, 0, "nan_tst_learned_reserve: too many cache queries per frame");
+    if (stream_capturing(nullptr))
+        return fail(nullptr, 0, "nan_tst_learned_reserve: called during graph capture");
+    if (reserve(tracer, train_samples, nq, nullptr) || nrc_reserve(nrc, (uint32_t)std::max<uint64_t>(nq, 1), nullptr) ||
+        nrc_reserve_train(nrc, std::max(train_samples, 1u), nullptr))
+        return -1;
+    return 0;
 }
 
 NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint32_t n, uint32_t max_bounces,
@@ -628,8 +657,8 @@ NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint3
     if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY))
         return fail(nullptr, 0, "nan_tst_learned_records: unknown flags");
     if (n == 0 || tracer->total_front == 0) return 0;
-    if (reserve(tracer, n, 0)) return -1;
     hipStream_t s = as_stream(stream);
+    if (reserve(tracer, n, 0, s)) return -1;
     const LearnedDev &B = tracer->learned->d;
     bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && tracer->scene->dev.emitter_count > 0;
     if (tracer->mode == NAN_TRACER_WAVEFRONT && !nee && n <= tracer->total_front) {

@@ -485,8 +485,22 @@ __global__ __launch_bounds__(64) void nrc_mlp_bwd(const _Float16 *wn, const _Flo
 // optimiser
 // ---------------------------------------------------------------------------
 struct AdamArgs {
-    float lr, b1, b2, eps, c1, c2;  // c1, c2: bias corrections 1 - beta^step
+    float lr, b1, b2, eps;
 };
+
+// Adam's step count lives on the device, so a captured training step that is
+// replayed takes the next step's bias corrections 1 - beta^step each time.
+// nrc_adam_mlp reads `step` and publishes step + 1 and its corrections;
+// nrc_adam_table reads those and stores `step` (neither kernel writes a field
+// that it reads).
+struct AdamState {
+    uint32_t step, next_step;
+    float c1, c2;
+};
+
+__device__ __forceinline__ float bias_correction(float beta, uint32_t step) {
+    return 1.0f - (float)pow((double)beta, (double)step);
+}
 
 __device__ __forceinline__ float grad_scale(const float *stats) {
     return 1.0f / (3.0f * fmaxf(stats[1], 1.0f));
@@ -511,17 +525,29 @@ __device__ __forceinline__ float reduce_partials(const float *partial, uint32_t 
 // MLP weights: Adam on the reduced gradient (block of 256 = 32 weights x 8).
 __global__ __launch_bounds__(256) void nrc_adam_mlp(float *w, float *m, float *v, const float *partial,
                                                      uint32_t parts, _Float16 *w16t, _Float16 *w16n,
-                                                     const float *stats, AdamArgs a) {
+                                                     const float *stats, AdamArgs a, AdamState *as) {
     __shared__ float lds[kReduceWeights * kReduceSplit];
+    __shared__ float corr[2];
     uint32_t k = blockIdx.x * kReduceWeights + threadIdx.x % kReduceWeights;
-    float g = reduce_partials(partial, parts, k, lds);
+    if (threadIdx.x == 0) {
+        uint32_t step = as->step + 1u;
+        corr[0] = bias_correction(a.b1, step);
+        corr[1] = bias_correction(a.b2, step);
+        if (blockIdx.x == 0) {
+            as->next_step = step;
+            as->c1 = corr[0];
+            as->c2 = corr[1];
+        }
+    }
+    float g = reduce_partials(partial, parts, k, lds);  // synchronises the block
     if (threadIdx.x >= kReduceWeights || k >= (uint32_t)kNrcWeights) return;
+    const float c1 = corr[0], c2 = corr[1];
     g *= grad_scale(stats);
     float mi = a.b1 * m[k] + (1.0f - a.b1) * g;
     float vi = a.b2 * v[k] + (1.0f - a.b2) * g * g;
     m[k] = mi;
     v[k] = vi;
-    float wi = w[k] - a.lr * (mi / a.c1) / (sqrtf(vi / a.c2) + a.eps);
+    float wi = w[k] - a.lr * (mi / c1) / (sqrtf(vi / c2) + a.eps);
     w[k] = wi;
     w16t[k] = (_Float16)wi;
     // plain [in][out] copy for the backward kernel
@@ -537,8 +563,10 @@ __global__ __launch_bounds__(256) void nrc_adam_mlp(float *w, float *m, float *v
 // An entry's state is one 32-byte record, so a touched entry costs one read
 // and one write of a single sector wherever it lies in the table.
 __global__ void nrc_adam_table(TableState *st, __half2 *grad, __half2 *w16, uint64_t n, const float *stats, AdamArgs a,
-                               int dense) {
+                               AdamState *as, int dense) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float c1 = as->c1, c2 = as->c2;
+    if (k == 0) as->step = as->next_step;
     if (k >= n) return;
     __half2 g2 = grad[k];
     float g[2] = {__low2float(g2), __high2float(g2)};
@@ -552,7 +580,7 @@ __global__ void nrc_adam_table(TableState *st, __half2 *grad, __half2 *w16, uint
         float gc = isfinite(g[c]) ? g[c] * sc : 0.0f;
         mv[c] = a.b1 * mv[c] + (1.0f - a.b1) * gc;
         vv[c] = a.b2 * vv[c] + (1.0f - a.b2) * gc * gc;
-        wv[c] = wv[c] - a.lr * (mv[c] / a.c1) / (sqrtf(vv[c] / a.c2) + a.eps);
+        wv[c] = wv[c] - a.lr * (mv[c] / c1) / (sqrtf(vv[c] / c2) + a.eps);
     }
     rec[0] = make_float4(wv[0], wv[1], mv[0], mv[1]);
     rec[1] = make_float4(vv[0], vv[1], 0.0f, 0.0f);
@@ -656,7 +684,7 @@ void forward_backward(NanNrc *c, const float *pos, const float *nrm, const float
 NanNrc::~NanNrc() {
     for (void *p : {(void *)w32, (void *)wm, (void *)wv, (void *)wpart, (void *)w16t, (void *)w16n, (void *)ts,
                     (void *)tg, (void *)t16, (void *)enc, (void *)save, (void *)dy,
-                    (void *)denc, (void *)pred, (void *)stats})
+                    (void *)denc, (void *)pred, (void *)stats, adam})
         hipFree(p);
 }
 
@@ -665,10 +693,14 @@ namespace nanx {
 int nrc_train_one(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                   const uint32_t *valid, uint32_t n, hipStream_t s);
 
-// Prediction scratch (encoded input, outputs) for n samples.
-int nrc_reserve(NanNrc *c, uint32_t n) {
+// Prediction scratch (encoded input, outputs) for n samples.  Growing it
+// allocates, so it is refused while `s` is being captured into a graph.
+int nrc_reserve(NanNrc *c, uint32_t n, hipStream_t s) {
     uint32_t groups = groups_for(n);
     if (groups <= c->cap_groups) return 0;
+    if (stream_capturing(s))
+        return fail(nullptr, 0, "nan_nrc: %u samples exceed the %u reserved; reserve before graph capture", n,
+                    c->cap_groups * 64);
     HIP_TRY(hipDeviceSynchronize(), nullptr, 0);  // earlier launches may still use the old buffers
     hipFree(c->enc);
     hipFree(c->pred);
@@ -684,10 +716,13 @@ int nrc_reserve(NanNrc *c, uint32_t n) {
 
 // Training scratch (saved activations, gradients) for n samples, plus the
 // prediction scratch.
-int nrc_reserve_train(NanNrc *c, uint32_t n) {
-    if (nrc_reserve(c, n)) return -1;
+int nrc_reserve_train(NanNrc *c, uint32_t n, hipStream_t s) {
+    if (nrc_reserve(c, n, s)) return -1;
     uint32_t groups = groups_for(n);
     if (groups <= c->cap_train_groups) return 0;
+    if (stream_capturing(s))
+        return fail(nullptr, 0, "nan_nrc: %u training records exceed the %u reserved; reserve before graph capture",
+                    n, c->cap_train_groups * 64);
     HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
     for (void *p : {(void *)c->save, (void *)c->dy, (void *)c->denc}) hipFree(p);
     c->save = c->dy = nullptr;
@@ -704,7 +739,7 @@ int nrc_reserve_train(NanNrc *c, uint32_t n) {
 int nrc_predict(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float4 *pred,
                 hipStream_t s) {
     if (n == 0) return 0;
-    if (nrc_reserve(c, n)) return -1;
+    if (nrc_reserve(c, n, s)) return -1;
     uint32_t groups = groups_for(n);
     encode(c, pos, nrm, alb, n, nullptr, s);
     hipLaunchKernelGGL((nrc_mlp_fwd<false>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, c->w16t, c->enc,
@@ -731,25 +766,22 @@ int nrc_train_step(NanNrc *c, const float *pos, const float *nrm, const float *a
 
 int nrc_train_one(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                   const uint32_t *valid, uint32_t n, hipStream_t s) {
-    if (nrc_reserve_train(c, n)) return -1;
+    if (nrc_reserve_train(c, n, s)) return -1;
     forward_backward(c, pos, nrm, alb, target, valid, n, s);
     hipLaunchKernelGGL(nrc_encode_bwd, dim3(grid_for((uint64_t)n * 8, 256), c->cfg.levels), dim3(256), 0, s,
                        grid_of(c), pos, n, c->denc, c->tg);
     uint32_t reduce_blocks = grid_for(kNrcWeights, kReduceWeights);
-    c->step++;
     const NanNrcConfig &cf = c->cfg;
     AdamArgs a;
     a.lr = cf.learning_rate;
     a.b1 = cf.beta1;
     a.b2 = cf.beta2;
     a.eps = cf.epsilon;
-    a.c1 = 1.0f - (float)std::pow((double)cf.beta1, (double)c->step);
-    a.c2 = 1.0f - (float)std::pow((double)cf.beta2, (double)c->step);
     hipLaunchKernelGGL(nrc_adam_mlp, dim3(reduce_blocks), dim3(256), 0, s, c->w32, c->wm, c->wv, c->wpart,
-                       bwd_waves(groups_for(n)), c->w16t, c->w16n, c->stats, a);
+                       bwd_waves(groups_for(n)), c->w16t, c->w16n, c->stats, a, (AdamState *)c->adam);
     uint64_t entries = c->table_params / 2;
     hipLaunchKernelGGL(nrc_adam_table, dim3(grid_for(entries, 256)), dim3(256), 0, s, c->ts, c->tg,
-                       (__half2 *)c->t16, entries, c->stats, a,
+                       (__half2 *)c->t16, entries, c->stats, a, (AdamState *)c->adam,
                        (int)cf.dense_table_adam);
     return check_launch("nrc train");
 }
@@ -871,8 +903,9 @@ NAN_API int nan_nrc_create(const NanNrcConfig *cfg, NanNrc **out, char *err, siz
               upload(&c->t16, t16.data(), t16.size() * 2, err, cap) == 0 &&
               upload(&c->tg, nullptr, t.size() * 2, err, cap) == 0 &&
               upload(&c->stats, stats.data(), stats.size() * 4, err, cap) == 0 &&
+              upload(&c->adam, zw.data(), 16, err, cap) == 0 &&
               hipMemset(c->tg, 0, t.size() * 2) == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
-              nrc_reserve_train(c, 65536) == 0;
+              nrc_reserve_train(c, 65536, nullptr) == 0;
     if (!ok) {
         if (err && cap && !err[0]) fail(err, cap, "nan_nrc_create: device allocation failed");
         delete c;
@@ -954,7 +987,7 @@ NAN_API int nan_nrc_time_forward(NanNrc *nrc, const float *pos, const float *nrm
     *ms_out = 0.0f;
     if (n == 0 || repeats == 0) return 0;
     hipStream_t s = as_stream(stream);
-    if (nrc_reserve(nrc, n)) return -1;
+    if (nrc_reserve(nrc, n, s)) return -1;
     uint32_t groups = groups_for(n);
     encode(nrc, pos, nrm, alb, n, nullptr, s);
     hipEvent_t a, b;
@@ -982,7 +1015,7 @@ NAN_API int nan_nrc_encode(NanNrc *nrc, const float *pos, const float *nrm, cons
                            void *stream) {
     if (!nrc || (n && (!pos || !nrm || !alb || !out))) return fail(nullptr, 0, "nan_nrc_encode: null argument");
     if (n == 0) return 0;
-    if (nrc_reserve(nrc, n)) return -1;
+    if (nrc_reserve(nrc, n, as_stream(stream))) return -1;
     encode(nrc, pos, nrm, alb, n, out, as_stream(stream));
     return check_launch("nrc encode");
 }
@@ -1004,7 +1037,7 @@ NAN_API int nan_nrc_backward_raw(NanNrc *nrc, const float *pos, const float *nrm
         return fail(nullptr, 0, "nan_nrc_backward_raw: null argument");
     if (n == 0) return 0;
     hipStream_t s = as_stream(stream);
-    if (nrc_reserve_train(nrc, n)) return -1;
+    if (nrc_reserve_train(nrc, n, s)) return -1;
     forward_backward(nrc, pos, nrm, alb, target, nullptr, n, s);
     hipLaunchKernelGGL(nrc_sum_partials, dim3(grid_for(kNrcWeights, kReduceWeights)), dim3(256), 0, s, nrc->wpart,
                        bwd_waves(groups_for(n)), nrc->stats, weight_grad);

@@ -66,15 +66,16 @@ struct NanNrc {
     float *denc = nullptr;     // [groups * 64][48]
     float4 *pred = nullptr;    // [groups * 64]
     float *stats = nullptr;    // [0] loss sum, [1] normalising count, [2] valid records (floats), [3] spare
-    uint32_t step = 0;
+    void *adam = nullptr;      // Adam step count and bias corrections (16 bytes, device)
     ~NanNrc();
 };
 
 namespace nanx {
 
-// Grow the prediction / training scratch to hold n samples.
-int nrc_reserve(NanNrc *c, uint32_t n);
-int nrc_reserve_train(NanNrc *c, uint32_t n);
+// Grow the prediction / training scratch to hold n samples (refused while
+// `stream` is being captured into a graph: growing allocates).
+int nrc_reserve(NanNrc *c, uint32_t n, hipStream_t stream);
+int nrc_reserve_train(NanNrc *c, uint32_t n, hipStream_t stream);
 // Network prediction for n samples into pred[n] (RGB + one spare output).
 int nrc_predict(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float4 *pred,
                 hipStream_t stream);

@@ -916,14 +916,13 @@ static int occupancy_blocks(int *per_cu) {
     return 0;
 }
 
-static int wavefront_alloc(NanTracer *t, uint32_t spp) {
+// The whole working set of the wavefront forms (per-path state, queues,
+// counters, spill area, the second stream and its events), sized from the
+// layout and the launch shape.  Allocated when the tracer is created and
+// again when nan_tracer_tune changes the launch shape, never on the execute
+// path, so the first execute may be captured into a graph.
+static int wavefront_alloc(NanTracer *t) {
     const WfTuning &tu = t->tuning;
-    if (t->wf && (t->wf->ring != tu.ring || t->wf->blocks_per_cu_req != tu.blocks_per_cu)) wavefront_release(t);
-    if (t->wf) {
-        if (spp > 1 && !t->wf->acc) HIP_TRY(hipMalloc((void **)&t->wf->acc, (size_t)t->wf->capacity * 16), nullptr, 0);
-        if (t->wf->ds) t->wf->ds->acc = t->wf->acc;
-        return 0;
-    }
     WfBuffers *W = new WfBuffers();
     t->wf = W;
     W->ring = tu.ring;
@@ -935,7 +934,8 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     HIP_TRY(hipMalloc((void **)&W->T, c4), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&W->rad, c4), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&W->sun, c4), nullptr, 0);
-    if (spp > 1) HIP_TRY(hipMalloc((void **)&W->acc, c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->acc, c4), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&W->qidx, (size_t)cap * 4), nullptr, 0);
     // A stage grid that is a multiple of kParts hands partition j the
     // 256-entry chunks c = j (mod kParts) of its input, so no partition
     // receives more than ceil(n / 256 / kParts) * 256 <= n / kParts + 256
@@ -1010,6 +1010,18 @@ static int wavefront_alloc(NanTracer *t, uint32_t spp) {
     return 0;
 }
 
+int wavefront_prepare(NanTracer *t) {
+    const WfTuning &tu = t->tuning;
+    if (t->wf && t->wf->ring == tu.ring && t->wf->blocks_per_cu_req == tu.blocks_per_cu) return 0;
+    wavefront_release(t);
+    if (wavefront_alloc(t)) {
+        std::string msg = g_last_error;
+        wavefront_release(t);
+        return fail(nullptr, 0, "wavefront buffers: %s", msg.c_str());
+    }
+    return 0;
+}
+
 // Persistent grid of a traversal launch: the occupancy-sized grid, or fewer
 // blocks (a multiple of kParts) when the queue holds at most max_rays rays.
 static uint32_t trav_grid(const WfBuffers &W, uint64_t max_rays) {
@@ -1042,12 +1054,21 @@ static void launch_shadow(const WfBuffers &W, const DevScene &S, uint32_t count_
                            fetch, steps, counters);
 }
 
+// Queue counter slots one sample of one sweep takes: the first ray and
+// shadow queues, then per bounce a refill counter for extend and for shadow
+// and the next ray and shadow queues.
+static inline uint32_t sample_slots(uint32_t bounces) { return 4 * bounces + 2; }
+
 // The sweeps of one set of paths (`list` null: front texel i is path i) on
 // one queue set and stream: per sample, generate, `bounces` rounds of extend /
-// shade / shadow, finish.  Counter slots come from [slot, slot_end).
+// shade / shadow, finish.  Counter slots come from [slot, slot_end); when a
+// sample would not fit, the range [slot_begin, slot_end) is zeroed again on
+// the stream and reused, so any number of samples and sweeps fits a range
+// that holds one sample (checked by the caller before the first launch).
 static int run_sweeps(NanTracer *t, WfBuffers &W, float4 *payload, const uint32_t *list, uint32_t count,
                       uint32_t side, uint32_t iteration, uint32_t spp, uint32_t bounces, uint32_t reset,
-                      unsigned long long *counters, hipStream_t stream, uint32_t &slot, uint32_t slot_end) {
+                      unsigned long long *counters, hipStream_t stream, uint32_t &slot, uint32_t slot_begin,
+                      uint32_t slot_end) {
     if (count == 0) return 0;
     const DevScene &S = t->scene->dev;
     DevLayout Ly = dev_layout(t);
@@ -1058,7 +1079,14 @@ static int run_sweeps(NanTracer *t, WfBuffers &W, float4 *payload, const uint32_
     uint32_t shade_blocks = std::min<uint32_t>(W.shade_blocks, round8);
     uint32_t finish_blocks = std::min<uint32_t>(W.finish_blocks, grid_for(count, kWfBlock));
     for (uint32_t k = 0; k < nspp; ++k) {
-        if (slot + 4 * bounces + 8 > slot_end) return fail(nullptr, 0, "wavefront: too many stages");
+        if (slot + sample_slots(bounces) > slot_end) {
+            // every earlier launch on this stream is done with the range by
+            // the time the zeroing runs (stream order): start it over
+            hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream,
+                               W.counters + (size_t)slot_begin * kParts * kCtrStride,
+                               (slot_end - slot_begin) * kParts * kCtrStride);
+            slot = slot_begin;
+        }
         uint32_t ext_slot = slot++, sh_slot = slot++;
         GenParams gp{t->device_frame ? W.frame : nullptr, list, count, side, k, iteration, spp, ext_slot, sh_slot};
         {
@@ -1123,18 +1151,21 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
                     kMaxStack);
     if (t->scene->batch && !t->payload_table_set)
         return fail(nullptr, 0, "wavefront: batch payloads are not set (nan_batch_set_payloads)");
-    if (wavefront_alloc(t, spp)) return -1;
+    if (!t->wf) return fail(nullptr, 0, "wavefront: working set missing (a failed nan_tracer_tune?)");
+    // each stream's counter range must hold one sample's queues; checked
+    // before any launch, so a refused call leaves the payload untouched
+    if ((uint64_t)sample_slots(bounces) > kCounterSlots / 2)
+        return fail(nullptr, 0, "wavefront: max_bounces %u exceeds the %u the queue counters hold", bounces,
+                    (kCounterSlots / 2 - 2) / 4);
     WfBuffers &W = *t->wf;
-    uint32_t nspp = spp > 1 ? spp : 1;
-    uint64_t per_sweep = (uint64_t)nspp * (4ull * bounces + 2);
-    bool split = W.ds != nullptr && !t->profile && 2 * per_sweep + 8 <= kCounterSlots / 2;
+    bool split = W.ds != nullptr && !t->profile;
     hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream, W.counters, kCounterWords);
     if (!split) {
         uint32_t slot = 0;
         if (run_sweeps(t, W, payload, nullptr, (uint32_t)t->total_front, 0, iteration, spp, bounces, reset,
-                       counters, stream, slot, kCounterSlots) ||
+                       counters, stream, slot, 0, kCounterSlots) ||
             run_sweeps(t, W, payload, t->ds_list, t->ds_count, 1, iteration, spp, bounces, reset, counters, stream,
-                       slot, kCounterSlots))
+                       slot, 0, kCounterSlots))
             return -1;
     } else {
         WfBuffers &D = *W.ds;
@@ -1143,13 +1174,13 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
         HIP_TRY(hipStreamWaitEvent(side_stream, W.ev_fork, 0), nullptr, 0);
         uint32_t dslot = kCounterSlots / 2, sslot = 0;
         int rc = run_sweeps(t, D, payload, t->ds_list, t->ds_count, 0, iteration, spp, bounces, reset, counters,
-                            side_stream, dslot, kCounterSlots);
+                            side_stream, dslot, kCounterSlots / 2, kCounterSlots);
         if (!rc)
             rc = run_sweeps(t, D, payload, t->ds_list, t->ds_count, 1, iteration, spp, bounces, reset, counters,
-                            side_stream, dslot, kCounterSlots);
+                            side_stream, dslot, kCounterSlots / 2, kCounterSlots);
         if (!rc)
             rc = run_sweeps(t, W, payload, t->ss_list, t->ss_count, 0, iteration, spp, bounces, reset, counters,
-                            stream, sslot, kCounterSlots / 2);
+                            stream, sslot, 0, kCounterSlots / 2);
         // the join is recorded whatever happened, so a capture never ends with the side stream forked
         hipError_t e1 = hipEventRecord(W.ev_join, side_stream);
         hipError_t e2 = hipStreamWaitEvent(stream, W.ev_join, 0);
@@ -1162,7 +1193,7 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
 }
 
 int wavefront_set_frame(NanTracer *t, uint32_t iteration, uint32_t reset) {
-    if (wavefront_alloc(t, 1)) return -1;
+    if (!t->wf) return fail(nullptr, 0, "wavefront: working set missing (a failed nan_tracer_tune?)");
     uint32_t v[2] = {iteration, reset ? 1u : 0u};
     HIP_TRY(hipMemcpy(t->wf->frame, v, sizeof(v), hipMemcpyHostToDevice), nullptr, 0);
     return 0;
@@ -1173,9 +1204,7 @@ static int learned_checks(NanTracer *t) {
     if (t->scene->info.max_stack > (uint32_t)kMaxStack)
         return fail(nullptr, 0, "wavefront: BVH needs a %u-entry stack (max %d)", t->scene->info.max_stack,
                     kMaxStack);
-    if (wavefront_alloc(t, 1)) return -1;
-    WfBuffers &W = *t->wf;
-    if (!W.qidx) HIP_TRY(hipMalloc((void **)&W.qidx, (size_t)W.capacity * 4), nullptr, 0);
+    if (!t->wf) return fail(nullptr, 0, "wavefront: working set missing (a failed nan_tracer_tune?)");
     return 0;
 }
 
@@ -1199,7 +1228,7 @@ int wavefront_learned_train_paths(NanTracer *t, const LearnedIO &io, uint32_t it
     WfBuffers &W = *t->wf;
     if (n == 0) return 0;
     if (n > W.capacity) return fail(nullptr, 0, "wavefront: %u training paths exceed the %u path slots", n, W.capacity);
-    if (4 * bounces + 8 > kCounterSlots) return fail(nullptr, 0, "wavefront: too many stages");
+    if (sample_slots(bounces) + 2 > kCounterSlots) return fail(nullptr, 0, "wavefront: too many stages");
     const DevScene &S = t->scene->dev;
     DevLayout Ly = dev_layout(t);
     uint32_t steps = std::max(1u, t->tuning.refill_steps);

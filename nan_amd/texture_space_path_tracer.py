@@ -138,6 +138,8 @@ class TextureSpacePathTracer:
         if counters is not None and (counters.dtype != torch.int64 or counters.numel() < _native.COUNTER_COUNT
                                      or not counters.is_cuda):
             raise ValueError("counters must be an int64 device tensor of at least 8 entries")
+        if not torch.cuda.is_current_stream_capturing():
+            self.reserve_learned(cache, train_samples)
         rc = _native.lib().nan_tst_execute_learned(
             self.handle, cache.handle, output.data_ptr(), max(0, int(iteration)), self.samples_per_texel,
             self.max_bounces, 1 if reset else 0, int(train_samples),
@@ -145,6 +147,16 @@ class TextureSpacePathTracer:
             | (_native.LEARNED_TEXEL_QUERY if texel_query else 0),
             None if counters is None else counters.data_ptr(), command_encoder.stream_handle)
         _native.check(rc, "nan_tst_execute_learned")
+
+    def reserve_learned(self, cache, train_samples: int = 65536) -> None:
+        """Sizes the learned form's buffers (training records, cache queries,
+        network scratch) for ``train_samples`` training paths per iteration.
+        ``execute_learned`` does this itself outside graph capture; call it
+        before capturing a first learned iteration into a graph, since a
+        captured call allocates nothing and refuses to grow them.  No
+        counterpart in the reference."""
+        _native.check(_native.lib().nan_tst_learned_reserve(self.handle, cache.handle, self.samples_per_texel,
+                                                            int(train_samples)), "nan_tst_learned_reserve")
 
     def tune(self, blocks_per_cu: int = 0, lds_stack: int = 16, refill_steps: int = 16) -> None:
         """Launch shape of the traversal kernels (results do not depend on it)."""

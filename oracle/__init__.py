@@ -285,7 +285,10 @@ def tonemap(inp: np.ndarray, exposure: float) -> np.ndarray:
 
 
 def cpu_count() -> int:
+    """Worker threads for the oracle: the process's CPUs, at most 16 (the
+    host share of one GPU on a shared machine)."""
     try:
-        return len(os.sched_getaffinity(0))
+        n = len(os.sched_getaffinity(0))
     except Exception:
-        return os.cpu_count() or 1
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
