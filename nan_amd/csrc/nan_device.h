@@ -292,15 +292,24 @@ __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, c
     uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
     // slab distance of plane q: (origin + q*s - o) * inv = q * (s*inv) + (origin - o) * inv,
     // one FMA per plane.  s = 2^e (e: the signed exponent byte), so s*inv =
-    // ldexp(inv, e) is exact; the rest
-    // rounds (origin - o), the product and the FMA: |error| <= 2eps|(origin - o)*inv| + eps|t|,
-    // no larger than the 3 roundings of (origin + q*s - o) * inv it replaces and
-    // well inside kPad and the slack
+    // ldexp(inv, e) is exact; the rest rounds (origin - o), the product b and
+    // the FMA: |error| <= 2eps|b| + eps|t|.  The eps|t| part is relative and
+    // inside kPad.  The 2eps|b| part is not: |b| = |(origin - o) * inv| grows as
+    // 1/|d| along an axis the ray nearly follows (a ray straight down from just
+    // above a vertex has |b| ~ 1e9 and t errors of tens of units on that axis,
+    // where the true slab interval is all or nothing).  So each axis's entry
+    // planes start from b - 2^-21|b| and its exit planes from b + 2^-21|b|
+    // (one FMA each, |b| a free source modifier): 8eps|b| covers the 2eps|b|
+    // and the FMA's own rounding, so the entry never exceeds and the exit
+    // never falls short of the exact slab distance by more than eps|t|.
     const float ax = __builtin_ldexpf(r.inv.x, (int)(int8_t)(w0.w & 0xFFu)),
                 ay = __builtin_ldexpf(r.inv.y, (int)(int8_t)((w0.w >> 8) & 0xFFu)),
                 az = __builtin_ldexpf(r.inv.z, (int)(int8_t)((w0.w >> 16) & 0xFFu));
     const float bx = (origin.x - r.o.x) * r.inv.x, by = (origin.y - r.o.y) * r.inv.y,
                 bz = (origin.z - r.o.z) * r.inv.z;
+    const float bnx = __builtin_fmaf(fabsf(bx), -0x1p-21f, bx), bfx = __builtin_fmaf(fabsf(bx), 0x1p-21f, bx);
+    const float bny = __builtin_fmaf(fabsf(by), -0x1p-21f, by), bfy = __builtin_fmaf(fabsf(by), 0x1p-21f, by);
+    const float bnz = __builtin_fmaf(fabsf(bz), -0x1p-21f, bz), bfz = __builtin_fmaf(fabsf(bz), 0x1p-21f, bz);
     // the entry plane of each axis follows the sign of the direction (s*inv has
     // the sign of inv and the FMA is monotone in q), so the near and far
     // planes of all four children are picked once per node instead of
@@ -311,12 +320,12 @@ __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, c
     float tn[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        float tnx = __builtin_fmaf((float)((nqx >> (8 * c)) & 0xFFu), ax, bx);
-        float tfx = __builtin_fmaf((float)((fqx >> (8 * c)) & 0xFFu), ax, bx);
-        float tny = __builtin_fmaf((float)((nqy >> (8 * c)) & 0xFFu), ay, by);
-        float tfy = __builtin_fmaf((float)((fqy >> (8 * c)) & 0xFFu), ay, by);
-        float tnz = __builtin_fmaf((float)((nqz >> (8 * c)) & 0xFFu), az, bz);
-        float tfz = __builtin_fmaf((float)((fqz >> (8 * c)) & 0xFFu), az, bz);
+        float tnx = __builtin_fmaf((float)((nqx >> (8 * c)) & 0xFFu), ax, bnx);
+        float tfx = __builtin_fmaf((float)((fqx >> (8 * c)) & 0xFFu), ax, bfx);
+        float tny = __builtin_fmaf((float)((nqy >> (8 * c)) & 0xFFu), ay, bny);
+        float tfy = __builtin_fmaf((float)((fqy >> (8 * c)) & 0xFFu), ay, bfy);
+        float tnz = __builtin_fmaf((float)((nqz >> (8 * c)) & 0xFFu), az, bnz);
+        float tfz = __builtin_fmaf((float)((fqz >> (8 * c)) & 0xFFu), az, bfz);
         float near = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
         float far = __builtin_fmaf(fminf(fminf(tfx, tfy), tfz), kPad, slack);
         far = fminf(far, limit);

@@ -56,6 +56,7 @@ def liboracle():
     lib.orc_address_to_ij.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p]
     lib.orc_nearest_texel.restype = ctypes.c_uint32
     lib.orc_nearest_texel.argtypes = [f32p, u32p, u32p]
+    lib.orc_sample_irradiance.argtypes = [f32p, u32p, f32p, u32p, u32p, u32p, ctypes.c_int, f32p]
     lib.orc_texel_raygen.restype = ctypes.c_int
     lib.orc_texel_raygen.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p]
     lib.orc_pack_rgb9e5.restype = ctypes.c_uint32
@@ -136,6 +137,32 @@ def address_to_ij(address: int, resolution: int) -> tuple[int, int]:
     ij = np.zeros(2, dtype=np.uint32)
     liboracle().orc_address_to_ij(address, resolution, _p(ij, u32p))
     return int(ij[0]), int(ij[1])
+
+
+def nearest_texel(bary, inst_row, face_row) -> int:
+    """mesh_colors_nearest_texel for one barycentric triple (float32[3])
+    on the face whose layout rows are given (instance uint32[4], face
+    uint32[2]); returns the slot index."""
+    b = np.ascontiguousarray(bary, dtype=np.float32).reshape(3)
+    i = np.ascontiguousarray(inst_row, dtype=np.uint32).reshape(4)
+    f = np.ascontiguousarray(face_row, dtype=np.uint32).reshape(2)
+    return int(liboracle().orc_nearest_texel(_p(b, f32p), _p(i, u32p), _p(f, u32p)))
+
+
+def sample_irradiance(bary, inst_row, face_row, side_row, back: bool = False, payload=None, packed=None):
+    """The resolve's 3-tap lookup (mesh_colors_sample_irradiance) for one
+    barycentric triple on one face, reading payload (float32[N,4]) or packed
+    RGB9E5 words; returns float32[3]."""
+    b = np.ascontiguousarray(bary, dtype=np.float32).reshape(3)
+    i = np.ascontiguousarray(inst_row, dtype=np.uint32).reshape(4)
+    f = np.ascontiguousarray(face_row, dtype=np.uint32).reshape(2)
+    sd = np.ascontiguousarray(side_row, dtype=np.uint32).reshape(2)
+    out = np.zeros(3, dtype=np.float32)
+    pl = None if payload is None else np.ascontiguousarray(payload, dtype=np.float32)
+    pk = None if packed is None else np.ascontiguousarray(packed, dtype=np.uint32)
+    liboracle().orc_sample_irradiance(_p(pl, f32p), _p(pk, u32p), _p(b, f32p), _p(i, u32p), _p(f, u32p),
+                                      _p(sd, u32p), 1 if back else 0, _p(out, f32p))
+    return out
 
 
 def texel_raygen(faces, instances, iteration: int, spp: int):

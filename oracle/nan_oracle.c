@@ -1259,6 +1259,19 @@ static f3 sample_irradiance(const float *payload, const uint32_t *packed, f3 b, 
     return add(add(muls(A, w.x), muls(B, w.y)), muls(C, w.z));
 }
 
+/* The lookup alone, for the checks that pin it (a linear field resolves to
+ * the query's barycentrics, a lattice point to its own texel): bary float[3],
+ * inst uint32[4], face uint32[2], side uint32[2] rows of the layout tables;
+ * payload float[][4] or packed RGB9E5 words (payload NULL); out float[3]. */
+ORC_API void orc_sample_irradiance(const float *payload, const uint32_t *packed, const float *bary,
+                                   const uint32_t *inst, const uint32_t *face, const uint32_t *side, int back,
+                                   float *out) {
+    f3 e = sample_irradiance(payload, packed, mk(bary[0], bary[1], bary[2]), inst, face, side, back);
+    out[0] = e.x;
+    out[1] = e.y;
+    out[2] = e.z;
+}
+
 /* sky plus sun disk for a primary miss (scene.slang:431-527) */
 static f3 sky_with_sun_disk(const orc_light *L, const float *lighting, f3 dir) {
     f3 out = miss_radiance(L, dir);
