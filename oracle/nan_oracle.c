@@ -501,6 +501,7 @@ typedef struct {
     int kx, ky, kz;
     float Sx, Sy, Sz;
     f3 inv;   /* slab reciprocals with zero components replaced */
+    int flat[3];  /* direction component below 1e-30: the ray keeps that coordinate */
 } orc_ray;
 
 static void ray_setup(orc_ray *r) {
@@ -519,7 +520,8 @@ static void ray_setup(orc_ray *r) {
     float dd[3] = {r->d.x, r->d.y, r->d.z}, iv[3];
     for (int k = 0; k < 3; ++k) {
         float x = dd[k];
-        if (fabsf(x) < 1e-30f) x = copysignf(1e-30f, x);
+        r->flat[k] = fabsf(x) < 1e-30f;
+        if (r->flat[k]) x = copysignf(1e-30f, x);
         iv[k] = 1.0f / x;
     }
     r->inv = mk(iv[0], iv[1], iv[2]);
@@ -567,10 +569,20 @@ static int tri_intersect(const orc_ray *r, const float *tri, float *t_out, float
  * hit just behind the origin) by a few ulps of the coordinates. */
 #define ORC_PAD 0x1.004p+0f  /* 1 + 2^-10 */
 
+/* An axis the ray does not move along (|d| < 1e-30: under 1e-27 over the
+ * longest ray) meets the slab iff the origin lies in it, faces included: with
+ * the clamped reciprocal a ray lying in a face plane would get the slab
+ * interval [-huge, 0] and lose the box, and an edge hit on that face with it. */
 static int box_hit(const orc_ray *r, const float *lo, const float *hi, float tfar_limit, float slack) {
-    float t0x = (lo[0] - r->o.x) * r->inv.x, t1x = (hi[0] - r->o.x) * r->inv.x;
-    float t0y = (lo[1] - r->o.y) * r->inv.y, t1y = (hi[1] - r->o.y) * r->inv.y;
-    float t0z = (lo[2] - r->o.z) * r->inv.z, t1z = (hi[2] - r->o.z) * r->inv.z;
+    const float o3[3] = {r->o.x, r->o.y, r->o.z};
+    for (int k = 0; k < 3; ++k)
+        if (r->flat[k] && !(o3[k] >= lo[k] && o3[k] <= hi[k])) return 0;
+    float t0x = r->flat[0] ? -INFINITY : (lo[0] - r->o.x) * r->inv.x;
+    float t1x = r->flat[0] ? INFINITY : (hi[0] - r->o.x) * r->inv.x;
+    float t0y = r->flat[1] ? -INFINITY : (lo[1] - r->o.y) * r->inv.y;
+    float t1y = r->flat[1] ? INFINITY : (hi[1] - r->o.y) * r->inv.y;
+    float t0z = r->flat[2] ? -INFINITY : (lo[2] - r->o.z) * r->inv.z;
+    float t1z = r->flat[2] ? INFINITY : (hi[2] - r->o.z) * r->inv.z;
     float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r->tmin));
     float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * ORC_PAD + slack;
     tf = fminf(tf, tfar_limit);
