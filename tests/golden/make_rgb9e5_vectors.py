@@ -1,10 +1,20 @@
 """Float64 RGB9E5 codec restated from the reference's own test
 (test_mesh_colors_rgb9e5.py:15-50): the definition the oracle and the
-kernels must reproduce bit for bit."""
+kernels must reproduce bit for bit.
+
+CPU-only: the CPU tests pin the oracle against this definition directly,
+and running this file writes the fixture the GPU tests read instead
+(``rgb9e5_vectors.npz`` beside it: the reference's seven vectors and the
+words this definition assigns them).  The file itself does not travel to
+GPU machines.
+
+    python tests/golden/make_rgb9e5_vectors.py
+"""
 
 from __future__ import annotations
 
 import math
+from pathlib import Path
 
 import numpy as np
 
@@ -76,3 +86,18 @@ REFERENCE_VECTORS = np.asarray(
 )
 REFERENCE_WORDS = np.asarray(
     [0x0, 0x84020100, 0x84010040, 0xB0105190, 0xFFFFFFFF, 0xFFFC0000, 0x4002001], dtype=np.uint32)
+
+
+FIXTURE = Path(__file__).resolve().parent / "rgb9e5_vectors.npz"
+
+
+def main() -> None:
+    words = pack_rgb9e5_reference_array(REFERENCE_VECTORS)
+    # the words transcribed above agree with the definition
+    assert (words == REFERENCE_WORDS).all()
+    np.savez(FIXTURE, vectors=REFERENCE_VECTORS, words=words)
+    print(f"wrote {FIXTURE}")
+
+
+if __name__ == "__main__":
+    main()

@@ -6,6 +6,7 @@ bit-exact; the tone map (powf/exp2f) is checked to 2e-6 absolute."""
 from __future__ import annotations
 
 import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -18,7 +19,11 @@ import oracle  # noqa: E402
 from nan_amd.mesh_colors import MeshColorsLayout  # noqa: E402
 from nan_amd.scene_arrays import flatten_scene  # noqa: E402
 from nan_amd.scenes import batch_scene, cornell_box, synthetic_scene  # noqa: E402
-from tests.rgb9e5_reference import REFERENCE_VECTORS, REFERENCE_WORDS, pack_rgb9e5_reference_array  # noqa: E402
+
+# the reference's seven RGB9E5 vectors and the words its float64 definition
+# assigns them (tests/golden/make_rgb9e5_vectors.py writes this fixture)
+_GOLDEN = np.load(Path(__file__).resolve().parent / "golden" / "rgb9e5_vectors.npz")
+REFERENCE_VECTORS, REFERENCE_WORDS = _GOLDEN["vectors"], _GOLDEN["words"]
 
 
 def _say(msg: str) -> None:

@@ -4,6 +4,7 @@ hits against FP64 brute force, and closed-form lighting invariants."""
 from __future__ import annotations
 
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -16,13 +17,26 @@ from nan_amd.scene_arrays import flatten_scene
 from nan_amd.scene_node import Lighting, SceneNode
 from nan_amd.scenes import batch_scene, cornell_box
 from nan_amd.transform import Transform
-from tests.rgb9e5_reference import (
-    REFERENCE_VECTORS,
-    REFERENCE_WORDS,
-    pack_rgb9e5_reference,
-    pack_rgb9e5_reference_array,
-    unpack_rgb9e5_reference,
-)
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _rgb9e5_float64():
+    """The float64 RGB9E5 definition restated from the reference's test
+    (tests/golden/make_rgb9e5_vectors.py; CPU machines only)."""
+    return pytest.importorskip("tests.golden.make_rgb9e5_vectors")
+
+
+def pack_rgb9e5_reference(v):
+    return _rgb9e5_float64().pack_rgb9e5_reference(v)
+
+
+def pack_rgb9e5_reference_array(v):
+    return _rgb9e5_float64().pack_rgb9e5_reference_array(v)
+
+
+def unpack_rgb9e5_reference(w):
+    return _rgb9e5_float64().unpack_rgb9e5_reference(w)
 
 
 # ---------------------------------------------------------------- RNG
@@ -55,9 +69,12 @@ def test_sincos_accuracy():
 
 # ---------------------------------------------------------------- RGB9E5
 def test_rgb9e5_reference_vectors():
-    got = oracle.pack_rgb9e5(REFERENCE_VECTORS)
-    np.testing.assert_array_equal(got, REFERENCE_WORDS)
-    for v, w in zip(REFERENCE_VECTORS, REFERENCE_WORDS):
+    fixture = np.load(GOLDEN / "rgb9e5_vectors.npz")
+    vectors, words = fixture["vectors"], fixture["words"]
+    np.testing.assert_array_equal(vectors.view(np.uint32), _rgb9e5_float64().REFERENCE_VECTORS.view(np.uint32))
+    got = oracle.pack_rgb9e5(vectors)
+    np.testing.assert_array_equal(got, words)
+    for v, w in zip(vectors, words):
         assert pack_rgb9e5_reference(v) == w
         np.testing.assert_array_equal(oracle.unpack_rgb9e5(int(w)), unpack_rgb9e5_reference(int(w)))
 
