@@ -297,6 +297,16 @@ NAN_API int nan_nrc_set_params(NanNrc *nrc, const float *weights, const float *t
  * albedo float[n][3], out float[n][3]; device memory. */
 NAN_API int nan_nrc_infer(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out, uint32_t n,
                           void *stream);
+/* nan_nrc_infer without any allocation: no reference counterpart; for callers
+ * that cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid it).  Same kernels and results as
+ * nan_nrc_infer, with the scratch (encoded input, raw outputs) in a 7 MiB
+ * device array of the library's own that each GPU holds until the process
+ * ends.  At most 65,536 samples per call (more: an error, nothing launched).
+ * All calls on one GPU share that array, so calls that overlap on one GPU
+ * (other streams, unordered threads) race. */
+NAN_API int nan_nrc_infer_noalloc(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out,
+                                  uint32_t n, void *stream);
 /* Training on n records with targets float[n][3]: forward, loss, backward and
  * Adam, once per minibatch (config.minibatches consecutive slices); adds the
  * last step's mean loss to *loss_out (device float, may be NULL). */

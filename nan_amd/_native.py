@@ -91,7 +91,7 @@ EXPORTED_SYMBOLS = (
     "nan_resolve", "nan_accumulate", "nan_tonemap", "nan_tracer_tune", "nan_batch_create",
     "nan_batch_set_payloads", "nan_batch_set_bounces", "nan_tracer_device_frame",
     "nan_nrc_create", "nan_nrc_destroy", "nan_nrc_param_counts", "nan_nrc_get_params", "nan_nrc_set_params",
-    "nan_nrc_infer", "nan_nrc_train", "nan_nrc_forward_raw", "nan_nrc_backward_raw", "nan_nrc_encode",
+    "nan_nrc_infer", "nan_nrc_infer_noalloc", "nan_nrc_train", "nan_nrc_forward_raw", "nan_nrc_backward_raw", "nan_nrc_encode",
     "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned", "nan_surface_filter",
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
@@ -149,6 +149,7 @@ def _declare(lib) -> None:
         "nan_nrc_get_params": (I, [vp, vp, vp]),
         "nan_nrc_set_params": (I, [vp, vp, vp]),
         "nan_nrc_infer": (I, [vp, vp, vp, vp, vp, U32, vp]),
+        "nan_nrc_infer_noalloc": (I, [vp, vp, vp, vp, vp, U32, vp]),
         "nan_nrc_train": (I, [vp, vp, vp, vp, vp, U32, vp, vp]),
         "nan_nrc_forward_raw": (I, [vp, vp, vp, vp, vp, U32, vp]),
         "nan_nrc_backward_raw": (I, [vp, vp, vp, vp, vp, U32, vp, vp, vp]),

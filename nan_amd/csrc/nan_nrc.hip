@@ -659,11 +659,18 @@ uint32_t fwd_blocks(uint32_t groups) { return std::max<uint32_t>(1, std::min<uin
 
 uint32_t bwd_waves(uint32_t groups) { return std::max<uint32_t>(1, std::min<uint32_t>(groups, kNrcBwdWaves)); }
 
-void encode(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float *enc_f32,
-            hipStream_t s) {
+// Encoded input of n samples into enc (every entry of the groups used is
+// written, padding rows included).
+void encode_to(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, _Float16 *enc,
+               float *enc_f32, hipStream_t s) {
     uint32_t np = groups_for(n) * 64;
     dim3 grid(grid_for(np, 256), (c->cfg.levels + kEncLevels - 1) / kEncLevels + 1);
-    hipLaunchKernelGGL(nrc_encode_fwd, grid, dim3(256), 0, s, grid_of(c), pos, nrm, alb, n, np, c->enc, enc_f32);
+    hipLaunchKernelGGL(nrc_encode_fwd, grid, dim3(256), 0, s, grid_of(c), pos, nrm, alb, n, np, enc, enc_f32);
+}
+
+void encode(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float *enc_f32,
+            hipStream_t s) {
+    encode_to(c, pos, nrm, alb, n, c->enc, enc_f32, s);
 }
 
 // Forward (saving activations), loss and backward into c->wpart / c->denc.
@@ -745,6 +752,18 @@ int nrc_predict(NanNrc *c, const float *pos, const float *nrm, const float *alb,
     hipLaunchKernelGGL((nrc_mlp_fwd<false>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, c->w16t, c->enc,
                        groups, n, pred, (_Float16 *)nullptr, c->cap_groups, (float *)nullptr);
     return check_launch("nrc predict");
+}
+
+int nrc_infer_into(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, _Float16 *enc,
+                   uint32_t enc_groups, float4 *pred, float *out, hipStream_t s) {
+    if (n == 0) return 0;
+    uint32_t groups = groups_for(n);
+    if (groups > enc_groups) return fail(nullptr, 0, "nan_nrc: %u samples exceed the %u scratch groups", n, enc_groups);
+    encode_to(c, pos, nrm, alb, n, enc, nullptr, s);
+    hipLaunchKernelGGL((nrc_mlp_fwd<false>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, c->w16t, enc, groups,
+                       n, pred, (_Float16 *)nullptr, enc_groups, (float *)nullptr);
+    hipLaunchKernelGGL(nrc_rgb, dim3(grid_for(n, 256)), dim3(256), 0, s, pred, n, out, 0);
+    return 0;
 }
 
 int nrc_train_step(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
@@ -956,8 +975,8 @@ NAN_API int nan_nrc_infer(NanNrc *nrc, const float *pos, const float *nrm, const
     if (!nrc || (n && (!pos || !nrm || !alb || !out))) return fail(nullptr, 0, "nan_nrc_infer: null argument");
     if (n == 0) return 0;
     hipStream_t s = as_stream(stream);
-    if (nrc_predict(nrc, pos, nrm, alb, n, nrc->pred, s)) return -1;
-    hipLaunchKernelGGL(nrc_rgb, dim3(grid_for(n, 256)), dim3(256), 0, s, nrc->pred, n, out, 0);
+    if (nrc_reserve(nrc, n, s) || nrc_infer_into(nrc, pos, nrm, alb, n, nrc->enc, nrc->cap_groups, nrc->pred, out, s))
+        return -1;
     return check_launch("nrc infer");
 }
 

@@ -15,6 +15,7 @@ constexpr int kNrcHidden = 4;   // hidden layers
 constexpr int kNrcLayers = kNrcHidden + 1;
 constexpr int kNrcMaxLevels = 16;
 constexpr int kNrcBwdWaves = 512;  // waves (one per block) of the backward kernel
+constexpr int kNrcEncGroup = 64 * kNrcIn;  // halves of encoded input per 64-sample group
 
 __host__ __device__ constexpr int nrc_layer_in(int l) { return l == 0 ? kNrcIn : kNrcWidth; }
 __host__ __device__ constexpr int nrc_layer_out(int l) { return l == kNrcLayers - 1 ? kNrcOut : kNrcWidth; }
@@ -79,6 +80,11 @@ int nrc_reserve_train(NanNrc *c, uint32_t n, hipStream_t stream);
 // Network prediction for n samples into pred[n] (RGB + one spare output).
 int nrc_predict(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float4 *pred,
                 hipStream_t stream);
+// The cache query of nan_nrc_infer with its scratch passed in: encoded input
+// enc (enc_groups groups of 64 samples) and raw outputs pred (enc_groups * 64
+// float4), then the clamped RGB into out[n][3].  Allocates nothing.
+int nrc_infer_into(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, _Float16 *enc,
+                   uint32_t enc_groups, float4 *pred, float *out, hipStream_t stream);
 // Adam steps (cfg.minibatches of them, over consecutive slices) on the
 // relative L2 loss over n records.  valid (nullable): per-record flag;
 // invalid records contribute nothing, and the loss is normalised by the
