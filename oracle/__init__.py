@@ -93,6 +93,8 @@ def liboracle():
     lib.orc_atmo_sun_transmittance.argtypes = [f32p, f32p, f32p]
     lib.orc_atmo_transmittance_lut.argtypes = [f32p]
     lib.orc_atmo_multiscattering_lut.argtypes = [f32p, f32p]
+    lib.orc_atmo_multiscattering_texel.argtypes = [f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                   f32p]
     lib.orc_atmo_sky_view_lut.argtypes = [f32p, f32p, f32p, f32p]
     lib.orc_atmo_view_transmittance.argtypes = [f32p, f32p, f32p]
     _LIB = lib

@@ -47,6 +47,17 @@ def multiscattering_lut(transmittance: np.ndarray) -> np.ndarray:
     return out
 
 
+def multiscattering_texel(transmittance: np.ndarray, px: int, py: int, directions: int, steps: int) -> np.ndarray:
+    """One texel of the multiple-scattering table computed with a
+    directions x directions rule and `steps` ray-march steps (the table uses
+    the reference's 8 and 20); float32[3]."""
+    t = np.ascontiguousarray(transmittance, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    liboracle().orc_atmo_multiscattering_texel(_p(t, f32p), int(px), int(py), int(directions), int(steps),
+                                               _p(out, f32p))
+    return out
+
+
 def sky_view_lut(transmittance: np.ndarray, multiscattering: np.ndarray, sun_direction) -> np.ndarray:
     t = np.ascontiguousarray(transmittance, dtype=np.float32)
     m = np.ascontiguousarray(multiscattering, dtype=np.float32)

@@ -18,11 +18,13 @@
  * geometric-series sum of the exponential densities; the horizon boundary of
  * the transmittance table; monotonicity in sun angle and height; the
  * azimuthal mirror symmetry of the sky-view table.  The multiple-scattering
- * table is pinned by invariants: finite and non-negative, exactly linear in
- * the transmittance table (doubling it doubles every value bit for bit; a
- * zero table gives zero), zero where the sun is under the ground; the
- * sky-view table is exactly linear in its two input tables jointly and the
- * sum of its single- and multiple-scattering parts. */
+ * table is pinned by invariants (finite and non-negative, exactly linear in
+ * the transmittance table, zero where the sun is under the ground) and by
+ * the integrals it discretises: the same code at 128 x 128 directions and 800
+ * steps agrees with an independent float64 evaluation of Hillaire's L2 and
+ * f_ms integrals within 1% (3% at grazing sun); the sky-view table is exactly
+ * linear in its two input tables jointly and the sum of its single- and
+ * multiple-scattering parts. */
 #include <math.h>
 #include <stdint.h>
 
@@ -167,60 +169,72 @@ ORC_API void orc_atmo_transmittance_lut(float *out) {
         }
 }
 
-/* multiscatt_lut_compute + getMulScattValues (atmosphere.slang:179-258):
- * out float[32][32][4] */
-ORC_API void orc_atmo_multiscattering_lut(const float *tlut, float *out) {
+/* getMulScattValues (atmosphere.slang:179-258) for one table texel with a
+ * direction rule of sq x sq and ray marches of `steps` (the reference: 8 and
+ * 20).  Larger values let the checks converge the same code to the integrals
+ * it discretises. */
+static v3 multiscattering_texel(const float *tlut, int px, int py, int sq, int steps) {
     const v3 ground_albedo = mk(0.3f, 0.3f, 0.3f);
-    const int sq = 8;
-    for (int py = 0; py < MH; ++py)
-        for (int px = 0; px < MW; ++px) {
-            v3 pos, sun;
-            lut_coords(px, py, MW, MH, &pos, &sun);
-            float inv = 1.0f / (float)(sq * sq);
-            v3 lum_total = mk(0.0f, 0.0f, 0.0f), fms = mk(0.0f, 0.0f, 0.0f);
-            for (int i = 0; i < sq; ++i)
-                for (int j = 0; j < sq; ++j) {
-                    float theta = kPi * ((float)i + 0.5f) / (float)sq;
-                    float phi = safeacos(1.0f - 2.0f * ((float)j + 0.5f) / (float)sq);
-                    float cp = cosf(phi), sp = sinf(phi), ct = cosf(theta), st = sinf(theta);
-                    v3 dir = mk(sp * st, cp, sp * ct);
-                    float atmo = sphere_hit(pos, dir, kTop), grd = sphere_hit(pos, dir, kGround);
-                    float tmax = grd > 0.0f ? grd : atmo;
-                    float c = dot(dir, sun);
-                    float mp = mie_phase(c), rp = rayleigh_phase(-c);
-                    v3 lum = mk(0.0f, 0.0f, 0.0f), lum_factor = mk(0.0f, 0.0f, 0.0f), tr = mk(1.0f, 1.0f, 1.0f);
-                    float t = 0.0f;
-                    for (float s = 0.0f; s < 20.0f; s += 1.0f) {
-                        float nt = ((s + 0.3f) / 20.0f) * tmax;
-                        float dt = nt - t;
-                        t = nt;
-                        v3 np = add(pos, muls(dir, t));
-                        v3 rs, ext;
-                        float ms;
-                        scattering(np, &rs, &ms, &ext);
-                        v3 stt = expv(mk(-dt * ext.x, -dt * ext.y, -dt * ext.z));
-                        v3 snp = mk(rs.x + ms, rs.y + ms, rs.z + ms);
-                        lum_factor = add(lum_factor, mul(tr, seg(snp, stt, ext)));
-                        v3 sunt = lut_at(tlut, TW, TH, np, sun);
-                        v3 ris = muls(rs, rp);
-                        float mis = ms * mp;
-                        v3 ins = mul(mk(ris.x + mis, ris.y + mis, ris.z + mis), sunt);
-                        lum = add(lum, mul(seg(ins, stt, ext), tr));
-                        tr = mul(tr, stt);
-                    }
-                    if (grd > 0.0f) {
-                        v3 hit = add(pos, muls(dir, grd));
-                        if (dot(pos, sun) > 0.0f) {
-                            hit = muls(normalize(hit), kGround);
-                            lum = add(lum, mul(mul(tr, ground_albedo), lut_at(tlut, TW, TH, hit, sun)));
-                        }
-                    }
-                    fms = add(fms, muls(lum_factor, inv));
-                    lum_total = add(lum_total, muls(lum, inv));
+    v3 pos, sun;
+    lut_coords(px, py, MW, MH, &pos, &sun);
+    float inv = 1.0f / (float)(sq * sq);
+    const float fsteps = (float)steps;
+    v3 lum_total = mk(0.0f, 0.0f, 0.0f), fms = mk(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < sq; ++i)
+        for (int j = 0; j < sq; ++j) {
+            float theta = kPi * ((float)i + 0.5f) / (float)sq;
+            float phi = safeacos(1.0f - 2.0f * ((float)j + 0.5f) / (float)sq);
+            float cp = cosf(phi), sp = sinf(phi), ct = cosf(theta), st = sinf(theta);
+            v3 dir = mk(sp * st, cp, sp * ct);
+            float atmo = sphere_hit(pos, dir, kTop), grd = sphere_hit(pos, dir, kGround);
+            float tmax = grd > 0.0f ? grd : atmo;
+            float c = dot(dir, sun);
+            float mp = mie_phase(c), rp = rayleigh_phase(-c);
+            v3 lum = mk(0.0f, 0.0f, 0.0f), lum_factor = mk(0.0f, 0.0f, 0.0f), tr = mk(1.0f, 1.0f, 1.0f);
+            float t = 0.0f;
+            for (float s = 0.0f; s < fsteps; s += 1.0f) {
+                float nt = ((s + 0.3f) / fsteps) * tmax;
+                float dt = nt - t;
+                t = nt;
+                v3 np = add(pos, muls(dir, t));
+                v3 rs, ext;
+                float ms;
+                scattering(np, &rs, &ms, &ext);
+                v3 stt = expv(mk(-dt * ext.x, -dt * ext.y, -dt * ext.z));
+                v3 snp = mk(rs.x + ms, rs.y + ms, rs.z + ms);
+                lum_factor = add(lum_factor, mul(tr, seg(snp, stt, ext)));
+                v3 sunt = lut_at(tlut, TW, TH, np, sun);
+                v3 ris = muls(rs, rp);
+                float mis = ms * mp;
+                v3 ins = mul(mk(ris.x + mis, ris.y + mis, ris.z + mis), sunt);
+                lum = add(lum, mul(seg(ins, stt, ext), tr));
+                tr = mul(tr, stt);
+            }
+            if (grd > 0.0f) {
+                v3 hit = add(pos, muls(dir, grd));
+                if (dot(pos, sun) > 0.0f) {
+                    hit = muls(normalize(hit), kGround);
+                    lum = add(lum, mul(mul(tr, ground_albedo), lut_at(tlut, TW, TH, hit, sun)));
                 }
-            put4(out + 4 * (py * MW + px),
-                 mk(lum_total.x / (1.0f - fms.x), lum_total.y / (1.0f - fms.y), lum_total.z / (1.0f - fms.z)));
+            }
+            fms = add(fms, muls(lum_factor, inv));
+            lum_total = add(lum_total, muls(lum, inv));
         }
+    return mk(lum_total.x / (1.0f - fms.x), lum_total.y / (1.0f - fms.y), lum_total.z / (1.0f - fms.z));
+}
+
+/* multiscatt_lut_compute (atmosphere.slang:172-258): out float[32][32][4] */
+ORC_API void orc_atmo_multiscattering_lut(const float *tlut, float *out) {
+    for (int py = 0; py < MH; ++py)
+        for (int px = 0; px < MW; ++px) put4(out + 4 * (py * MW + px), multiscattering_texel(tlut, px, py, 8, 20));
+}
+
+/* One texel of the table with another quadrature resolution (for the checks). */
+ORC_API void orc_atmo_multiscattering_texel(const float *tlut, int px, int py, int sq, int steps, float *out3) {
+    v3 v = multiscattering_texel(tlut, px, py, sq, steps);
+    out3[0] = v.x;
+    out3[1] = v.y;
+    out3[2] = v.z;
 }
 
 /* sky_view_lut_compute + raymarchScattering (atmosphere.slang:267-397):
