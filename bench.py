@@ -1,28 +1,43 @@
-"""Benchmark: plain texture-space path tracer on one MI355X.
+"""Benchmark: the texture-space path tracer on one MI355X.
 
-Default workload (one "step" = one progressive iteration over every texel):
-the seeded synthetic 1.21M-triangle scene (512x512-quad value-noise
-heightfield, 64x64 extruded boxes, 500 level-3 icospheres; sun at 35 deg +
-sky 0.3), texels_per_unit 8, min_resolution 2, max_resolution 16, budget
-16,777,216 slots, 1 sample/texel, 3 bounces, FP32.  ``--workload cornell``
-runs the small Cornell box instead.  The default run also measures the
-learned radiance cache against the plain tracer at equal RMSE on the Cornell
-box (``learned_vs_plain``; ``--workload learned`` runs only that) and, on the
-1.21M-triangle scene itself, both learned forms (one bounce into the cache,
-and the cache queried at the texel) against a 512-frame plain bake
-(``learned_vs_plain_at_scale``).
+The metric is effective converged texel-path samples/s at equal RMSE
+against a 4096-frame plain bake, learned versus plain.  The default run
+(``python bench.py``) measures it on the configuration it is defined on: the
+Cornell box (36 triangles, texels_per_unit 16, res 4-64, 1 spp, 3 bounces)
+with the learned radiance cache in its specified form (65,536 training
+suffixes per frame, paths ending in the cache after one bounce).  ``value``
+is the plain samples that reach the learned RMSE after 256 frames over the
+learned wall time, the learned frames priced by exactly ``--steps`` timed
+frames (after ``--warmup`` more); ``value_vs_plain`` is its ratio to the
+plain tracer's own samples/s.
 
-Prints one JSON line.  ``value`` is texel-path samples/s (slot updates per
-second) with the scene, layout and cache resident in HBM, timed with HIP
-events on the launch stream around the production path (the double-sided
-texels' paths on a second stream beside the rest).  A profiled replay of the
-same iterations then brackets every tracer kernel launch with HIP events on
-its stream (``nan_tracer_profile``, one-stream order, so every launch is
-timed alone); work counts come from a counted replay.  The roofline entry
-is for the stage with the largest time (the closest-hit traversal kernel
-``wf_extend`` in wavefront mode): its algorithmic bytes (64 B per counted
-node visit, 48 B per triangle test, the queued ray and the hit record) over
-its launch time in the profiled replay.
+Beside it, in the same JSON line:
+
+* ``plain_at_scale``: the plain tracer on the seeded synthetic 1.21M-triangle
+  scene (512x512-quad value-noise heightfield, 64x64 extruded boxes, 500
+  level-3 icospheres; sun at 35 deg + sky 0.3; texels_per_unit 8, res 2-16,
+  budget 16,777,216), ``--steps`` iterations timed with HIP events on the
+  launch stream around the production path (the double-sided texels' paths
+  on a second stream beside the rest), then a profiled replay with HIP
+  events around every tracer launch (one-stream order, so every launch is
+  timed alone) and a counted replay for the work counts.
+* ``roofline``: the dominant kernel of that leg, the closest-hit traversal
+  ``wf_extend``: algorithmic bytes (64 B per counted node visit, 48 B per
+  triangle test, the queued ray and hit record) over its launch time in the
+  profiled replay, against 8 TB/s; with HBM traffic, L2 and issue counters
+  from the committed rocprofv3 summary, and ``bound`` naming the resource
+  closest to saturation.
+* ``learned_vs_plain_at_scale``: both learned forms on the 1.21M-triangle
+  scene against a 4096-frame bake (the texel-query form is a biased variant
+  outside the specification).
+* ``combined``: the 4.0M-triangle scene as a batch of one under hipGraph
+  replay (latency percentiles, Mrays/s), a 3840x2160 resolve, the learned
+  forms against a 4096-frame bake.
+* ``cpu_baseline``: the oracle tracer on a strided sample of the
+  1.21M-triangle scene on the host's cores, and the reference's own CPU
+  ray path (tinybvh) beside it.
+
+``--workload cornell|batch|learned|combined`` runs one leg alone.
 """
 
 from __future__ import annotations
@@ -40,7 +55,7 @@ if str(ROOT) not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 # bumped whenever the tracer kernels change, so stale rocprof summaries are ignored
-KERNEL_VERSION = 5
+KERNEL_VERSION = 6
 METRIC = "texel-path samples/s (Mrays/s) at equal RMSE vs 4096-spp bake; learned vs plain"
 
 # algorithmic bytes (see the module docstring)
@@ -197,12 +212,17 @@ def _plain_frames_for(curve: dict, r: float) -> float:
 
 
 def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, plain_curve_frames: int = 2048,
-                        train_samples: int = 65536, mlp_queries: int = 1 << 22) -> dict:
+                        train_samples: int = 65536, mlp_queries: int = 1 << 22, steps: int = 10,
+                        warmup: int = 2) -> dict:
     """Learned radiance cache against the plain tracer on the Cornell box
-    (texels_per_unit 16, res 4-64, 1 spp, 3 bounces): RMSE of both against a
-    plain bake, the plain frames that reach each learned RMSE, the
-    effective converged texel-path samples/s, and the fused MLP forward
-    kernel's MFMA rate on a large query batch."""
+    (texels_per_unit 16, res 4-64, 1 spp, 3 bounces), the specified form
+    (paths end in the cache after one bounce): RMSE of both against a plain
+    bake after 16, 64 and 256 frames, the plain frames that reach each
+    learned RMSE and the effective converged texel-path samples/s (those
+    plain samples over the learned wall time).  After the RMSE run,
+    `warmup` more learned frames, then exactly `steps` timed ones: their
+    mean frame time prices the learned frames in `value`.  Also the fused
+    MLP forward kernel's MFMA rate on a large query batch."""
     import numpy as np
     import torch
 
@@ -269,6 +289,19 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
         }
     stages = tracer.stage_times()
     tracer.profile(False)
+    # the timed steps: learned frames continuing the run above
+    it = frames[-1]
+    for _ in range(warmup):
+        tracer.execute_learned(enc, learned, it, cache, train_samples=train_samples)
+        it += 1
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(steps):
+        tracer.execute_learned(enc, learned, it, cache, train_samples=train_samples)
+        it += 1
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    timed_ms = ev0.elapsed_time(ev1) / steps
     # fused MLP forward alone on a large batch of queries inside the scene box
     rng = np.random.default_rng(5)
     lo, hi = scene.bounds()
@@ -291,8 +324,13 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
         "plain_samples_per_s": slots / (plain_ms * 1e-3),
         "plain_rmse": {str(k): v for k, v in curve.items()},
         "learned": out_frames,
-        "value": last["effective_samples_per_s"],
-        "speedup_vs_plain": last["speedup_vs_plain"],
+        "form": "one bounce, the specified estimator: E = sun + NEE(emitters, MIS) + (az/pdf)(Le(y1) + "
+                "max(0, cache(y1)))",
+        "timed_ms_per_frame": timed_ms,
+        "value_frames": frames[-1],
+        "bake_frames": bake_frames,
+        "value": last["plain_frames_for_equal_rmse"] * slots / (frames[-1] * timed_ms * 1e-3),
+        "speedup_vs_plain": last["plain_frames_for_equal_rmse"] * plain_ms / (frames[-1] * timed_ms),
         "learned_stage_ms_per_frame": {k: v[0] / frames[-1] for k, v in stages.items()},
         "mlp_roofline": {
             "bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
@@ -307,15 +345,25 @@ LEARNED_FORMS = {
     "one_bounce": (False, 65536, 8),
     "texel_query": (True, 131072, 16),
 }
+LEARNED_FORM_NOTES = {
+    "one_bounce": "the specified estimator: paths end in the cache after one bounce (unbiased for a converged "
+                  "cache)",
+    "texel_query": "a biased variant outside the specification: paths end in the cache at the texel itself (no "
+                   "bounce ray), so the fit's bias stays in every frame",
+}
 LEARNED_FRAMES = (1, 2, 4, 8, 16, 64)
 
 
-def learned_forms_vs_plain(device, scene, tracer, rmse, curve: dict, plain_ms: float, frames=LEARNED_FRAMES) -> dict:
+def learned_forms_vs_plain(device, scene, tracer, rmse, curve: dict, plain_ms: float, frames=LEARNED_FRAMES,
+                           graph: bool = False) -> dict:
     """Each form of LEARNED_FORMS from a fresh cache: its RMSE (``rmse``
     against the plain bake) after each frame count in ``frames``, the plain
     frames that reach it (``curve``: plain RMSE per frame count) and the
     equal-RMSE speedup (plain frames x ``plain_ms`` over the learned time),
-    plus the learned frame's stage times."""
+    plus the learned frame's stage times.  With ``graph`` every learned frame
+    is a replay of one captured iteration in device-frame mode (first frame
+    included), and the per-frame latency percentiles are reported."""
+    import numpy as np
     import torch
 
     from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
@@ -330,32 +378,48 @@ def learned_forms_vs_plain(device, scene, tracer, rmse, curve: dict, plain_ms: f
         learned = torch.zeros((slots, 4), dtype=torch.float32, device=device.torch_device)
         res = {}
         elapsed = 0.0
+        lat = []
+        g = None
+        if graph:
+            g = tracer.capture_learned(enc, learned, cache, 0, reset=True, train_samples=n_train,
+                                       texel_query=texel_query)
         for it in range(max(frames)):
             ev0.record(stream)
-            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=n_train,
-                                   texel_query=texel_query)
+            if g is not None:
+                g.replay()
+            else:
+                tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=n_train,
+                                       texel_query=texel_query)
             ev1.record(stream)
             torch.cuda.synchronize()
-            elapsed += ev0.elapsed_time(ev1)
+            lat.append(ev0.elapsed_time(ev1))
+            elapsed += lat[-1]
             if it + 1 in frames:
                 r = rmse(learned)
                 m = _plain_frames_for(curve, r)
                 res[str(it + 1)] = {"rmse": r, "plain_frames_for_equal_rmse": m,
                                     "learned_ms_per_frame": elapsed / (it + 1),
                                     "speedup_vs_plain": m * plain_ms / elapsed}
+        if g is not None:
+            tracer.clear_device_frame()
+            del g
         tracer.profile(True)
         for it in range(2):
             tracer.execute_learned(enc, learned, max(frames) + it, cache, train_samples=n_train,
                                    texel_query=texel_query)
         stages = {k: v[0] / 2 for k, v in tracer.stage_times().items()}
         tracer.profile(False)
-        forms[name] = {"train_samples": n_train, "minibatches": mb, "frames": res, "stage_ms_per_frame": stages}
+        forms[name] = {"form": LEARNED_FORM_NOTES[name], "train_samples": n_train, "minibatches": mb, "frames": res,
+                       "stage_ms_per_frame": stages}
+        if graph:
+            forms[name]["graph_replays"] = True
+            forms[name]["latency_ms"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}
         log(f"learned {name}: " + ", ".join(f"{k}: x{v['speedup_vs_plain']:.2f}" for k, v in res.items()))
         del cache, learned
     return forms
 
 
-def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 512, frames=LEARNED_FRAMES) -> dict:
+def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 4096, frames=LEARNED_FRAMES) -> dict:
     """Both learned forms against the plain tracer on the workload's own
     scene at equal RMSE against a plain bake (``learned_forms_vs_plain``).
     ``one_bounce`` is the configured form (65,536 suffixes per frame, paths
@@ -371,7 +435,7 @@ def learned_at_scale(device, scene, layout, tracer, bake_frames: int = 512, fram
     bake = torch.zeros((slots, 4), dtype=torch.float32, device=dev)
     for it in range(bake_frames):  # its own RNG streams: iterations 1,000,000 + k
         tracer.execute(enc, bake, 1_000_000 + it, reset=(it == 0))
-        if it % 128 == 127:
+        if it % 512 == 511:
             torch.cuda.synchronize()
             log(f"scale bake {it + 1}/{bake_frames}")
     b = bake[:, :3].double()
@@ -432,24 +496,36 @@ COMBINED_LAYOUT = dict(texels_per_unit=2.0, min_resolution=1, max_resolution=4, 
 
 
 def bench_combined(args) -> int:
+    """The combined full-size leg alone (see combined_leg), one JSON line."""
+    from nan_amd.device import Device
+
+    res = combined_leg(args, Device(0))
+    out = {"metric": METRIC, "value": res["plain_samples_per_s"], "unit": "texel-path samples/s", "n_gpus": 1,
+           "steps": res["frames_timed"], "warmup": args.warmup, "ms_per_step": res["latency_ms"]["mean"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": res["workload"], "parallelism": "single GPU"}, "combined": res,
+           "roofline": res["mlp_roofline"], "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def combined_leg(args, device) -> dict:
     """Combined full-size run: the synthetic generator scaled 3.3x (4.0M
     triangles; 45% of object materials double-sided, terrain single-sided, so
     20.5% of the 16.7M slots are back slots), traced by the batched persistent
     wavefront with every frame a replayed hipGraph (per-frame latency
-    percentiles), a 3840x2160 resolve, and the learned form on the same scene
-    against a plain bake."""
+    percentiles), a 3840x2160 resolve, and the learned forms on the same
+    scene against a 4096-frame plain bake."""
     import numpy as np
     import torch
 
     from nan_amd import _native
     from nan_amd.autotune import autotune
     from nan_amd.batch import SceneBatch
-    from nan_amd.device import Device
     from nan_amd.passes import MeshColorsResolve
     from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
     from nan_amd.scenes import synthetic_scene
 
-    device = Device(0)
     dev = device.torch_device
     t0 = time.perf_counter()
     node = synthetic_scene(0, 3.3, double_sided=0.45, single_sided_terrain=True)
@@ -463,13 +539,13 @@ def bench_combined(args) -> int:
     tune = autotune(batch, enc, space={"blocks_per_cu": (0, 4), "lds_stack": (16, 32), "refill_steps": (16, 32)},
                     frames=1)
     log(f"autotuned {tune['best']}")
-    batch.execute(enc, 0, reset=True)
-    torch.cuda.synchronize()
-    graph = batch.capture(enc, 1)
+    for p in batch.payloads:
+        p.zero_()
+    graph = batch.capture(enc, 0, reset=True)  # the first frame itself is a replay
     for _ in range(args.warmup):
         graph.replay()
     torch.cuda.synchronize()
-    frames = args.steps
+    frames = max(args.steps, 64)  # enough frames for the latency percentiles
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(frames)]
     for a, b in evs:
         a.record(stream)
@@ -479,7 +555,7 @@ def bench_combined(args) -> int:
     lat = np.array([a.elapsed_time(b) for a, b in evs])
     batch.clear_device_frame()
     counters = torch.zeros(8, dtype=torch.int64, device=dev)
-    scratch_iter = 1 + args.warmup
+    scratch_iter = args.warmup + frames
     batch.execute(enc, scratch_iter, counters=counters)  # counts of one frame (payload keeps accumulating)
     torch.cuda.synchronize()
     c = counters.cpu().numpy().astype(np.int64)
@@ -522,8 +598,35 @@ def bench_combined(args) -> int:
         tracer.execute(enc, plain, it, reset=(it == 0))
         if (it + 1) & it == 0:
             curve[it + 1] = rmse(plain)
-    forms = learned_forms_vs_plain(device, scene, tracer, rmse, curve, mean_ms)
-    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig())
+    forms = learned_forms_vs_plain(device, scene, tracer, rmse, curve, mean_ms, graph=True)
+    # one captured graph holding the batched plain frame and the learned
+    # frame (the specified one-bounce form) of the same scene, each advancing
+    # its own device frame per replay
+    n_train, mb = LEARNED_FORMS["one_bounce"][1:]
+    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=mb))
+    learned = torch.zeros_like(bake)
+    tracer.reserve_learned(cache, n_train)
+    batch.set_device_frame(2_000_000, True)
+    tracer.set_device_frame(0, True)
+    torch.cuda.synchronize()
+    both = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(both):
+        batch.execute(enc, 0)
+        tracer.execute_learned(enc, learned, 0, cache, train_samples=n_train)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(frames)]
+    for a, b in evs:
+        a.record(stream)
+        both.replay()
+        b.record(stream)
+    torch.cuda.synchronize()
+    lat_both = np.array([a.elapsed_time(b) for a, b in evs])
+    batch.clear_device_frame()
+    tracer.clear_device_frame()
+    del both
+    one_graph = {"frames": frames, "learned_rmse": rmse(learned),
+                 "latency_ms": {"p50": float(np.percentile(lat_both, 50)), "p99": float(np.percentile(lat_both, 99)),
+                                "mean": float(lat_both.mean())}}
+    log(f"plain + learned in one graph: p50 {one_graph['latency_ms']['p50']:.2f} ms")
     q = slots  # one cache query per texel sample
     rng = np.random.default_rng(3)
     lo, hi = scene.bounds()
@@ -532,31 +635,18 @@ def bench_combined(args) -> int:
     alb = torch.rand((q, 3), device=dev)
     mlp_ms = cache.time_forward(pos, nrm, alb, repeats=5, encoder=enc)
     tflops = MLP_FLOPS_PER_QUERY * q / (mlp_ms * 1e-3) / 1e12
-    out = {
-        "metric": METRIC,
-        "value": slots / (mean_ms * 1e-3),
-        "unit": "texel-path samples/s",
-        "n_gpus": 1,
-        "steps": frames,
-        "warmup": args.warmup,
-        "ms_per_step": mean_ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": ("combined: synthetic generator scaled 3.3x (seed 0, 45% of object materials "
-                         "double-sided, terrain single-sided), layout texels_per_unit 2, res 1-4, budget "
-                         "16,777,216; batched persistent wavefront (batch of one scene), every frame a replayed "
-                         "hipGraph; 1 spp, 3 bounces; resolve 3840x2160; learned form on the same scene"),
-            "triangles": node.triangle_count,
-            "payload_slots": slots,
-            "back_slots": slots - layout.total_surface_texels,
-            "layout": COMBINED_LAYOUT,
-            "parallelism": "single GPU",
-        },
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+    return {
+        "workload": ("combined: synthetic generator scaled 3.3x (seed 0, 45% of object materials double-sided, "
+                     "terrain single-sided), layout texels_per_unit 2, res 1-4, budget 16,777,216; batched "
+                     "persistent wavefront (batch of one scene), every frame a replayed hipGraph; 1 spp, "
+                     "3 bounces; resolve 3840x2160; learned forms on the same scene"),
+        "triangles": node.triangle_count,
+        "payload_slots": slots,
+        "back_slots": slots - layout.total_surface_texels,
+        "layout": COMBINED_LAYOUT,
+        "frames_timed": frames,
+        "plain_samples_per_s": slots / (mean_ms * 1e-3),
+        "latency_ms": {"mean": mean_ms, "p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "min": float(lat.min()), "max": float(lat.max())},
         "mrays_per_s": (ext + shd) / (mean_ms * 1e-3) / 1e6,
         "rays_per_step": {"extend": ext, "shadow": shd},
@@ -565,13 +655,11 @@ def bench_combined(args) -> int:
         "resolve_3840x2160_ms": resolve_ms,
         "learned_vs_plain": {"bake_frames": args.bake_frames, "plain_rmse": {str(k): v for k, v in curve.items()},
                              "forms": forms},
-        "roofline": {"bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
-                     "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / MFMA_F16_PEAK_TFLOPS},
-        "cpu_baseline": None,
+        "plain_and_learned_one_graph": one_graph,
+        "mlp_roofline": {"bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
+                         "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tflops / MFMA_F16_PEAK_TFLOPS},
     }
-    print(json.dumps(out), flush=True)
-    return 0
 
 
 def bench_batch(args) -> int:
@@ -650,13 +738,16 @@ def main() -> int:
                     default="synthetic")
     ap.add_argument("--bake-frames", type=int, default=4096, help="plain bake length of --workload combined")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-cache comparison")
-    ap.add_argument("--scale-bake-frames", type=int, default=512,
+    ap.add_argument("--scale-bake-frames", type=int, default=4096,
                     help="plain bake length of the learned comparison on the workload's own scene")
+    ap.add_argument("--no-combined", action="store_true", help="skip the combined full-size leg")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (counter calibration)")
+    ap.add_argument("--calibrate-valu", action="store_true",
+                    help="only run the VALU-bound FMA kernel (issue-rate calibration of SQ_INSTS_VALU)")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -671,19 +762,23 @@ def main() -> int:
         print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n, "copies": 3}))
         return 0
 
+    if args.calibrate_valu:
+        import ctypes
+
+        from nan_amd import _native
+
+        ms, insts = ctypes.c_double(), ctypes.c_double()
+        _native.check(_native.lib().nan_calibrate_valu(20000, ctypes.byref(ms), ctypes.byref(insts)),
+                      "nan_calibrate_valu")
+        print(json.dumps({"calibrate_valu_ms": ms.value, "fma_wave_insts": insts.value,
+                          "wave_insts_per_s": insts.value / (ms.value * 1e-3)}))
+        return 0
+
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the path shards by texel range with no "
                                    "exchange during tracing, and multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         return 2
-
-    import numpy as np
-    import torch
-
-    from nan_amd import _native
-    from nan_amd.device import Device
-    from nan_amd.scene import Scene
-    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
 
     if args.workload == "batch":
         return bench_batch(args)
@@ -692,12 +787,27 @@ def main() -> int:
     if args.workload == "combined":
         return bench_combined(args)
 
+    return bench_default(args)
+
+
+def plain_leg(args, device) -> tuple[dict, tuple]:
+    """The plain tracer on the workload's scene: K timed iterations of the
+    production path, a profiled replay (every launch timed alone, one-stream
+    order) for the stage times and the roofline's launch duration, and a
+    counted replay for the work counts.  Returns the plain block and
+    (node, layout, scene, tracer, stages, shadow_rays)."""
+    import numpy as np
+    import torch
+
+    from nan_amd import _native
+    from nan_amd.scene import Scene
+    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
     t0 = time.perf_counter()
     node, layout, layout_kw, desc = build_workload(args.workload)
     log(f"scene {node.triangle_count} tris, {len(node.instances)} instances; layout "
         f"{layout.total_surface_texels} surface + {layout.total_payload_count - layout.total_surface_texels} back "
         f"slots ({time.perf_counter() - t0:.1f} s)")
-    device = Device(0)
     t0 = time.perf_counter()
     scene = Scene(device, node)
     log(f"BVH: {scene.info.node_count} nodes, {scene.info.leaf_count} leaves, max stack {scene.info.max_stack}, "
@@ -779,82 +889,159 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     kernel_name = {"extend": "wf_extend", "shadow": "wf_shadow", "megakernel": "tst_megakernel"}[dominant]
 
-    out = {
-        "metric": METRIC,
+    roofline = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "kernel": kernel_name,
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+        "avg_launch_ms": avg_launch_ms,
+        "launches_per_step": dom_launches / args.steps,
+    }
+    # HBM traffic, L2 and issue counters of this kernel on this workload, from
+    # the committed rocprofv3 summary (copy-calibrated bytes, FMA-calibrated
+    # VALU issue); absent when the summary is for another kernel version or
+    # workload.  The algorithmic bytes count every node and triangle read,
+    # most of them served by L2 and the MALL, so `frac` can reach 1 while HBM
+    # runs far below it: `bound` names the resource the counters show closest
+    # to saturation.
+    prof = ROOT / "profiles" / f"traffic_{kernel_name}_{args.workload}.json"
+    if prof.exists():
+        rec = json.loads(prof.read_text())
+        if str(layout.total_payload_count) in rec.get("workload", "") and rec.get("kernel_version") == KERNEL_VERSION:
+            launch_s = avg_launch_ms * 1e-3
+            roofline.update({
+                "bound": rec.get("bound", "hbm"),
+                "traffic": rec["traffic_bytes_per_launch"],
+                "hbm_gbs": rec["traffic_bytes_per_launch"] / launch_s / 1e9,
+                "hbm_frac": rec["traffic_bytes_per_launch"] / launch_s / 1e9 / HBM_PEAK_GBS,
+                "l2_hit_rate": rec.get("l2_hit_rate"),
+                "l2_read_requests_per_s": (rec["l2_read_requests_per_launch"] / launch_s
+                                           if "l2_read_requests_per_launch" in rec else None),
+                "valu_issue_frac": rec.get("valu_issue_frac"),
+                "valu_dual_issue_share": rec.get("valu_dual_issue_share"),
+                "wave_wait_frac": rec.get("wait_frac"),
+                "profile": rec.get("source"),
+            })
+    plain = {
+        "workload": desc,
         "value": value,
         "unit": "texel-path samples/s",
-        "n_gpus": 1,
+        "ms_per_step": total_ms / args.steps,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": total_ms / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": desc,
-            "triangles": node.triangle_count,
-            "instances": len(node.instances),
-            "payload_slots": layout.total_payload_count,
-            "surface_texels": layout.total_surface_texels,
-            "layout": layout_kw,
-            "samples_per_texel": 1,
-            "max_bounces": 3,
-            "tracer": args.mode,
-            "parallelism": "single GPU",
-        },
+        "triangles": node.triangle_count,
+        "instances": len(node.instances),
+        "payload_slots": layout.total_payload_count,
+        "surface_texels": layout.total_surface_texels,
+        "layout": layout_kw,
+        "samples_per_texel": 1,
+        "max_bounces": 3,
+        "tracer": args.mode,
         "mrays_per_s": mrays,
         "rays_per_step": {"extend": ext, "shadow": shd},
         "traversal_per_step": {"extend_node_visits": nodes, "extend_tri_tests": tris,
                                "shadow_node_visits": snodes, "shadow_tri_tests": stris},
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
         "profiled_ms_per_step": profiled_ms / args.steps,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": kernel_name,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": avg_launch_ms,
-            "launches_per_step": dom_launches / args.steps,
-        },
-        "cpu_baseline": None,
+        "roofline": roofline,
     }
-    # HBM traffic of this kernel on this workload, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); null when the summary
-    # is for another kernel variant or workload
-    prof = ROOT / "profiles" / f"traffic_{kernel_name}_{args.workload}.json"
-    if prof.exists():
-        rec = json.loads(prof.read_text())
-        if str(layout.total_payload_count) in rec.get("workload", "") and rec.get("kernel_version") == KERNEL_VERSION:
-            out["roofline"]["traffic"] = rec["traffic_bytes_per_launch"]
-            # the algorithmic bytes count every node and triangle read, most of
-            # them served by L2 and the MALL: the HBM rate is the traffic's
-            out["roofline"]["hbm_gbs"] = rec["traffic_bytes_per_launch"] / (avg_launch_ms * 1e-3) / 1e9
-            out["roofline"]["l2_hit_rate"] = rec.get("l2_hit_rate")
-            out["roofline"]["traffic_source"] = rec.get("source")
-    if not args.no_learned and args.workload == "synthetic":
+    return plain, (node, layout, scene, tracer, stages, shd)
+
+
+def host_cpu() -> dict:
+    """The host's CPU model and counts (the CPU baseline's threads are the
+    process's CPU affinity, at most 16: one GPU's share of the host)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def bench_default(args) -> int:
+    """The default run: the metric on its configuration (the learned form
+    against the plain tracer on the Cornell box, learned_vs_plain), the plain
+    tracer at scale with its roofline, both learned forms at scale, the
+    combined full-size leg and the CPU baselines."""
+    from nan_amd.device import Device
+
+    device = Device(0)
+    plain, (node, layout, scene, tracer, stages, shd) = plain_leg(args, device)
+    out_learned = at_scale = combined = None
+    if not args.no_learned:
         t0 = time.perf_counter()
-        res = learned_measurement(device)
+        out_learned = learned_measurement(device, steps=args.steps, warmup=args.warmup)
         log(f"learned measurement {time.perf_counter() - t0:.1f} s")
-        out["learned_vs_plain"] = res
+        if args.workload == "synthetic":
+            t0 = time.perf_counter()
+            at_scale = learned_at_scale(device, scene, layout, tracer, bake_frames=args.scale_bake_frames)
+            log(f"learned at scale {time.perf_counter() - t0:.1f} s")
+    if not args.no_combined and args.workload == "synthetic":
         t0 = time.perf_counter()
-        del payload
-        out["learned_vs_plain_at_scale"] = learned_at_scale(device, scene, layout, tracer,
-                                                             bake_frames=args.scale_bake_frames)
-        log(f"learned at scale {time.perf_counter() - t0:.1f} s")
+        combined = combined_leg(args, device)
+        log(f"combined leg {time.perf_counter() - t0:.1f} s")
+    if out_learned is not None:
+        head = {
+            "value": out_learned["value"],
+            "unit": "effective converged texel-path samples/s",
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": out_learned["timed_ms_per_frame"],
+            "dtype": "f32 tracing / f16 MFMA network, f32 accumulate",
+            "config": {"workload": "learned vs plain at equal RMSE: " + out_learned["workload"],
+                       "frames": out_learned["value_frames"], "bake_frames": out_learned["bake_frames"],
+                       "payload_slots": out_learned["payload_slots"], "parallelism": "single GPU"},
+            "value_vs_plain": out_learned["speedup_vs_plain"],
+        }
+    else:
+        head = {
+            "value": plain["value"], "unit": plain["unit"], "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": plain["ms_per_step"], "dtype": "f32",
+            "config": {"workload": plain["workload"], "parallelism": "single GPU"},
+        }
+    out = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": head["unit"],
+        "n_gpus": 1,
+        "steps": head["steps"],
+        "warmup": head["warmup"],
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head["dtype"],
+        "data": "synthetic",
+        "config": head["config"],
+    }
+    if "value_vs_plain" in head:
+        out["value_vs_plain"] = head["value_vs_plain"]
+        out["learned_vs_plain"] = out_learned
+    out["plain_samples_per_s"] = plain["value"]
+    out["plain_at_scale"] = {k: v for k, v in plain.items() if k != "roofline"}
+    out["roofline"] = plain["roofline"]
+    if at_scale is not None:
+        out["learned_vs_plain_at_scale"] = at_scale
+    if combined is not None:
+        out["combined"] = combined
+    out["cpu_baseline"] = None
     if not args.no_cpu_baseline:
-        threads = len(os.sched_getaffinity(0))
-        threads = min(threads, 16)
+        threads = min(len(os.sched_getaffinity(0)), 16)
         log(f"cpu baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(node, layout, threads, args.cpu_seconds)
+        out["cpu_baseline"]["host"] = host_cpu()
         ref = tinybvh_baseline(node, threads)
         if ref is not None:
             ref["gpu_shadow_mrays_per_s"] = shd / (stages["shadow"][0] / args.steps * 1e-3) / 1e6
+            ref["host"] = host_cpu()
             out["cpu_baseline_reference_visibility"] = ref
     print(json.dumps(out), flush=True)
     return 0

@@ -170,6 +170,15 @@ NAN_API int nan_tracer_stage_times(NanTracer *tracer, double *ms, uint32_t *laun
  * counterpart (autotuner hook). */
 NAN_API int nan_tracer_tune(NanTracer *tracer, uint32_t blocks_per_cu, uint32_t lds_stack, uint32_t refill_steps);
 
+/* Measurement hook (no reference counterpart): a VALU-bound kernel of
+ * independent FP32 FMA chains (no memory traffic), `trips` loop trips of 32
+ * FMAs per thread, launched to fill every SIMD and timed with HIP events on
+ * the default stream (the second of two launches).  Returns the
+ * milliseconds and the FMA wave-instructions of the timed launch: their
+ * ratio is the VALU issue ceiling of plain FP32 code, the denominator of the
+ * traversal kernels' measured SQ_INSTS_VALU rate. */
+NAN_API int nan_calibrate_valu(uint32_t trips, double *ms_out, double *fma_wave_insts_out);
+
 /* Many independent scenes traced by one set of launches: concatenates the
  * scenes' BVHs, shading records and layouts (each tracer must be over the
  * scene at the same index) into a batched tracer (wavefront mode).  Each
@@ -185,10 +194,11 @@ NAN_API int nan_batch_set_payloads(NanTracer *batch, void *const *payloads);
  * max_bounces must be at least the largest. */
 NAN_API int nan_batch_set_bounces(NanTracer *batch, const uint32_t *max_bounces);
 
-/* Device-frame mode for graph replay: while enabled, nan_tst_execute takes the
- * iteration and reset flag from a device counter (set here to iteration/reset)
- * and advances it at the end of each call, so one captured hipGraph of a call
- * replays successive iterations.  No reference counterpart. */
+/* Device-frame mode for graph replay: while enabled, nan_tst_execute and
+ * nan_tst_execute_learned take the iteration and reset flag from a device
+ * counter (set here to iteration/reset) and advance it at the end of each
+ * call, so one captured hipGraph of a call replays successive iterations.
+ * Wavefront mode only.  No reference counterpart. */
 NAN_API int nan_tracer_device_frame(NanTracer *tracer, int enable, uint32_t iteration, uint32_t reset);
 
 /* One progressive iteration over every texel of every instance

@@ -95,7 +95,7 @@ EXPORTED_SYMBOLS = (
     "nan_nrc_stats", "nan_nrc_time_forward", "nan_tst_execute_learned", "nan_surface_filter",
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
-    "nan_tst_learned_records", "nan_tst_learned_reserve", "nan_scene_set_russian_roulette",
+    "nan_tst_learned_records", "nan_tst_learned_reserve", "nan_scene_set_russian_roulette", "nan_calibrate_valu",
 )
 
 _LIB = None
@@ -150,6 +150,7 @@ def _declare(lib) -> None:
         "nan_nrc_set_params": (I, [vp, vp, vp]),
         "nan_nrc_infer": (I, [vp, vp, vp, vp, vp, U32, vp]),
         "nan_nrc_infer_noalloc": (I, [vp, vp, vp, vp, vp, U32, vp]),
+        "nan_calibrate_valu": (I, [U32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "nan_nrc_train": (I, [vp, vp, vp, vp, vp, U32, vp, vp]),
         "nan_nrc_forward_raw": (I, [vp, vp, vp, vp, vp, U32, vp]),
         "nan_nrc_backward_raw": (I, [vp, vp, vp, vp, vp, U32, vp, vp, vp]),

@@ -250,6 +250,29 @@ __global__ void tonemap_kernel(const float4 *in, float4 *out, uint64_t n, float 
     out[i] = make_float4(rec709(t.x), rec709(t.y), rec709(t.z), 1.0f);
 }
 
+// Issue-rate calibration: eight independent FP32 FMA chains per thread, four
+// rounds per loop trip (32 v_fma_f32 per trip), no memory traffic but one
+// store; with every SIMD full the instruction rate is the VALU issue ceiling
+// of plain (non-packed) FP32 code such as the traversal kernels.
+constexpr int kValuChains = 8;
+constexpr int kValuUnroll = 4;
+
+__global__ __launch_bounds__(256) void valu_calibration_kernel(float *out, uint32_t trips, float a, float b) {
+    float v[kValuChains];
+#pragma unroll
+    for (int k = 0; k < kValuChains; ++k) v[k] = (float)(threadIdx.x + k);
+    for (uint32_t t = 0; t < trips; ++t) {
+#pragma unroll
+        for (int u = 0; u < kValuUnroll; ++u)
+#pragma unroll
+            for (int k = 0; k < kValuChains; ++k) v[k] = __builtin_fmaf(v[k], a, b);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kValuChains; ++k) s += v[k];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
@@ -862,6 +885,36 @@ NAN_API int nan_tonemap(const float *input, float *output, uint64_t n, float exp
     return check_launch("tonemap");
 }
 
+
+NAN_API int nan_calibrate_valu(uint32_t trips, double *ms_out, double *fma_wave_insts_out) {
+    if (!ms_out || !fma_wave_insts_out || trips == 0) return fail(nullptr, 0, "nan_calibrate_valu: bad argument");
+    int dev = 0, cus = 0, per_cu = 0;
+    HIP_TRY(hipGetDevice(&dev), nullptr, 0);
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, valu_calibration_kernel, 256, 0), nullptr, 0);
+    uint32_t blocks = (uint32_t)(cus * std::max(per_cu, 1));
+    float *out = nullptr;
+    HIP_TRY(hipMalloc((void **)&out, (size_t)blocks * 256 * sizeof(float)), nullptr, 0);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(valu_calibration_kernel, dim3(blocks), dim3(256), 0, nullptr, out, trips, 0.999f, 1e-3f);
+        hipEventRecord(a, nullptr);
+        hipLaunchKernelGGL(valu_calibration_kernel, dim3(blocks), dim3(256), 0, nullptr, out, trips, 0.999f, 1e-3f);
+        hipEventRecord(b, nullptr);
+        e = hipEventSynchronize(b);
+    }
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    hipFree(out);
+    HIP_TRY(e, nullptr, 0);
+    *ms_out = ms;
+    *fma_wave_insts_out = (double)blocks * (256 / 64) * (double)trips * kValuUnroll * kValuChains;
+    return check_launch("valu calibration");
+}
 
 NAN_API int nan_tracer_tune(NanTracer *tracer, uint32_t blocks_per_cu, uint32_t lds_stack, uint32_t refill_steps) {
     if (!tracer) return fail(nullptr, 0, "nan_tracer_tune: null tracer");

@@ -250,6 +250,7 @@ struct LearnedIO {
     uint32_t *tr_valid;
     float *q_pos, *q_nrm, *q_alb, *q_w;  // cache queries [n][3], weight (0: none)
     float4 *q_pred;
+    const uint32_t *frame;  // device [iteration, reset] in device-frame mode, else null (host values)
 };
 
 // Learned forms on the wavefront queues (nan_wavefront.hip), 1 sample per
@@ -266,6 +267,10 @@ int wavefront_learned_side(NanTracer *t, const LearnedIO &io, uint32_t iteration
                            uint32_t *count, hipStream_t stream);
 int wavefront_learned_finish(NanTracer *t, const LearnedIO &io, float4 *payload, uint32_t side, uint32_t reset,
                              bool at_x0, bool ramped, hipStream_t stream);
+// Device-frame mode: the tracer's device [iteration, reset] (null when off)
+// and the end-of-frame step to the next iteration.
+const uint32_t *wavefront_frame(const NanTracer *t);
+void wavefront_advance_frame(NanTracer *t, hipStream_t stream);
 
 
 }  // namespace nanx

@@ -22,6 +22,7 @@ struct LearnedDev {
     float *q_pos, *q_nrm, *q_alb, *q_w;  // cache queries [slots * spp][3], weight (0: none)
     float4 *q_pred;
     float4 *partial;  // per payload slot: sum over samples of the terms outside the cache
+    const uint32_t *frame;  // device [iteration, reset] in device-frame mode, else null (host values)
 };
 
 struct LearnedBuffers {
@@ -230,6 +231,7 @@ __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayo
                                                               uint32_t spp, uint32_t max_bounces, uint32_t n,
                                                               uint32_t flags, LearnedDev B,
                                                               unsigned long long *counters) {
+    if (B.frame) iteration = B.frame[0];
     __shared__ uint32_t stack[STACK * BLOCK];
     uint32_t *col = stack + threadIdx.x;
     PathCounters pc{0, 0, 0, 0, 0, 0, 0};
@@ -296,6 +298,7 @@ template <int STACK, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLayout Ly, uint32_t iteration,
                                                                uint32_t spp, uint32_t flags, LearnedDev B,
                                                                unsigned long long *counters) {
+    if (B.frame) iteration = B.frame[0];
     __shared__ uint32_t stack[STACK * BLOCK];
     uint32_t *col = stack + threadIdx.x;
     PathCounters pc{0, 0, 0, 0, 0, 0, 0};
@@ -354,6 +357,7 @@ template <int STACK, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void learned_texel_direct(DevScene S, DevLayout Ly, uint32_t iteration,
                                                                uint32_t spp, uint32_t flags, LearnedDev B,
                                                                unsigned long long *counters) {
+    if (B.frame) iteration = B.frame[0];
     __shared__ uint32_t stack[STACK * BLOCK];
     uint32_t *col = stack + threadIdx.x;
     PathCounters pc{0, 0, 0, 0, 0, 0, 0};
@@ -393,6 +397,7 @@ __global__ __launch_bounds__(BLOCK) void learned_texel_direct(DevScene S, DevLay
 template <bool COUNT>
 __global__ void learned_finish(DevLayout Ly, LearnedDev B, float4 *payload, uint32_t spp, uint32_t reset,
                                uint32_t flags, unsigned long long *counters) {
+    if (B.frame) reset = B.frame[1];
     PathCounters pc{0, 0, 0, 0, 0, 0, 0};
     uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w < Ly.total_front) {
@@ -471,7 +476,8 @@ int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query, hipStream_t s) {
 }
 
 LearnedIO learned_io(const LearnedDev &B) {
-    return LearnedIO{B.tr_pos, B.tr_nrm, B.tr_alb, B.tr_target, B.tr_valid, B.q_pos, B.q_nrm, B.q_alb, B.q_w, B.q_pred};
+    return LearnedIO{B.tr_pos, B.tr_nrm, B.tr_alb, B.tr_target, B.tr_valid, B.q_pos, B.q_nrm, B.q_alb, B.q_w, B.q_pred,
+                     B.frame};
 }
 
 // The learned forms on the wavefront queues (same results as launch() below).
@@ -597,8 +603,6 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
     if (!tracer || !nrc || !payload) return fail(nullptr, 0, "nan_tst_execute_learned: null argument");
     if (tracer->scene->batch || tracer->scene_count != 1)
         return fail(nullptr, 0, "nan_tst_execute_learned: scene batches are not supported");
-    if (tracer->device_frame)
-        return fail(nullptr, 0, "nan_tst_execute_learned: device-frame mode is not supported");
     if (tracer->scene->roulette)
         return fail(nullptr, 0, "nan_tst_execute_learned: Russian roulette is not supported by the learned forms");
     if (tracer->scene->stack_depth == 0)
@@ -615,17 +619,25 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
         nrc_reserve_train(nrc, train_samples, s))
         return -1;
     unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
+    // device-frame mode: iteration and reset come from the tracer's device
+    // frame, which the call advances at its end (a captured call replays
+    // successive iterations)
+    tracer->learned->d.frame = wavefront_frame(tracer);
     // wavefront mode runs the forms on the tracer's queues when they need no
     // per-texel extras: 1 sample per texel, no emitter light sampling, no
     // counters, training paths within the tracer's path slots
     bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && tracer->scene->dev.emitter_count > 0;
+    int rc;
     if (tracer->mode == NAN_TRACER_WAVEFRONT && nspp == 1 && !nee && !c && train_samples <= tracer->total_front)
-        return launch_wavefront(tracer, nrc, (float4 *)payload, iteration, max_bounces, reset, train_samples, flags, s);
-    if (tracer->scene->stack_depth <= 32)
-        return launch<32>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
-                          s);
-    return launch<64>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
-                      s);
+        rc = launch_wavefront(tracer, nrc, (float4 *)payload, iteration, max_bounces, reset, train_samples, flags, s);
+    else if (tracer->scene->stack_depth <= 32)
+        rc = launch<32>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
+                        s);
+    else
+        rc = launch<64>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
+                        s);
+    if (rc == 0) wavefront_advance_frame(tracer, s);
+    return rc;
 }
 
 NAN_API int nan_tst_learned_reserve(NanTracer *tracer, NanNrc *nrc, uint32_t samples_per_texel,
@@ -659,6 +671,7 @@ NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint3
     if (n == 0 || tracer->total_front == 0) return 0;
     hipStream_t s = as_stream(stream);
     if (reserve(tracer, n, 0, s)) return -1;
+    tracer->learned->d.frame = nullptr;  // records of the given iteration
     const LearnedDev &B = tracer->learned->d;
     bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && tracer->scene->dev.emitter_count > 0;
     if (tracer->mode == NAN_TRACER_WAVEFRONT && !nee && n <= tracer->total_front) {

@@ -667,7 +667,7 @@ __global__ __launch_bounds__(kWfBlock, 6) void wf_learned_generate(DevScene S, D
         if (active) {
             w = P.list ? P.list[i] : i;
             Rng rng;
-            TexelSurface ts = texel_surface(Ly, w, P.iteration, 1u, rng);
+            TexelSurface ts = texel_surface(Ly, w, io.frame ? io.frame[0] : P.iteration, 1u, rng);
             scene = ts.scene;
             const Light &L = S.scenes[scene].light;
             if (P.side != 0) rng.state = W.rng[w];
@@ -782,8 +782,9 @@ __global__ __launch_bounds__(kWfBlock) void wf_learned_finish(DevLayout Ly, WfBu
         uint32_t inst = Ly.face_inst[f];
         uint32_t local = w - Ly.front_prefix[inst];
         uint32_t slot = P.side ? Ly.sides[inst].x + local : Ly.instances[inst].z + local;
-        if (P.ramped) store_ramped(payload, slot, acc, P.reset);
-        else store_accumulated(payload, slot, acc, P.reset);
+        uint32_t reset = io.frame ? io.frame[1] : P.reset;
+        if (P.ramped) store_ramped(payload, slot, acc, reset);
+        else store_accumulated(payload, slot, acc, reset);
     }
 }
 
@@ -792,6 +793,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_learned_finish(DevLayout Ly, WfBu
 __global__ __launch_bounds__(kWfBlock, 6) void wf_train_generate(DevScene S, DevLayout Ly, WfBuffers W,
                                                                  LearnedIO io, uint32_t n, uint32_t iteration,
                                                                  uint32_t at_x0, uint32_t ext_slot) {
+    if (io.frame) iteration = io.frame[0];
     uint32_t stride = gridDim.x * kWfBlock;
     uint32_t *ext_count = queue_counter(W.counters, ext_slot, blockIdx.x % kParts);
     uint32_t *unused = queue_counter(W.counters, ext_slot + 1, blockIdx.x % kParts);
@@ -1190,6 +1192,12 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
     }
     if (t->device_frame) hipLaunchKernelGGL(wf_advance_frame, dim3(1), dim3(64), 0, stream, W.frame);
     return check_launch("wavefront");
+}
+
+const uint32_t *wavefront_frame(const NanTracer *t) { return t->device_frame && t->wf ? t->wf->frame : nullptr; }
+
+void wavefront_advance_frame(NanTracer *t, hipStream_t stream) {
+    if (t->device_frame && t->wf) hipLaunchKernelGGL(wf_advance_frame, dim3(1), dim3(64), 0, stream, t->wf->frame);
 }
 
 int wavefront_set_frame(NanTracer *t, uint32_t iteration, uint32_t reset) {

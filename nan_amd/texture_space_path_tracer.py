@@ -148,6 +148,34 @@ class TextureSpacePathTracer:
             None if counters is None else counters.data_ptr(), command_encoder.stream_handle)
         _native.check(rc, "nan_tst_execute_learned")
 
+    def set_device_frame(self, iteration: int, reset: bool = False) -> None:
+        """Device-frame mode: execute and execute_learned take the iteration
+        and reset flag from a device counter (set here) and advance it at the
+        end of each call, so one captured graph of a call replays successive
+        iterations.  No counterpart in the reference."""
+        _native.check(_native.lib().nan_tracer_device_frame(self.handle, 1, int(iteration), 1 if reset else 0),
+                      "nan_tracer_device_frame")
+
+    def clear_device_frame(self) -> None:
+        _native.check(_native.lib().nan_tracer_device_frame(self.handle, 0, 0, 0), "nan_tracer_device_frame")
+
+    def capture_learned(self, command_encoder: CommandEncoder, output: torch.Tensor, cache, next_iteration: int, *,
+                        reset: bool = False, train_samples: int = 65536, **kw) -> "torch.cuda.CUDAGraph":
+        """Captures one learned iteration (``execute_learned`` with ``kw``)
+        into a hipGraph in device-frame mode: replays run iterations
+        next_iteration, next_iteration + 1, ..., the first resetting the
+        slots when ``reset``, each training the cache once more (Adam's step
+        count lives on the device).  The working set is reserved first, so
+        this may be the tracer's first learned iteration.  No counterpart in
+        the reference."""
+        self.reserve_learned(cache, train_samples)
+        self.set_device_frame(next_iteration, reset)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self.execute_learned(command_encoder, output, 0, cache, train_samples=train_samples, **kw)
+        return graph
+
     def reserve_learned(self, cache, train_samples: int = 65536) -> None:
         """Sizes the learned form's buffers (training records, cache queries,
         network scratch) for ``train_samples`` training paths per iteration.

@@ -175,3 +175,40 @@ def test_sixteen_samples_eight_bounces(device):
         orc.trace_texels(layout.face_table, layout.instance_table, layout.side_table, ref, it, 16, 8, it == 0,
                          threads=oracle.cpu_count())
     np.testing.assert_array_equal(outs[0].view(np.uint32), ref.view(np.uint32))
+
+
+def test_learned_iterations_replayed_from_one_capture(device):
+    """Device-frame mode: one captured learned iteration replayed three
+    times equals three eager iterations (iterations 0, 1, 2 from a reset) on
+    a twin cache, to the tolerance of the atomically summed hash-grid
+    gradients, on the wavefront queues (a sunlit scene: no emitter light
+    sampling)."""
+    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+
+    node = batch_scene(21, 3000)
+    kw = dict(texels_per_unit=6.0, min_resolution=1, max_resolution=8, max_total_texels=2_000_000)
+    layout, sc, tracer = _tracer(device, node, **kw)
+    _, _, eager = _tracer(device, node, **kw)
+    dev = device.torch_device
+    enc = device.create_command_encoder()
+    train = 8192
+    caches = [RadianceCache.for_scene(device, sc, RadianceCacheConfig(minibatches=2)) for _ in range(2)]
+    w0, _ = caches[0].get_params()
+    payloads = [torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=dev) for _ in range(2)]
+    _say("learned: capture in device-frame mode, replay 3 iterations")
+    g = tracer.capture_learned(enc, payloads[0], caches[0], 0, reset=True, train_samples=train)
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    tracer.clear_device_frame()
+    for it in range(3):
+        eager.execute_learned(enc, payloads[1], it, caches[1], reset=(it == 0), train_samples=train)
+    torch.cuda.synchronize()
+    wg, _ = caches[0].get_params()
+    we, _ = caches[1].get_params()
+    np.testing.assert_allclose(wg - w0, we - w0, rtol=2e-2, atol=2e-4)
+    a, b = payloads[0].cpu().numpy(), payloads[1].cpu().numpy()
+    np.testing.assert_array_equal(a.view(np.uint32)[:, 3], b.view(np.uint32)[:, 3])  # sample counts: 3 iterations
+    assert (a.view(np.uint32)[:, 3] == 3).all()
+    rel = np.abs(a[:, :3] - b[:, :3]).mean() / np.abs(b[:, :3]).mean()
+    assert rel < 1e-2, rel
