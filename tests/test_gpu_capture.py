@@ -53,6 +53,16 @@ def _bits_equal(a: torch.Tensor, b: torch.Tensor, tag: str) -> None:
     assert len(bad) == 0, f"{tag}: {len(bad)} of {len(x)} slots differ, first {bad[0]}"
 
 
+def _same_updates(dg: np.ndarray, de: np.ndarray) -> None:
+    """Weight updates of a replayed and an eager run: equal up to the
+    atomics' rounding order (99% of weights within 2% or 2e-4, update norms
+    within 2%)."""
+    close = np.isclose(dg, de, rtol=2e-2, atol=2e-4)
+    assert close.mean() > 0.99, close.mean()
+    ratio = float(np.linalg.norm(dg) / np.linalg.norm(de))
+    assert abs(ratio - 1.0) < 0.02, ratio
+
+
 def test_first_execute_and_first_after_tune_captured(device):
     # 1/16 of the texels double-sided: the wavefront forks its second stream inside the capture
     node = synthetic_scene(3, terrain_quads=48, box_grid=8, spheres=24, extent=20.0)
@@ -114,11 +124,13 @@ def test_first_learned_iteration_captured_and_replayed(device):
     wg, _ = caches[0].get_params()
     we, _ = caches[1].get_params()
     # the hash-grid gradients are summed with atomics (order-dependent FP16
-    # rounding), so the twins agree closely, not bit for bit; a replay stuck
-    # at step 1's bias corrections would move the weights ~1.9x as far in step 2
+    # rounding), so the twins agree closely, not bit for bit (weights whose
+    # gradient is near zero take Adam's sign-like step either way); a replay
+    # stuck at step 1's bias corrections would move every weight ~1.9x as far
+    # in step 2
     dg, de = wg - w0, we - w0
     assert np.abs(de).max() > 1e-3
-    np.testing.assert_allclose(dg, de, rtol=2e-2, atol=2e-4)
+    _same_updates(dg, de)
     a = payloads[0][:, :3].double()
     b = payloads[1][:, :3].double()
     assert torch.isfinite(a).all()
@@ -206,7 +218,7 @@ def test_learned_iterations_replayed_from_one_capture(device):
     torch.cuda.synchronize()
     wg, _ = caches[0].get_params()
     we, _ = caches[1].get_params()
-    np.testing.assert_allclose(wg - w0, we - w0, rtol=2e-2, atol=2e-4)
+    _same_updates(wg - w0, we - w0)
     a, b = payloads[0].cpu().numpy(), payloads[1].cpu().numpy()
     np.testing.assert_array_equal(a.view(np.uint32)[:, 3], b.view(np.uint32)[:, 3])  # sample counts: 3 iterations
     assert (a.view(np.uint32)[:, 3] == 3).all()
