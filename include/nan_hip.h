@@ -23,7 +23,7 @@ extern "C" {
 
 #define NAN_API __attribute__((visibility("default")))
 
-#define NAN_HIP_ABI_VERSION 2
+#define NAN_HIP_ABI_VERSION 3
 
 /* Scene input: the buffers the reference's Scene uploads (scene.py:330-437).
  *   vertices        float[V][8]  object-space position, normal, uv
@@ -32,10 +32,20 @@ extern "C" {
  *   instance_descs  uint32[I][4] mesh_id, material_id, transform_id, flags (bit0 odd-negative)
  *   transforms      float[X][16] row-major object-to-world
  *   inv_transposes  float[X][16] row-major transpose(inverse(M))
- *   materials       uint32[Mat][8] base_color.rgb (f32), flags (bit0 double-sided),
- *                                  emissive.rgb (f32), alpha_mode
+ *   materials       uint32[Mat][12] base_color.rgb (f32), flags (bit0 double-sided),
+ *                                  emissive.rgb (f32), alpha_mode (0 opaque, 1 mask, 2 blend),
+ *                                  base-colour, emissive and normal texture index
+ *                                  (0xFFFFFFFF: none), alpha_cutoff (f32)
+ *                                  (MaterialDesc, scene.slang; material.py:27-67)
  *   lighting        float[16]    sun dir.xyz, cos_half_angle, color*intensity.rgb, pad,
- *                                transmittance.rgb, pad, sky radiance.rgb, pad          */
+ *                                transmittance.rgb, pad, sky radiance.rgb, pad
+ *   textures        uint32[Tx][4] first texel, width, height, flags (bit0: RGB is sRGB)
+ *   texels          uint32[N]    RGBA8 texels of every texture, rows top to bottom,
+ *                                R in the low byte (texture_manager.py:70-160)
+ * Textures are sampled like the reference's SampleLevel(linear_sampler, uv, 0) on
+ * mip 0: bilinear, wrap addressing, sRGB decoded per texel before filtering
+ * (scene.slang:124-131, 313-368).  Materials whose alpha_mode is not opaque are
+ * alpha tested on every candidate hit (scene.slang:134-149, 1017-1172). */
 typedef struct NanSceneDesc {
     const float *vertices;
     uint32_t vertex_count;
@@ -51,6 +61,10 @@ typedef struct NanSceneDesc {
     const uint32_t *materials;
     uint32_t material_count;
     const float *lighting;
+    const uint32_t *textures;  /* NULL with texture_count 0 */
+    uint32_t texture_count;
+    const uint32_t *texels;
+    uint64_t texel_count;
 } NanSceneDesc;
 
 /* BVH build options (0 selects the default). */
@@ -111,6 +125,11 @@ NAN_API int nan_scene_set_sky_view_lut(NanScene *scene, const float *sky_view_lu
  * and paths end after 1 to max_bounces bounces.  Batches copy the flag of each
  * scene at nan_batch_create.  The learned forms refuse such scenes. */
 NAN_API int nan_scene_set_russian_roulette(NanScene *scene, uint32_t enable);
+/* Frame index of the stochastic alpha test of blend materials (Scene.frame_index,
+ * scene.py:163, 1213-1216; read at scene.slang:146-148), written on `stream` by a
+ * kernel, so a captured graph may hold the write.  Batches hold a frame index of
+ * their own (nan_batch_set_frame_index). */
+NAN_API int nan_scene_set_frame_index(NanScene *scene, uint32_t frame_index, void *stream);
 
 /* Closest hit and occlusion for rays float[n][8] (origin, tmin, dir, tmax) in device
  * memory; writes hits float[n][4] (t, b1, b2, 0; t < 0 on a miss), ids uint32[n][2]
@@ -193,6 +212,9 @@ NAN_API int nan_batch_set_payloads(NanTracer *batch, void *const *payloads);
 /* Per-scene path length (0 = the launch's max_bounces); the launch's
  * max_bounces must be at least the largest. */
 NAN_API int nan_batch_set_bounces(NanTracer *batch, const uint32_t *max_bounces);
+/* Frame index of the stochastic alpha test for every scene of the batch (as
+ * nan_scene_set_frame_index; the batch's scenes share it, starting at 0). */
+NAN_API int nan_batch_set_frame_index(NanTracer *batch, uint32_t frame_index, void *stream);
 
 /* Device-frame mode for graph replay: while enabled, nan_tst_execute and
  * nan_tst_execute_learned take the iteration and reset flag from a device

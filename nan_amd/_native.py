@@ -12,6 +12,7 @@ import ctypes
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libnan_hip.so"
+ABI_VERSION = 3  # NAN_HIP_ABI_VERSION of include/nan_hip.h
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
 u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -36,6 +37,8 @@ class NanSceneDesc(ctypes.Structure):
         ("transforms", f32p), ("inv_transposes", f32p), ("transform_count", ctypes.c_uint32),
         ("materials", u32p), ("material_count", ctypes.c_uint32),
         ("lighting", f32p),
+        ("textures", u32p), ("texture_count", ctypes.c_uint32),
+        ("texels", u32p), ("texel_count", ctypes.c_uint64),
     ]
 
 
@@ -96,6 +99,7 @@ EXPORTED_SYMBOLS = (
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
     "nan_tst_learned_records", "nan_tst_learned_reserve", "nan_scene_set_russian_roulette", "nan_calibrate_valu",
+    "nan_scene_set_frame_index", "nan_batch_set_frame_index",
 )
 
 _LIB = None
@@ -115,6 +119,8 @@ def _declare(lib) -> None:
         "nan_scene_set_lighting": (I, [vp, f32p]),
         "nan_scene_set_sky_view_lut": (I, [vp, vp]),
         "nan_scene_set_russian_roulette": (I, [vp, ctypes.c_uint32]),
+        "nan_scene_set_frame_index": (I, [vp, ctypes.c_uint32, vp]),
+        "nan_batch_set_frame_index": (I, [vp, ctypes.c_uint32, vp]),
         "nan_atmosphere_transmittance_lut": (I, [vp, vp]),
         "nan_atmosphere_multiscattering_lut": (I, [vp, vp, vp]),
         "nan_atmosphere_sky_view_lut": (I, [vp, vp, f32p, vp, vp]),
@@ -180,6 +186,9 @@ def lib():
     except OSError as exc:
         raise NativeLibraryError(f"cannot load {LIB_PATH}: {exc}") from exc
     _declare(handle)
+    if handle.nan_version() != ABI_VERSION:
+        raise NativeLibraryError(f"{LIB_PATH} has ABI version {handle.nan_version()}, this package needs "
+                                 f"{ABI_VERSION}; rebuild it (`python -m nan_amd.build`)")
     _LIB = handle
     return handle
 

@@ -101,6 +101,14 @@ class SceneBatch:
                                                        1 if reset else 0, counters.data_ptr(), stream)
         _native.check(rc, "nan_tst_execute (batch)")
 
+    def set_frame_index(self, command_encoder: CommandEncoder, frame_index: int) -> None:
+        """Frame index of the blend alpha test, shared by every scene of the
+        batch (a one-at-a-time run of a scene with the same frame index gives
+        the same slots)."""
+        _native.check(_native.lib().nan_batch_set_frame_index(self.handle, int(frame_index) & 0xFFFFFFFF,
+                                                              command_encoder.stream_handle),
+                      "nan_batch_set_frame_index")
+
     def profile(self, enable: bool) -> None:
         _native.check(_native.lib().nan_tracer_profile(self.handle, 1 if enable else 0), "nan_tracer_profile")
 

@@ -27,7 +27,7 @@
 
 using namespace nanx;
 
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __global__ __launch_bounds__(BLOCK) void tst_megakernel(DevScene S, DevLayout Ly, float4 *payload,
                                                         uint32_t iteration, uint32_t spp, uint32_t max_bounces,
                                                         uint32_t reset, unsigned long long *counters) {
@@ -41,12 +41,12 @@ __global__ __launch_bounds__(BLOCK) void tst_megakernel(DevScene S, DevLayout Ly
         uint32_t nspp = spp > 1u ? spp : 1u;
         f3 n = world_flat_normal(S, ts.tri);
         const bool roulette = S.scenes[ts.scene].roulette != 0u;
-        f3 e = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.tri, ts.b1, ts.b2, n, nspp, max_bounces, rng, col, pc,
+        f3 e = estimate_irradiance<STACK, BLOCK, COUNT, MAT>(S, ts.tri, ts.b1, ts.b2, n, nspp, max_bounces, rng, col, pc,
                                                         roulette);
         store_accumulated(payload, ts.global_texel, e, reset);
         if (COUNT) pc.slots++;
         if (ts.side.y & 1u) {
-            f3 eb = estimate_irradiance<STACK, BLOCK, COUNT>(S, ts.tri, ts.b1, ts.b2, -n, nspp, max_bounces, rng, col,
+            f3 eb = estimate_irradiance<STACK, BLOCK, COUNT, MAT>(S, ts.tri, ts.b1, ts.b2, -n, nspp, max_bounces, rng, col,
                                                              pc, roulette);
             store_accumulated(payload, ts.side.x + ts.local, eb, reset);
             if (COUNT) pc.slots++;
@@ -64,7 +64,7 @@ __global__ void tst_raygen_kernel(DevLayout Ly, uint32_t iteration, uint32_t spp
     if (rng_out) rng_out[w] = rng.state;
 }
 
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool MAT>
 __global__ __launch_bounds__(BLOCK) void trace_rays_kernel(DevScene S, const float4 *rays, uint64_t n, float4 *hits,
                                                            uint2 *ids, uint32_t *occluded) {
     __shared__ uint32_t stack[STACK * BLOCK];
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(BLOCK) void trace_rays_kernel(DevScene S, const flo
     ray_setup(r);
     Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
     TraversalStats st{0, 0};
-    bool hit = traverse<STACK, BLOCK, false, false>(S, r, h, col, st);
+    bool hit = traverse<STACK, BLOCK, false, false, MAT>(S, r, h, col, st);
     if (hit) {
         hits[k] = make_float4(h.t, h.b1, h.b2, 0.0f);
         ids[k] = S.tri_ids[h.id];
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(BLOCK) void trace_rays_kernel(DevScene S, const flo
     }
     if (occluded) {
         Hit o{r.tmax, 0.0f, 0.0f, kInvalid};
-        occluded[k] = traverse<STACK, BLOCK, true, false>(S, r, o, col, st) ? 1u : 0u;
+        occluded[k] = traverse<STACK, BLOCK, true, false, MAT>(S, r, o, col, st) ? 1u : 0u;
     }
 }
 
@@ -140,7 +140,7 @@ __device__ f3 sky_with_sun_disk(const Light &L, f3 dir) {
     return out;
 }
 
-template <int STACK, int BLOCK, bool FROZEN>
+template <int STACK, int BLOCK, bool FROZEN, bool MAT>
 __global__ __launch_bounds__(BLOCK) void resolve_kernel(DevScene S, DevLayout Ly, const void *cache, DevCamera cam,
                                                         float4 *out, uint32_t width, uint32_t height) {
     __shared__ uint32_t stack[STACK * BLOCK];
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(BLOCK) void resolve_kernel(DevScene S, DevLayout Ly
     Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
     TraversalStats st{0, 0};
     f3 color;
-    if (!traverse<STACK, BLOCK, false, false>(S, r, h, col, st)) {
+    if (!traverse<STACK, BLOCK, false, false, MAT>(S, r, h, col, st)) {
         color = sky_with_sun_disk(S.light, d);
     } else {
         uint2 ids = S.tri_ids[h.id];
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(BLOCK) void resolve_kernel(DevScene S, DevLayout Ly
             f3 bc = mk(1.0f - h.b1 - h.b2, h.b1, h.b2);
             f3 E = sample_irradiance<FROZEN>(cache, bc, ii.z, face, side, back);
             Shading sd;
-            shading_data(S, h.id, h.b1, h.b2, -d, sd);
+            shading_data<MAT>(S, h.id, h.b1, h.b2, -d, sd);
             color = sd.emissive + (sd.diffuse * kInvPi) * E;
         }
     }
@@ -313,19 +313,29 @@ void launch_megakernel(const NanTracer *t, float4 *payload, uint32_t iteration, 
                        uint32_t reset, unsigned long long *counters, hipStream_t stream) {
     DevLayout L = dev_layout(t);
     uint32_t grid = grid_for(t->total_front, kTraceBlock);
-    if (counters)
-        hipLaunchKernelGGL((tst_megakernel<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, stream,
-                           t->scene->dev, L, payload, iteration, spp, bounces, reset, counters);
-    else
-        hipLaunchKernelGGL((tst_megakernel<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, stream,
-                           t->scene->dev, L, payload, iteration, spp, bounces, reset, counters);
+    const bool mat = t->scene->dev.mat_features != 0u;
+#define NAN_MEGAKERNEL(C, M)                                                                                \
+    hipLaunchKernelGGL((tst_megakernel<STACK, kTraceBlock, C, M>), dim3(grid), dim3(kTraceBlock), 0, stream, \
+                       t->scene->dev, L, payload, iteration, spp, bounces, reset, counters)
+    if (counters) {
+        if (mat) NAN_MEGAKERNEL(true, true);
+        else NAN_MEGAKERNEL(true, false);
+    } else {
+        if (mat) NAN_MEGAKERNEL(false, true);
+        else NAN_MEGAKERNEL(false, false);
+    }
+#undef NAN_MEGAKERNEL
 }
 
 template <int STACK>
 void launch_trace_rays(const NanScene *s, const float4 *rays, uint64_t n, float4 *hits, uint2 *ids, uint32_t *occ,
                        hipStream_t stream) {
-    hipLaunchKernelGGL((trace_rays_kernel<STACK, kTraceBlock>), dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), 0,
-                       stream, s->dev, rays, n, hits, ids, occ);
+    if (s->dev.mat_features)
+        hipLaunchKernelGGL((trace_rays_kernel<STACK, kTraceBlock, true>), dim3(grid_for(n, kTraceBlock)),
+                           dim3(kTraceBlock), 0, stream, s->dev, rays, n, hits, ids, occ);
+    else
+        hipLaunchKernelGGL((trace_rays_kernel<STACK, kTraceBlock, false>), dim3(grid_for(n, kTraceBlock)),
+                           dim3(kTraceBlock), 0, stream, s->dev, rays, n, hits, ids, occ);
 }
 
 template <int STACK>
@@ -333,12 +343,18 @@ void launch_resolve(const NanTracer *t, const void *cache, int frozen, const Dev
                     uint32_t w, uint32_t h, hipStream_t stream) {
     DevLayout L = dev_layout(t);
     uint64_t n = (uint64_t)w * h;
-    if (frozen)
-        hipLaunchKernelGGL((resolve_kernel<STACK, kTraceBlock, true>), dim3(grid_for(n, kTraceBlock)),
-                           dim3(kTraceBlock), 0, stream, t->scene->dev, L, cache, cam, out, w, h);
-    else
-        hipLaunchKernelGGL((resolve_kernel<STACK, kTraceBlock, false>), dim3(grid_for(n, kTraceBlock)),
-                           dim3(kTraceBlock), 0, stream, t->scene->dev, L, cache, cam, out, w, h);
+    const bool mat = t->scene->dev.mat_features != 0u;
+#define NAN_RESOLVE(F, M)                                                                                 \
+    hipLaunchKernelGGL((resolve_kernel<STACK, kTraceBlock, F, M>), dim3(grid_for(n, kTraceBlock)),         \
+                       dim3(kTraceBlock), 0, stream, t->scene->dev, L, cache, cam, out, w, h)
+    if (frozen) {
+        if (mat) NAN_RESOLVE(true, true);
+        else NAN_RESOLVE(true, false);
+    } else {
+        if (mat) NAN_RESOLVE(false, true);
+        else NAN_RESOLVE(false, false);
+    }
+#undef NAN_RESOLVE
 }
 
 
@@ -357,6 +373,79 @@ NAN_API int nan_version(void) { return NAN_HIP_ABI_VERSION; }
 
 NAN_API const char *nan_last_error(void) { return g_last_error.c_str(); }
 
+// Texel decode table of every texture: c / 255, then the inverse sRGB
+// transfer function (IEC 61966-2-1) of c / 255, each in double, rounded once.
+static void make_texel_lut(float *lut) {
+    for (int c = 0; c < 256; ++c) {
+        double x = c / 255.0;
+        lut[c] = (float)x;
+        lut[256 + c] = (float)(x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+    }
+}
+
+// 1 when some material is textured or alpha tested (DevScene::mat_features)
+static uint32_t material_features(const uint32_t *materials, size_t count) {
+    for (size_t m = 0; m < count; ++m) {
+        const uint32_t *w = materials + kMaterialWords * m;
+        if (w[7] != 0u || w[8] != 0xFFFFFFFFu || w[9] != 0xFFFFFFFFu || w[10] != 0xFFFFFFFFu) return 1u;
+    }
+    return 0u;
+}
+
+static int check_materials_and_textures(const NanSceneDesc *d, char *err, size_t cap) {
+    if (d->texture_count && (!d->textures || !d->texels))
+        return fail(err, cap, "nan_scene_create: null texture buffer");
+    for (uint32_t k = 0; k < d->texture_count; ++k) {
+        const uint32_t *t = d->textures + 4 * (size_t)k;
+        if (t[1] == 0 || t[2] == 0 || t[1] > (1u << 24) || t[2] > (1u << 24) ||
+            (uint64_t)t[0] + (uint64_t)t[1] * t[2] > d->texel_count)
+            return fail(err, cap, "nan_scene_create: texture %u (%ux%u at texel %u) out of range", k, t[1], t[2], t[0]);
+    }
+    for (uint32_t m = 0; m < d->material_count; ++m) {
+        const uint32_t *w = d->materials + kMaterialWords * (size_t)m;
+        if (w[7] > 2u) return fail(err, cap, "nan_scene_create: material %u has unknown alpha_mode %u", m, w[7]);
+        for (int j = 8; j < 11; ++j)
+            if (w[j] != 0xFFFFFFFFu && w[j] >= d->texture_count)
+                return fail(err, cap, "nan_scene_create: material %u references missing texture %u", m, w[j]);
+    }
+    return 0;
+}
+
+// Textures, the texel decode table and the per-triangle uv records on the
+// device (nothing without textures); the frame index of the alpha test always.
+static int upload_textures(NanScene *s, const uint32_t *materials, size_t material_count,
+                           const std::vector<uint32_t> &textures, const std::vector<uint32_t> &texels,
+                           const std::vector<float> &tri_uv, char *err, size_t cap) {
+    auto up = [&](auto **dst, const void *src, size_t bytes) -> int {
+        int rc = upload(dst, src, bytes, err, cap);
+        if (rc == 0) s->allocations.push_back((void *)*dst);
+        return rc;
+    };
+    const uint32_t zero = 0;
+    uint32_t *frame;
+    if (up(&frame, &zero, 4)) return -1;
+    s->dev.frame_index = frame;
+    s->dev.mat_features = material_features(materials, material_count);
+    if (textures.empty()) return 0;
+    float lut[512];
+    make_texel_lut(lut);
+    uint4 *dtex;
+    uint32_t *dtexels;
+    float *dlut;
+    float4 *duv;
+    int rc = 0;
+    rc |= up(&dtex, textures.data(), textures.size() * 4);
+    rc |= up(&dtexels, texels.data(), texels.size() * 4);
+    rc |= up(&dlut, lut, sizeof(lut));
+    rc |= up(&duv, tri_uv.data(), tri_uv.size() * 4);
+    if (rc) return -1;
+    s->dev.textures = dtex;
+    s->dev.texels = dtexels;
+    s->dev.texel_lut = dlut;
+    s->dev.tri_uv = duv;
+    return 0;
+}
+
 NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options, NanScene **out, char *err,
                              size_t cap) {
     if (!d || !out) return fail(err, cap, "nan_scene_create: null argument");
@@ -365,6 +454,7 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
         !d->inv_transposes || !d->materials || !d->lighting)
         return fail(err, cap, "nan_scene_create: null scene buffer");
     if (d->instance_count == 0) return fail(err, cap, "nan_scene_create: scene has no instances");
+    if (check_materials_and_textures(d, err, cap)) return -1;
     auto t0 = std::chrono::steady_clock::now();
     // validate descriptors and count triangles
     uint64_t tri_total = 0;
@@ -452,6 +542,31 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     for (uint32_t s : kStackDepths)
         if (bvh.max_stack <= s) { depth = s; break; }
     if (!depth) return fail(err, cap, "nan_scene_create: BVH needs a %u-entry stack (max 64)", bvh.max_stack);
+    // triangles of non-opaque materials carry the alpha-test flag in their record
+    for (size_t r = 0; r < bvh.tris.size(); r += nanbvh::kTriWords) {
+        uint32_t id;
+        std::memcpy(&id, &bvh.tris[r + 9], 4);
+        const uint32_t *inst = d->instance_descs + 4 * (size_t)tri_ids[2 * (size_t)id];
+        uint32_t flag = d->materials[kMaterialWords * (size_t)inst[1] + 7] != 0u ? 1u : 0u;
+        std::memcpy(&bvh.tris[r + 10], &flag, 4);
+    }
+    // per-triangle uv records (with textures): uv0 uv1 | uv2 0 0
+    std::vector<float> tri_uv;
+    if (d->texture_count) {
+        tri_uv.assign((size_t)tri_total * 8, 0.0f);
+        g = 0;
+        for (uint32_t i = 0; i < d->instance_count; ++i) {
+            const uint32_t *id = d->instance_descs + 4 * (size_t)i;
+            const uint32_t *md = d->mesh_descs + 4 * (size_t)id[0];
+            uint32_t ntri = md[1] / 3;
+            for (uint32_t p = 0; p < ntri; ++p, ++g)
+                for (int k = 0; k < 3; ++k) {
+                    const float *v = d->vertices + 8 * (size_t)(md[2] + d->indices[md[3] + 3 * p + k]);
+                    tri_uv[8 * g + 2 * k] = v[6];
+                    tri_uv[8 * g + 2 * k + 1] = v[7];
+                }
+        }
+    }
     auto t1 = std::chrono::steady_clock::now();
 
     NanScene *s = new NanScene();
@@ -477,15 +592,20 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     rc |= up(&inst_descs, d->instance_descs, (size_t)d->instance_count * 16);
     rc |= up(&xf, d->transforms, (size_t)d->transform_count * 64);
     rc |= up(&itf, d->inv_transposes, (size_t)d->transform_count * 64);
-    rc |= up(&mats, d->materials, (size_t)d->material_count * 32);
+    rc |= up(&mats, d->materials, (size_t)d->material_count * kMaterialWords * 4);
     rc |= up(&shrec, shade_rec.data(), shade_rec.size() * 4);
+    std::vector<uint32_t> h_textures(d->textures, d->textures + (size_t)d->texture_count * 4);
+    std::vector<uint32_t> h_texels;
+    if (d->texture_count) h_texels.assign(d->texels, d->texels + d->texel_count);
+    rc |= upload_textures(s, d->materials, d->material_count, h_textures, h_texels, tri_uv, err, cap);
     if (rc) {
         delete s;
         return -1;
     }
     bytes_total = bvh.nodes.size() * 4 + bvh.tris.size() * 4 + tri_ids.size() * 4 + (size_t)d->vertex_count * 32 +
                   (size_t)d->index_count * 4 + (size_t)d->mesh_count * 16 + (size_t)d->instance_count * 16 +
-                  (size_t)d->transform_count * 128 + (size_t)d->material_count * 32 + shade_rec.size() * 4;
+                  (size_t)d->transform_count * 128 + (size_t)d->material_count * kMaterialWords * 4 +
+                  shade_rec.size() * 4 + h_textures.size() * 4 + h_texels.size() * 4 + tri_uv.size() * 4;
     s->dev.nodes = nodes;
     s->dev.tris = tris;
     s->dev.tri_ids = ids;
@@ -502,7 +622,10 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
     s->h_shade = std::move(shade_rec);
     s->h_xf.assign(d->transforms, d->transforms + (size_t)d->transform_count * 16);
     s->h_itf.assign(d->inv_transposes, d->inv_transposes + (size_t)d->transform_count * 16);
-    s->h_mats.assign(d->materials, d->materials + (size_t)d->material_count * 8);
+    s->h_mats.assign(d->materials, d->materials + (size_t)d->material_count * kMaterialWords);
+    s->h_tri_uv = std::move(tri_uv);
+    s->h_textures = std::move(h_textures);
+    s->h_texels = std::move(h_texels);
     s->dev.root = bvh.root_entry;
     {
         // emissive triangles (material emission > 0), areas in float64
@@ -510,7 +633,7 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
         std::vector<double> areas;
         for (uint64_t t = 0; t < tri_total; ++t) {
             const uint32_t *id = d->instance_descs + 4 * (size_t)tri_ids[2 * t];
-            const uint32_t *mw = d->materials + 8 * (size_t)id[1];
+            const uint32_t *mw = d->materials + kMaterialWords * (size_t)id[1];
             float le[3];
             std::memcpy(le, mw + 4, 12);
             if (!(le[0] > 0.0f || le[1] > 0.0f || le[2] > 0.0f)) continue;
@@ -525,11 +648,16 @@ NAN_API int nan_scene_create(const NanSceneDesc *d, const NanBvhOptions *options
             c[2] = e1[0] * e2[1] - e1[1] * e2[0];
             double len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
             if (!(len > 0.0)) continue;
+            // textured or alpha-tested emitters evaluate the point they sample
+            uint32_t tri_bits = (uint32_t)t, varying = (mw[9] != 0xFFFFFFFFu || mw[7] != 0u) ? 1u : 0u;
+            float tri_f, varying_f;
+            std::memcpy(&tri_f, &tri_bits, 4);
+            std::memcpy(&varying_f, &varying, 4);
             float rec[20] = {w[0], w[1], w[2], (float)(0.5 * len),
-                             (float)e1[0], (float)e1[1], (float)e1[2], 0.0f,
+                             (float)e1[0], (float)e1[1], (float)e1[2], tri_f,
                              (float)e2[0], (float)e2[1], (float)e2[2], 0.0f,
                              (float)(c[0] / len), (float)(c[1] / len), (float)(c[2] / len), 0.0f,
-                             le[0], le[1], le[2], 0.0f};
+                             le[0], le[1], le[2], varying_f};
             em.insert(em.end(), rec, rec + 20);
             areas.push_back(0.5 * len);
         }
@@ -639,6 +767,21 @@ NAN_API int nan_scene_set_russian_roulette(NanScene *scene, uint32_t enable) {
     if (scene->batch) return fail(nullptr, 0, "nan_scene_set_russian_roulette: set it on each scene of a batch");
     scene->roulette = enable ? 1u : 0u;
     return refresh_light(scene);
+}
+
+__global__ void set_frame_index_kernel(uint32_t *dst, uint32_t value) { *dst = value; }
+
+static int set_frame_index(NanScene *scene, uint32_t frame_index, void *stream) {
+    hipLaunchKernelGGL(set_frame_index_kernel, dim3(1), dim3(1), 0, as_stream(stream),
+                       const_cast<uint32_t *>(scene->dev.frame_index), frame_index);
+    HIP_TRY(hipGetLastError(), nullptr, 0);
+    return 0;
+}
+
+NAN_API int nan_scene_set_frame_index(NanScene *scene, uint32_t frame_index, void *stream) {
+    if (!scene) return fail(nullptr, 0, "nan_scene_set_frame_index: null scene");
+    if (scene->batch) return fail(nullptr, 0, "nan_scene_set_frame_index: use nan_batch_set_frame_index on a batch");
+    return set_frame_index(scene, frame_index, stream);
 }
 
 NAN_API int nan_scene_set_sky_view_lut(NanScene *scene, const float *sky_view_lut) {
@@ -932,8 +1075,9 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
                              char *err, size_t cap) {
     if (!scenes || !tracers || !out || count == 0) return fail(err, cap, "nan_batch_create: null argument");
     *out = nullptr;
-    uint64_t nodes = 0, tris = 0, xfs = 0, mats = 0, faces = 0, insts = 0, front = 0, ds = 0;
+    uint64_t nodes = 0, tris = 0, xfs = 0, mats = 0, faces = 0, insts = 0, front = 0, ds = 0, texel_total = 0;
     uint32_t max_stack = 0;
+    bool any_textures = false;
     for (uint32_t k = 0; k < count; ++k) {
         const NanScene *sc = scenes[k];
         const NanTracer *tr = tracers[k];
@@ -942,18 +1086,24 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
         nodes += sc->h_nodes.size() / nanbvh::kNodeWords;
         tris += sc->h_tris.size() / nanbvh::kTriWords;
         xfs += sc->h_xf.size() / 16;
-        mats += sc->h_mats.size() / 8;
+        mats += sc->h_mats.size() / kMaterialWords;
+        texel_total += sc->h_texels.size();
+        any_textures = any_textures || !sc->h_textures.empty();
         faces += tr->face_count;
         insts += tr->instance_count;
         front += tr->total_front;
         ds += tr->ds_count;
         max_stack = std::max(max_stack, sc->info.max_stack);
     }
-    if (tris > nanbvh::kLeafFirstMask || nodes >= (1ull << 31) || front >= (1ull << 32) || faces >= (1ull << 32))
+    if (tris > nanbvh::kLeafFirstMask || nodes >= (1ull << 31) || front >= (1ull << 32) || faces >= (1ull << 32) ||
+        texel_total >= (1ull << 32))
         return fail(err, cap, "nan_batch_create: batch too large for 32-bit indices");
     std::vector<uint32_t> h_nodes;
     std::vector<float> h_tris, h_shade, h_xf, h_itf;
     std::vector<uint32_t> h_mats, h_faces, h_inst, h_sides, h_tf, h_fi, h_prefix, h_ds, h_face_scene, h_inst_base;
+    std::vector<uint32_t> h_textures, h_texels;
+    std::vector<float> h_tri_uv;
+    if (any_textures) h_tri_uv.reserve(tris * 8);
     std::vector<SceneRec> recs(count);
     h_nodes.reserve(nodes * nanbvh::kNodeWords);
     h_tris.reserve(tris * nanbvh::kTriWords);
@@ -992,7 +1142,23 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
         }
         h_xf.insert(h_xf.end(), sc->h_xf.begin(), sc->h_xf.end());
         h_itf.insert(h_itf.end(), sc->h_itf.begin(), sc->h_itf.end());
+        size_t m0 = h_mats.size();
         h_mats.insert(h_mats.end(), sc->h_mats.begin(), sc->h_mats.end());
+        const uint32_t tex_off = (uint32_t)(h_textures.size() / 4), texel_off = (uint32_t)h_texels.size();
+        for (size_t i = m0; i < h_mats.size(); i += kMaterialWords)
+            for (int j = 8; j < 11; ++j)
+                if (h_mats[i + j] != 0xFFFFFFFFu) h_mats[i + j] += tex_off;
+        for (size_t i = 0; i < sc->h_textures.size(); i += 4) {
+            h_textures.push_back(sc->h_textures[i] + texel_off);
+            h_textures.insert(h_textures.end(), sc->h_textures.begin() + i + 1, sc->h_textures.begin() + i + 4);
+        }
+        h_texels.insert(h_texels.end(), sc->h_texels.begin(), sc->h_texels.end());
+        if (any_textures) {
+            if (sc->h_tri_uv.empty())
+                h_tri_uv.resize(h_tri_uv.size() + sc->h_tris.size() / nanbvh::kTriWords * 8, 0.0f);
+            else
+                h_tri_uv.insert(h_tri_uv.end(), sc->h_tri_uv.begin(), sc->h_tri_uv.end());
+        }
         recs[k] = SceneRec{};
         recs[k].root = rebase_entry(sc->dev.root);
         recs[k].slack = sc->dev.slack;
@@ -1019,7 +1185,7 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
         node_off += (uint32_t)(sc->h_nodes.size() / nanbvh::kNodeWords);
         tri_off += (uint32_t)(sc->h_tris.size() / nanbvh::kTriWords);
         xf_off += (uint32_t)(sc->h_xf.size() / 16);
-        mat_off += (uint32_t)(sc->h_mats.size() / 8);
+        mat_off += (uint32_t)(sc->h_mats.size() / kMaterialWords);
         face_off += tr->face_count;
         inst_off += tr->instance_count;
         front_off += (uint32_t)tr->total_front;
@@ -1042,6 +1208,7 @@ NAN_API int nan_batch_create(NanScene *const *scenes, NanTracer *const *tracers,
     rc |= up(&d_itf, h_itf.data(), h_itf.size() * 4);
     rc |= up(&d_mats, h_mats.data(), h_mats.size() * 4);
     rc |= up(&d_recs, recs.data(), recs.size() * sizeof(SceneRec));
+    rc |= upload_textures(bs, h_mats.data(), h_mats.size() / kMaterialWords, h_textures, h_texels, h_tri_uv, err, cap);
     if (rc) {
         delete bs;
         return -1;
@@ -1112,6 +1279,11 @@ NAN_API int nan_batch_set_bounces(NanTracer *batch, const uint32_t *max_bounces)
     HIP_TRY(hipMemcpy((void *)batch->scene->dev.scenes, recs.data(), recs.size() * sizeof(SceneRec),
                       hipMemcpyHostToDevice), nullptr, 0);
     return 0;
+}
+
+NAN_API int nan_batch_set_frame_index(NanTracer *batch, uint32_t frame_index, void *stream) {
+    if (!batch || !batch->scene->batch) return fail(nullptr, 0, "nan_batch_set_frame_index: not a batch");
+    return set_frame_index(batch->scene, frame_index, stream);
 }
 
 NAN_API int nan_tracer_device_frame(NanTracer *tracer, int enable, uint32_t iteration, uint32_t reset) {

@@ -31,7 +31,8 @@ constexpr uint32_t kLeafCountShift = 27;
 constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
 constexpr uint32_t kMaxLeafSize = 8;
 constexpr uint32_t kNodeWords = 16;
-// Triangle record: 12 floats = 48 bytes: v0.xyz v1.x | v1.yz v2.xy | v2.z id pad pad
+// Triangle record: 12 floats = 48 bytes: v0.xyz v1.x | v1.yz v2.xy | v2.z id flags pad
+// (flags: written 0 here; the scene sets 1 for a triangle whose material is alpha tested)
 constexpr uint32_t kTriWords = 12;
 
 struct BuildOptions {

@@ -194,7 +194,8 @@ struct DevScene {
     const uint4 *instance_descs;
     const float4 *transforms;     // 4 float4 rows
     const float4 *inv_transposes; // 4 float4 rows
-    const float4 *materials;      // 2 float4: base.rgb flags | emissive.rgb alpha
+    // 3 float4: base.rgb flags | emissive.rgb alpha_mode | base_tex emissive_tex normal_tex cutoff
+    const float4 *materials;
     const float4 *tri_shading;    // 5 float4 per global triangle (load_triangle)
     uint32_t root;
     float slack;                  // absolute box-test slack (see kPad)
@@ -206,6 +207,16 @@ struct DevScene {
     const float *emitter_cdf;     // cumulative area fraction, last entry 1
     uint32_t emitter_count;
     float emitter_area;           // total emissive area
+    // textures (null without; see sample_texture)
+    const float4 *tri_uv;         // 2 float4 per global triangle: uv0 uv1 | uv2 0 0
+    const uint4 *textures;        // first texel, width, height, flags (bit 0: sRGB colour)
+    const uint32_t *texels;       // RGBA8, R in the low byte
+    const float *texel_lut;       // [0, 256): c / 255; [256, 512): sRGB decode of c / 255
+    const uint32_t *frame_index;  // frame index of the stochastic alpha test (device memory)
+    // 1: some material is textured or alpha tested.  Selects the kernels'
+    // MAT variants; the others never read a texture or test an alpha, so a
+    // scene without such materials keeps their registers and occupancy.
+    uint32_t mat_features;
 };
 
 struct Ray {
@@ -277,6 +288,123 @@ __device__ __forceinline__ bool tri_intersect(const Ray &r, f3 v0, f3 v1, f3 v2,
 struct TraversalStats {
     uint32_t nodes, tris;
 };
+
+// ---------------------------------------------------------------------------
+// textures and the alpha test (scene.slang:124-149, 313-368; common.slang:516-577)
+// ---------------------------------------------------------------------------
+// One axis of a bilinear footprint under wrap addressing: the two texels it
+// straddles and the weight of the second.  u is wrapped to [0, 1] first
+// (u - floor(u)), so x = u * n - 0.5 lies in [-0.5, n - 0.5] and no index
+// leaves the texture; a non-finite u reads as 0.
+__device__ __forceinline__ void wrap_axis(float u, uint32_t n, uint32_t &i0, uint32_t &i1, float &a) {
+    if (!(fabsf(u) <= 0x1.fffffep+127f)) u = 0.0f;
+    float f = u - floorf(u);
+    float x = f * (float)n - 0.5f;
+    float fl = floorf(x);
+    a = x - fl;
+    int k = (int)fl;
+    i0 = k < 0 ? n - 1u : (uint32_t)k;
+    i1 = i0 + 1u == n ? 0u : i0 + 1u;
+}
+
+__device__ __forceinline__ float4 texel_value(const float *lut, const float *colour_lut, uint32_t w) {
+    return make_float4(colour_lut[w & 0xFFu], colour_lut[(w >> 8) & 0xFFu], colour_lut[(w >> 16) & 0xFFu],
+                       lut[w >> 24]);
+}
+
+// SampleLevel(linear_sampler, uv, 0) of an RGBA8 texture: the four texels
+// decoded (c / 255, RGB through the sRGB curve for colour textures) and
+// blended along x, then y, one FMA per lerp.
+__device__ __forceinline__ float4 sample_texture(const DevScene &S, uint32_t tex, float u, float v) {
+    const uint4 d = S.textures[tex];
+    const float *clut = S.texel_lut + ((d.w & 1u) ? 256 : 0);
+    uint32_t x0, x1, y0, y1;
+    float ax, ay;
+    wrap_axis(u, d.y, x0, x1, ax);
+    wrap_axis(v, d.z, y0, y1, ay);
+    const uint32_t *row0 = S.texels + d.x + (size_t)y0 * d.y;
+    const uint32_t *row1 = S.texels + d.x + (size_t)y1 * d.y;
+    const float4 t00 = texel_value(S.texel_lut, clut, row0[x0]), t10 = texel_value(S.texel_lut, clut, row0[x1]);
+    const float4 t01 = texel_value(S.texel_lut, clut, row1[x0]), t11 = texel_value(S.texel_lut, clut, row1[x1]);
+    auto lerp2 = [&](float p00, float p10, float p01, float p11) {
+        float r0 = __builtin_fmaf(ax, p10 - p00, p00);
+        float r1 = __builtin_fmaf(ax, p11 - p01, p01);
+        return __builtin_fmaf(ay, r1 - r0, r0);
+    };
+    return make_float4(lerp2(t00.x, t10.x, t01.x, t11.x), lerp2(t00.y, t10.y, t01.y, t11.y),
+                       lerp2(t00.z, t10.z, t01.z, t11.z), lerp2(t00.w, t10.w, t01.w, t11.w));
+}
+
+// MurmurHash3's block mix and finaliser over the key (h, k) (Appleby,
+// MurmurHash3, public domain; common.slang:520-545).
+__device__ __forceinline__ uint32_t murmur_key_mix(uint32_t h, uint32_t k) {
+    k *= 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    h ^= k;
+    h = (h << 13) | (h >> 19);
+    h = h * 5u + 0xe6546b64u;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// float -> int32 by truncation, saturating, NaN -> 0 (the GPU's conversion,
+// spelled out so that no case depends on it)
+__device__ __forceinline__ int32_t to_i32_sat(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return -2147483647 - 1;
+    return (int32_t)x;
+}
+
+// Hash of the 1e-4-unit voxel holding p (position_hashing(p, 10000),
+// common.slang:547-564).
+__device__ __forceinline__ uint32_t position_hash(f3 p) {
+    uint32_t vx = ((uint32_t)to_i32_sat(p.x * 10000.0f) + 0x3ffffu) & 0x7ffffu;
+    uint32_t vy = ((uint32_t)to_i32_sat(p.y * 10000.0f) + 0x3ffffu) & 0x7ffffu;
+    uint32_t vz = ((uint32_t)to_i32_sat(p.z * 10000.0f) + 0x3ffffu) & 0x7ffffu;
+    uint32_t k0 = ((vx & 0x7fffu) << 7) | (vy & 0x7fu);
+    uint32_t k1 = ((vy & 0xfffu) << 20) | ((vz & 0x7fffu) << 1);
+    return murmur_key_mix(k0, k1);
+}
+
+// Alpha test of a candidate hit on a non-opaque triangle (eval_alpha_test,
+// scene.slang:124-149, as Scene.intersect applies it, 1025-1060): mask
+// keeps opacity >= cutoff; blend keeps opacity >= u, u from the hash of the
+// world position's voxel and the frame index.  Opacity is the base-colour
+// texture's alpha at the hit's uv, 1 without one.
+__device__ __forceinline__ bool alpha_accept(const DevScene &S, uint32_t tri, float b1, float b2) {
+    const float4 *rec = S.tri_shading + 5 * (size_t)tri;
+    const float4 r4 = rec[4];
+    const uint32_t mat = __float_as_uint(r4.w) & 0x7FFFFFFFu;
+    const float4 m1 = S.materials[3 * (size_t)mat + 1], m2 = S.materials[3 * (size_t)mat + 2];
+    const uint32_t mode = __float_as_uint(m1.w);
+    if (mode == 0u) return true;
+    const float w0 = 1.0f - b1 - b2;
+    float opacity = 1.0f;
+    const uint32_t base_tex = __float_as_uint(m2.x);
+    if (base_tex != kInvalid) {
+        const float4 *uvp = S.tri_uv + 2 * (size_t)tri;
+        const float4 u01 = uvp[0], u2 = uvp[1];
+        float u = w0 * u01.x + b1 * u01.z + b2 * u2.x;
+        float v = w0 * u01.y + b1 * u01.w + b2 * u2.y;
+        opacity = sample_texture(S, base_tex, u, v).w;
+    }
+    if (mode == 1u) return opacity >= m2.w;
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    const f3 p = mk(w0 * r0.x + b1 * r1.x + b2 * r2.x, w0 * r0.y + b1 * r1.y + b2 * r2.y,
+                    w0 * r0.z + b1 * r1.z + b2 * r2.z);
+    const float4 *M = S.transforms + 4 * (size_t)__float_as_uint(r4.z);
+    const float4 x0 = M[0], x1 = M[1], x2 = M[2];
+    const f3 wp = mk(x0.x * p.x + x0.y * p.y + x0.z * p.z + x0.w, x1.x * p.x + x1.y * p.y + x1.z * p.z + x1.w,
+                     x2.x * p.x + x2.y * p.y + x2.z * p.z + x2.w);
+    const uint32_t seed = murmur_key_mix(position_hash(wp), *S.frame_index);
+    return opacity >= (float)seed * 0x1p-32f;
+}
 
 // The children of a 4-wide node a ray enters: conservative slab tests on the
 // quantized boxes (decode origin + q * 2^e, one rounding), culled beyond
@@ -357,7 +485,7 @@ __device__ __forceinline__ int node_children(const uint4 &w0, const uint4 &w1, c
 
 // Stack of child entries in LDS: STACK rows of BLOCK lanes, lane-major so
 // the 64 lanes of a wave hit 64 different banks.
-template <int STACK, int BLOCK, bool ANY_HIT, bool COUNT>
+template <int STACK, int BLOCK, bool ANY_HIT, bool COUNT, bool MAT>
 __device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &hit, uint32_t *lds_col,
                                          TraversalStats &st) {
     uint32_t cur = S.root;
@@ -392,12 +520,14 @@ __device__ __forceinline__ bool traverse(const DevScene &S, const Ray &r, Hit &h
             float t, b1, b2;
             if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
                 uint32_t id = __float_as_uint(c.y);
+                bool tested = MAT && __float_as_uint(c.z) != 0u;  // non-opaque material
                 if (ANY_HIT) {
+                    if (tested && !alpha_accept(S, id, b1, b2)) continue;
                     hit.t = t;
                     hit.id = id;
                     return true;
                 }
-                if (t < hit.t || (t == hit.t && id < hit.id)) {
+                if ((t < hit.t || (t == hit.t && id < hit.id)) && (!tested || alpha_accept(S, id, b1, b2))) {
                     hit.t = t;
                     hit.b1 = b1;
                     hit.b2 = b2;
@@ -457,6 +587,7 @@ __device__ __forceinline__ void load_triangle(const DevScene &S, uint32_t tri, f
     odd = (m >> 31) != 0u;
 }
 
+template <bool MAT>
 __device__ __forceinline__ void shading_data(const DevScene &S, uint32_t tri, float b1, float b2, f3 view,
                                              Shading &sd) {
     f3 v0, v1, v2, nrm[3];
@@ -500,13 +631,48 @@ __device__ __forceinline__ void shading_data(const DevScene &S, uint32_t tri, fl
     f3 ni = nrm[0] * bc.x + nrm[1] * bc.y + nrm[2] * bc.z;
     f3 geometry_normal = normalize(mk(dot(row3(it0), ni), dot(row3(it1), ni), dot(row3(it2), ni)));
     f3 face_normal = world_normal;
-    float4 ma = S.materials[2 * (size_t)mat], mb = S.materials[2 * (size_t)mat + 1];
+    const float4 *mp = S.materials + 3 * (size_t)mat;
+    float4 ma = mp[0], mb = mp[1], mc = mp[2];
+    f3 diffuse = mk(ma.x, ma.y, ma.z), emissive = mk(mb.x, mb.y, mb.z);
+    f3 shading_normal = geometry_normal;
+    const uint32_t base_tex = __float_as_uint(mc.x), emissive_tex = __float_as_uint(mc.y),
+                   normal_tex = __float_as_uint(mc.z);
+    if (MAT && (base_tex & emissive_tex & normal_tex) != kInvalid) {
+        // textures at the hit's uv (scene.slang:297, 313-368): base colour
+        // replaces the factor, emission scales it, the normal map tilts the
+        // interpolated normal in a tangent frame from object-space dp/duv
+        const float4 *uvp = S.tri_uv + 2 * (size_t)tri;
+        const float4 u01 = uvp[0], u2 = uvp[1];
+        float u = bc.x * u01.x + bc.y * u01.z + bc.z * u2.x;
+        float v = bc.x * u01.y + bc.y * u01.w + bc.z * u2.y;
+        if (base_tex != kInvalid) {
+            float4 t = sample_texture(S, base_tex, u, v);
+            diffuse = mk(t.x, t.y, t.z);
+        }
+        if (emissive_tex != kInvalid) {
+            float4 t = sample_texture(S, emissive_tex, u, v);
+            emissive = mk(t.x * mb.x, t.y * mb.y, t.z * mb.z);
+        }
+        if (normal_tex != kInvalid) {
+            float4 t = sample_texture(S, normal_tex, u, v);
+            f3 tn = normalize(mk(t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f));
+            float d1u = u01.z - u01.x, d1v = u01.w - u01.y, d2u = u2.x - u01.x, d2v = u2.y - u01.y;
+            float det = d1u * d2v - d1v * d2u;
+            if (fabsf(det) > 1e-6f) {
+                f3 tg = e1 * d2v - e2 * d1v;
+                tg = normalize(mk(tg.x / det, tg.y / det, tg.z / det));
+                tg = mk(dot(row3(it0), tg), dot(row3(it1), tg), dot(row3(it2), tg));
+                tg = normalize(tg - geometry_normal * dot(tg, geometry_normal));
+                f3 bt = cross(geometry_normal, tg);
+                shading_normal = normalize(tg * tn.x + bt * tn.y + geometry_normal * tn.z);
+            }
+        }
+    }
     bool double_sided = (__float_as_uint(ma.w) & 1u) != 0;
     bool front = dot(face_normal, view) > 0.0f;
     float sign = (double_sided && !front) ? -1.0f : 1.0f;
     face_normal = face_normal * sign;
-    geometry_normal = geometry_normal * sign;
-    f3 shading_normal = geometry_normal;
+    shading_normal = shading_normal * sign;
     f3 D = -view;
     float dn = 2.0f * dot(shading_normal, D);
     f3 R = D - shading_normal * dn;
@@ -514,8 +680,8 @@ __device__ __forceinline__ void shading_data(const DevScene &S, uint32_t tri, fl
     if (k < 0.0f) shading_normal = normalize(normalize(R - face_normal * k) - D);
     sd.origin_front = wp + face_normal * world_offset;
     sd.frame = make_frame(shading_normal);
-    sd.diffuse = mk(ma.x, ma.y, ma.z);
-    sd.emissive = mk(mb.x, mb.y, mb.z);
+    sd.diffuse = diffuse;
+    sd.emissive = emissive;
 }
 
 // texture_space_path_tracer.slang:39-55

@@ -63,6 +63,7 @@ inline bool stream_capturing(hipStream_t s) {
 
 constexpr int kTraceBlock = 128;
 constexpr int kFlatBlock = 256;
+constexpr uint32_t kMaterialWords = 12;  // material record, 3 float4 (nan_hip.h)
 
 int wavefront_set_frame(NanTracer *t, uint32_t iteration, uint32_t reset);
 
@@ -85,6 +86,8 @@ struct NanScene {
     std::vector<uint32_t> h_nodes;
     std::vector<float> h_tris, h_shade, h_xf, h_itf;
     std::vector<uint32_t> h_mats;
+    std::vector<float> h_tri_uv;                // 8 floats per triangle; empty without textures
+    std::vector<uint32_t> h_textures, h_texels;  // texture table (4 words each) and RGBA8 texels
     bool batch = false;  // assembled from several scenes (wavefront only)
     float lighting[16] = {};            // the lighting record last set
     uint32_t roulette = 0;              // Russian roulette on this scene's paths (1: on)

@@ -57,6 +57,7 @@ __device__ __forceinline__ float power_weight(float a, float b) {
 
 // Uniform point over the emitters' total area: triangle by area CDF, point by
 // the square-root parametrisation.
+template <bool MAT>
 __device__ __forceinline__ bool sample_emitter(const DevScene &S, f3 o, float u0, float u1, float u2,
                                                EmitterSample &es) {
     uint32_t lo = 0, hi = S.emitter_count - 1;
@@ -79,6 +80,22 @@ __device__ __forceinline__ bool sample_emitter(const DevScene &S, f3 o, float u0
     if (!(cy > 1e-6f)) return false;
     es.pdf = d2 / (cy * S.emitter_area);
     es.le = mk(r4.x, r4.y, r4.z);
+    if (MAT && __float_as_uint(r4.w) != 0u) {
+        // emission at the sampled point (barycentrics 1 - a - b, a, b): the
+        // emissive texture times the factor, zero where the alpha test cuts
+        // the surface away
+        uint32_t tri = __float_as_uint(r1.w);
+        if (!alpha_accept(S, tri, a, b)) es.le = mk(0.0f, 0.0f, 0.0f);
+        const uint32_t mat = __float_as_uint(S.tri_shading[5 * (size_t)tri + 4].w) & 0x7FFFFFFFu;
+        const uint32_t etex = __float_as_uint(S.materials[3 * (size_t)mat + 2].y);
+        if (etex != kInvalid) {
+            const float4 *uvp = S.tri_uv + 2 * (size_t)tri;
+            const float4 u01 = uvp[0], u2 = uvp[1];
+            const float w0 = 1.0f - a - b;
+            float4 tx = sample_texture(S, etex, w0 * u01.x + a * u01.z + b * u2.x, w0 * u01.y + a * u01.w + b * u2.y);
+            es.le = mk(es.le.x * tx.x, es.le.y * tx.y, es.le.z * tx.z);
+        }
+    }
     return true;
 }
 
@@ -89,7 +106,7 @@ __device__ __forceinline__ float emitter_pdf(const DevScene &S, uint32_t tri, f3
     return cy > 1e-6f ? t * t / (cy * S.emitter_area) : INFINITY;
 }
 
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __device__ __forceinline__ bool visible_to(const DevScene &S, f3 o, f3 dir, float tmax, uint32_t *col,
                                            PathCounters &pc) {
     Ray r;
@@ -100,7 +117,7 @@ __device__ __forceinline__ bool visible_to(const DevScene &S, f3 o, f3 dir, floa
     ray_setup(r);
     Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
     TraversalStats st{0, 0};
-    bool occluded = traverse<STACK, BLOCK, true, COUNT>(S, r, h, col, st);
+    bool occluded = traverse<STACK, BLOCK, true, COUNT, MAT>(S, r, h, col, st);
     if (COUNT) {
         pc.shadow++;
         pc.shadow_nodes += st.nodes;
@@ -111,15 +128,15 @@ __device__ __forceinline__ bool visible_to(const DevScene &S, f3 o, f3 dir, floa
 
 // Light-sampled emission reaching the surface sd, MIS-weighted against the
 // cosine-sampled bounce; returns Le * cos / pdf * w (cosine-weighted, no BSDF).
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __device__ __forceinline__ f3 emitter_nee(const DevScene &S, const Shading &sd, Rng &rng, uint32_t *col,
                                           PathCounters &pc) {
     float u0 = rng.next_1d(), u1 = rng.next_1d(), u2 = rng.next_1d();
     EmitterSample es;
-    if (!sample_emitter(S, sd.origin_front, u0, u1, u2, es)) return mk(0.0f, 0.0f, 0.0f);
+    if (!sample_emitter<MAT>(S, sd.origin_front, u0, u1, u2, es)) return mk(0.0f, 0.0f, 0.0f);
     float cx = dot(sd.frame.n, es.dir);
     if (!(cx > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
-    if (!visible_to<STACK, BLOCK, COUNT>(S, sd.origin_front, es.dir, es.dist * (1.0f - 1e-4f), col, pc))
+    if (!visible_to<STACK, BLOCK, COUNT, MAT>(S, sd.origin_front, es.dir, es.dist * (1.0f - 1e-4f), col, pc))
         return mk(0.0f, 0.0f, 0.0f);
     float w = power_weight(es.pdf, cx * kInvPi);
     return es.le * (cx / es.pdf * w);
@@ -130,7 +147,7 @@ __device__ __forceinline__ f3 emitter_nee(const DevScene &S, const Shading &sd, 
 // sun term at y1 on, for the remaining max_bounces - 1 bounces.  With nee,
 // emission is also light-sampled at every vertex that has a successor, and
 // bounce-sampled emission carries the matching MIS weight.
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __device__ f3 suffix_radiance(const DevScene &S, Shading sd, uint32_t max_bounces, bool nee, Rng &rng,
                               uint32_t *col, PathCounters &pc) {
     const Light &L = S.light;
@@ -140,10 +157,10 @@ __device__ f3 suffix_radiance(const DevScene &S, Shading sd, uint32_t max_bounce
         f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
         float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
         f3 f = sd.diffuse * kInvPi;
-        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT, MAT>(S, sd.origin_front, ld, col, pc))
             rad = rad + T * ((f * L.sun_illum) * ndl);
         if (bounce + 1 >= max_bounces) break;
-        if (nee) rad = rad + T * (f * emitter_nee<STACK, BLOCK, COUNT>(S, sd, rng, col, pc));
+        if (nee) rad = rad + T * (f * emitter_nee<STACK, BLOCK, COUNT, MAT>(S, sd, rng, col, pc));
         float ex = rng.next_1d(), ey = rng.next_1d();
         f3 sdir;
         float pdf;
@@ -159,7 +176,7 @@ __device__ f3 suffix_radiance(const DevScene &S, Shading sd, uint32_t max_bounce
         ray_setup(r);
         Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
         TraversalStats st{0, 0};
-        bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
+        bool hit = traverse<STACK, BLOCK, false, COUNT, MAT>(S, r, h, col, st);
         if (COUNT) {
             pc.extend++;
             pc.nodes += st.nodes;
@@ -170,7 +187,7 @@ __device__ f3 suffix_radiance(const DevScene &S, Shading sd, uint32_t max_bounce
             break;
         }
         f3 dir = r.d;
-        shading_data(S, h.id, h.b1, h.b2, -dir, sd);
+        shading_data<MAT>(S, h.id, h.b1, h.b2, -dir, sd);
         if (sd.emissive.x > 0.0f || sd.emissive.y > 0.0f || sd.emissive.z > 0.0f) {
             float w = nee ? power_weight(pdf, emitter_pdf(S, h.id, dir, h.t)) : 1.0f;
             rad = rad + T * sd.emissive * w;
@@ -188,7 +205,7 @@ struct FirstHit {
 };
 
 // One cosine-sampled bounce from sd; on a hit fills y1's shading.
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __device__ __forceinline__ bool first_bounce(const DevScene &S, const Shading &sd, float ex, float ey, FirstHit &fh,
                                              Shading &y1, uint32_t *col, PathCounters &pc) {
     f3 sdir;
@@ -207,7 +224,7 @@ __device__ __forceinline__ bool first_bounce(const DevScene &S, const Shading &s
     fh.dir = r.d;
     Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
     TraversalStats st{0, 0};
-    bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
+    bool hit = traverse<STACK, BLOCK, false, COUNT, MAT>(S, r, h, col, st);
     if (COUNT) {
         pc.extend++;
         pc.nodes += st.nodes;
@@ -215,7 +232,7 @@ __device__ __forceinline__ bool first_bounce(const DevScene &S, const Shading &s
     }
     fh.t = h.t;
     fh.tri = h.id;
-    if (hit) shading_data(S, h.id, h.b1, h.b2, -fh.dir, y1);
+    if (hit) shading_data<MAT>(S, h.id, h.b1, h.b2, -fh.dir, y1);
     return hit;
 }
 
@@ -226,7 +243,7 @@ __device__ __forceinline__ bool first_bounce(const DevScene &S, const Shading &s
 // the path as target: (az/pdf) times the sky on a miss, else times the
 // suffix radiance of y1 plus y1's emission (MIS-weighted against the light
 // sampling x0 does when emitters are light-sampled).
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayout Ly, uint32_t iteration,
                                                               uint32_t spp, uint32_t max_bounces, uint32_t n,
                                                               uint32_t flags, LearnedDev B,
@@ -246,14 +263,14 @@ __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayo
         rng.state = st.rng;
         f3 nf = world_flat_normal(S, ts.tri);
         Shading sd0, y1;
-        shading_data(S, ts.tri, ts.b1, ts.b2, back ? -nf : nf, sd0);
+        shading_data<MAT>(S, ts.tri, ts.b1, ts.b2, back ? -nf : nf, sd0);
         float ex = rng.next_1d(), ey = rng.next_1d();
         FirstHit fh;
-        bool valid = first_bounce<STACK, BLOCK, COUNT>(S, sd0, ex, ey, fh, y1, col, pc);
+        bool valid = first_bounce<STACK, BLOCK, COUNT, MAT>(S, sd0, ex, ey, fh, y1, col, pc);
         f3 target = mk(0.0f, 0.0f, 0.0f);
         if (at_x0) {
             if (valid) {
-                target = suffix_radiance<STACK, BLOCK, COUNT>(S, y1, max_bounces, nee, rng, col, pc);
+                target = suffix_radiance<STACK, BLOCK, COUNT, MAT>(S, y1, max_bounces, nee, rng, col, pc);
                 float wm = nee ? power_weight(fh.pdf, emitter_pdf(S, fh.tri, fh.dir, fh.t)) : 1.0f;
                 target = (target + y1.emissive * wm) * fh.weight;
             } else if (fh.weight > 0.0f) {
@@ -264,7 +281,7 @@ __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayo
             put3(B.tr_nrm, j, sd0.frame.n);
             put3(B.tr_alb, j, sd0.diffuse);
         } else if (valid) {
-            target = suffix_radiance<STACK, BLOCK, COUNT>(S, y1, max_bounces, nee, rng, col, pc);
+            target = suffix_radiance<STACK, BLOCK, COUNT, MAT>(S, y1, max_bounces, nee, rng, col, pc);
             put3(B.tr_pos, j, y1.origin_front);
             put3(B.tr_nrm, j, y1.frame.n);
             put3(B.tr_alb, j, y1.diffuse);
@@ -294,7 +311,7 @@ __device__ __forceinline__ void texel_slots(const DevLayout &Ly, uint64_t w, uin
 // Every texel sample: sun term at x0, (with nee) light-sampled emission, and
 // the first bounce; queries for the cache go to q = slot * spp + k (weight 0:
 // no query).
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLayout Ly, uint32_t iteration,
                                                                uint32_t spp, uint32_t flags, LearnedDev B,
                                                                unsigned long long *counters) {
@@ -314,19 +331,19 @@ __global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLay
             if (side == 1 && !(ts.side.y & 1u)) break;
             uint32_t slot = side ? ts.side.x + ts.local : ts.global_texel;
             Shading sd, y1;
-            shading_data(S, ts.tri, ts.b1, ts.b2, side ? -nf : nf, sd);
+            shading_data<MAT>(S, ts.tri, ts.b1, ts.b2, side ? -nf : nf, sd);
             f3 acc = mk(0.0f, 0.0f, 0.0f);
             for (uint32_t k = 0; k < nspp; ++k) {
                 float ux = rng.next_1d(), uy = rng.next_1d();
                 f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
                 float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
-                if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+                if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT, MAT>(S, sd.origin_front, ld, col, pc))
                     acc = acc + L.sun_illum * ndl;
-                if (nee) acc = acc + emitter_nee<STACK, BLOCK, COUNT>(S, sd, rng, col, pc);
+                if (nee) acc = acc + emitter_nee<STACK, BLOCK, COUNT, MAT>(S, sd, rng, col, pc);
                 float ex = rng.next_1d(), ey = rng.next_1d();
                 FirstHit fh;
                 uint64_t q = (uint64_t)slot * nspp + k;
-                bool hit = first_bounce<STACK, BLOCK, COUNT>(S, sd, ex, ey, fh, y1, col, pc);
+                bool hit = first_bounce<STACK, BLOCK, COUNT, MAT>(S, sd, ex, ey, fh, y1, col, pc);
                 if (hit) {
                     if (y1.emissive.x > 0.0f || y1.emissive.y > 0.0f || y1.emissive.z > 0.0f) {
                         float wm = nee ? power_weight(fh.pdf, emitter_pdf(S, fh.tri, fh.dir, fh.t)) : 1.0f;
@@ -353,7 +370,7 @@ __global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLay
 // Texel-query form: every texel sample takes the sun term at x0 and (with
 // nee) MIS-weighted light-sampled emission, and queries the cache at x0
 // itself for the rest of its irradiance; no bounce ray is traced.
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __global__ __launch_bounds__(BLOCK) void learned_texel_direct(DevScene S, DevLayout Ly, uint32_t iteration,
                                                                uint32_t spp, uint32_t flags, LearnedDev B,
                                                                unsigned long long *counters) {
@@ -373,15 +390,15 @@ __global__ __launch_bounds__(BLOCK) void learned_texel_direct(DevScene S, DevLay
             if (side == 1 && !(ts.side.y & 1u)) break;
             uint32_t slot = side ? ts.side.x + ts.local : ts.global_texel;
             Shading sd;
-            shading_data(S, ts.tri, ts.b1, ts.b2, side ? -nf : nf, sd);
+            shading_data<MAT>(S, ts.tri, ts.b1, ts.b2, side ? -nf : nf, sd);
             f3 acc = mk(0.0f, 0.0f, 0.0f);
             for (uint32_t k = 0; k < nspp; ++k) {
                 float ux = rng.next_1d(), uy = rng.next_1d();
                 f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
                 float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
-                if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+                if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT, MAT>(S, sd.origin_front, ld, col, pc))
                     acc = acc + L.sun_illum * ndl;
-                if (nee) acc = acc + emitter_nee<STACK, BLOCK, COUNT>(S, sd, rng, col, pc);
+                if (nee) acc = acc + emitter_nee<STACK, BLOCK, COUNT, MAT>(S, sd, rng, col, pc);
                 uint64_t q = (uint64_t)slot * nspp + k;
                 put3(B.q_pos, q, sd.origin_front);
                 put3(B.q_nrm, q, sd.frame.n);
@@ -496,14 +513,19 @@ int launch_wavefront(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteratio
             const DevScene &S = t->scene->dev;
             DevLayout Ly = dev_layout(t);
             const LearnedDev &B = t->learned->d;
-            if (t->scene->stack_depth <= 32)
-                hipLaunchKernelGGL((learned_train_paths<32, kTraceBlock, false>), dim3(grid_for(n_train, kTraceBlock)),
-                                   dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, bounces, n_train, flags, B,
-                                   (unsigned long long *)nullptr);
-            else
-                hipLaunchKernelGGL((learned_train_paths<64, kTraceBlock, false>), dim3(grid_for(n_train, kTraceBlock)),
-                                   dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, bounces, n_train, flags, B,
-                                   (unsigned long long *)nullptr);
+#define NAN_TRAIN_PATHS(D, M)                                                                                   \
+    hipLaunchKernelGGL((learned_train_paths<D, kTraceBlock, false, M>), dim3(grid_for(n_train, kTraceBlock)),     \
+                       dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, bounces, n_train, flags, B,                \
+                       (unsigned long long *)nullptr)
+            const bool mat = S.mat_features != 0u;
+            if (t->scene->stack_depth <= 32) {
+                if (mat) NAN_TRAIN_PATHS(32, true);
+                else NAN_TRAIN_PATHS(32, false);
+            } else {
+                if (mat) NAN_TRAIN_PATHS(64, true);
+                else NAN_TRAIN_PATHS(64, false);
+            }
+#undef NAN_TRAIN_PATHS
             if (check_launch("learned training paths")) return -1;
         } else if (wavefront_learned_train_paths(t, io, iteration, n_train, bounces, at_x0, s)) {
             return -1;
@@ -525,9 +547,9 @@ int launch_wavefront(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteratio
     return 0;
 }
 
-template <int STACK>
-int launch(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
-           uint32_t reset, uint32_t n_train, uint32_t flags, unsigned long long *counters, hipStream_t s) {
+template <int STACK, bool MAT>
+int launch_mat(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
+               uint32_t reset, uint32_t n_train, uint32_t flags, unsigned long long *counters, hipStream_t s) {
     const DevScene &S = t->scene->dev;
     DevLayout Ly = dev_layout(t);
     const LearnedDev &B = t->learned->d;
@@ -536,10 +558,10 @@ int launch(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_
         StageTimer tm(t, s, kStageLearnedPaths);
         if (n_train) {
             if (counters)
-                hipLaunchKernelGGL((learned_train_paths<STACK, kTraceBlock, true>), dim3(grid_for(n_train, kTraceBlock)),
+                hipLaunchKernelGGL((learned_train_paths<STACK, kTraceBlock, true, MAT>), dim3(grid_for(n_train, kTraceBlock)),
                                    dim3(kTraceBlock), 0, s, S, Ly, iteration, nspp, bounces, n_train, flags, B, counters);
             else
-                hipLaunchKernelGGL((learned_train_paths<STACK, kTraceBlock, false>),
+                hipLaunchKernelGGL((learned_train_paths<STACK, kTraceBlock, false, MAT>),
                                    dim3(grid_for(n_train, kTraceBlock)), dim3(kTraceBlock), 0, s, S, Ly, iteration,
                                    nspp, bounces, n_train, flags, B, counters);
         }
@@ -554,17 +576,17 @@ int launch(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_
         bool at_x0 = (flags & NAN_LEARNED_TEXEL_QUERY) != 0;
         if (counters) {
             if (at_x0)
-                hipLaunchKernelGGL((learned_texel_direct<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0,
+                hipLaunchKernelGGL((learned_texel_direct<STACK, kTraceBlock, true, MAT>), dim3(grid), dim3(kTraceBlock), 0,
                                    s, S, Ly, iteration, nspp, flags, B, counters);
             else
-                hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0,
+                hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, true, MAT>), dim3(grid), dim3(kTraceBlock), 0,
                                    s, S, Ly, iteration, nspp, flags, B, counters);
         } else {
             if (at_x0)
-                hipLaunchKernelGGL((learned_texel_direct<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock),
+                hipLaunchKernelGGL((learned_texel_direct<STACK, kTraceBlock, false, MAT>), dim3(grid), dim3(kTraceBlock),
                                    0, s, S, Ly, iteration, nspp, flags, B, counters);
             else
-                hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, false>), dim3(grid), dim3(kTraceBlock),
+                hipLaunchKernelGGL((learned_first_bounce<STACK, kTraceBlock, false, MAT>), dim3(grid), dim3(kTraceBlock),
                                    0, s, S, Ly, iteration, nspp, flags, B, counters);
         }
     }
@@ -584,6 +606,14 @@ int launch(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_
                                reset, flags, counters);
     }
     return check_launch("learned tracer");
+}
+
+template <int STACK>
+int launch(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
+           uint32_t reset, uint32_t n_train, uint32_t flags, unsigned long long *counters, hipStream_t s) {
+    if (t->scene->dev.mat_features)
+        return launch_mat<STACK, true>(t, c, payload, iteration, spp, bounces, reset, n_train, flags, counters, s);
+    return launch_mat<STACK, false>(t, c, payload, iteration, spp, bounces, reset, n_train, flags, counters, s);
 }
 
 }  // namespace
@@ -681,12 +711,18 @@ NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint3
     } else {
         const DevScene &S = tracer->scene->dev;
         DevLayout Ly = dev_layout(tracer);
-        if (tracer->scene->stack_depth <= 32)
-            hipLaunchKernelGGL((learned_train_paths<32, kTraceBlock, false>), dim3(grid_for(n, kTraceBlock)),
-                               dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, max_bounces, n, flags, B, nullptr);
-        else
-            hipLaunchKernelGGL((learned_train_paths<64, kTraceBlock, false>), dim3(grid_for(n, kTraceBlock)),
-                               dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, max_bounces, n, flags, B, nullptr);
+#define NAN_TRAIN_PATHS(D, M)                                                                                \
+    hipLaunchKernelGGL((learned_train_paths<D, kTraceBlock, false, M>), dim3(grid_for(n, kTraceBlock)),        \
+                       dim3(kTraceBlock), 0, s, S, Ly, iteration, 1u, max_bounces, n, flags, B, nullptr)
+        const bool mat = S.mat_features != 0u;
+        if (tracer->scene->stack_depth <= 32) {
+            if (mat) NAN_TRAIN_PATHS(32, true);
+            else NAN_TRAIN_PATHS(32, false);
+        } else {
+            if (mat) NAN_TRAIN_PATHS(64, true);
+            else NAN_TRAIN_PATHS(64, false);
+        }
+#undef NAN_TRAIN_PATHS
         if (check_launch("learned training paths")) return -1;
     }
     size_t b3 = (size_t)n * 12;

@@ -11,7 +11,7 @@ struct PathCounters {
     uint32_t extend, shadow, nodes, tris, slots, shadow_nodes, shadow_tris;
 };
 
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __device__ __forceinline__ bool shadow_visible(const DevScene &S, f3 origin, f3 dir, uint32_t *col,
                                                PathCounters &pc) {
     Ray r;
@@ -22,7 +22,7 @@ __device__ __forceinline__ bool shadow_visible(const DevScene &S, f3 origin, f3 
     ray_setup(r);
     Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
     TraversalStats st{0, 0};
-    bool occluded = traverse<STACK, BLOCK, true, COUNT>(S, r, h, col, st);
+    bool occluded = traverse<STACK, BLOCK, true, COUNT, MAT>(S, r, h, col, st);
     if (COUNT) {
         pc.shadow++;
         pc.shadow_nodes += st.nodes;
@@ -45,7 +45,7 @@ __device__ __forceinline__ bool roulette_survives(f3 &T, Rng &rng) {
 }
 
 // trace_radiance (texture_space_path_tracer.slang:57-84)
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_bounces, Rng &rng, uint32_t *col,
                              PathCounters &pc, bool roulette = false) {
     const Light &L = S.light;
@@ -59,7 +59,7 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
         ray_setup(r);
         Hit h{r.tmax, 0.0f, 0.0f, kInvalid};
         TraversalStats st{0, 0};
-        bool hit = traverse<STACK, BLOCK, false, COUNT>(S, r, h, col, st);
+        bool hit = traverse<STACK, BLOCK, false, COUNT, MAT>(S, r, h, col, st);
         if (COUNT) {
             pc.extend++;
             pc.nodes += st.nodes;
@@ -67,12 +67,12 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
         }
         if (!hit) return rad + T * miss_radiance(L, dir);
         Shading sd;
-        shading_data(S, h.id, h.b1, h.b2, -dir, sd);
+        shading_data<MAT>(S, h.id, h.b1, h.b2, -dir, sd);
         rad = rad + T * sd.emissive;
         float ux = rng.next_1d(), uy = rng.next_1d();
         f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
         float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
-        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc)) {
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT, MAT>(S, sd.origin_front, ld, col, pc)) {
             f3 f = sd.diffuse * kInvPi;
             f3 li = (f * L.sun_illum) * ndl;
             rad = rad + T * li;
@@ -93,27 +93,27 @@ __device__ f3 trace_radiance(const DevScene &S, f3 origin, f3 dir, uint32_t max_
 }
 
 // estimate_surface_irradiance (texture_space_path_tracer.slang:86-117)
-template <int STACK, int BLOCK, bool COUNT>
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
 __device__ f3 estimate_irradiance(const DevScene &S, uint32_t tri, float b1, float b2, f3 view,
                                   uint32_t spp, uint32_t max_bounces, Rng &rng, uint32_t *col, PathCounters &pc,
                                   bool roulette = false) {
     const Light &L = S.light;
     Shading sd;
-    shading_data(S, tri, b1, b2, view, sd);
+    shading_data<MAT>(S, tri, b1, b2, view, sd);
     f3 acc = mk(0.0f, 0.0f, 0.0f);
     for (uint32_t k = 0; k < spp; ++k) {
         f3 sample = mk(0.0f, 0.0f, 0.0f);
         float ux = rng.next_1d(), uy = rng.next_1d();
         f3 ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
         float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
-        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT>(S, sd.origin_front, ld, col, pc))
+        if (L.sun_on && ndl > 0.0f && shadow_visible<STACK, BLOCK, COUNT, MAT>(S, sd.origin_front, ld, col, pc))
             sample = L.sun_illum * ndl;
         float ex = rng.next_1d(), ey = rng.next_1d();
         f3 sdir;
         float pdf;
         cosine_sample(ex, ey, sdir, pdf);
         if (pdf > 1e-6f) {
-            f3 rad = trace_radiance<STACK, BLOCK, COUNT>(S, sd.origin_front, to_global(sd.frame, sdir), max_bounces,
+            f3 rad = trace_radiance<STACK, BLOCK, COUNT, MAT>(S, sd.origin_front, to_global(sd.frame, sdir), max_bounces,
                                                          rng, col, pc, roulette);
             float az = fabsf(sdir.z);
             sample = sample + mk(rad.x * az / pdf, rad.y * az / pdf, rad.z * az / pdf);

@@ -230,7 +230,7 @@ __device__ __forceinline__ bool node_step(const DevScene &S, const Ray &r, float
 }
 
 // Leaf phase step: tests the parked leaf; returns true when the ray is done.
-template <bool ANY_HIT, bool COUNT>
+template <bool ANY_HIT, bool COUNT, bool MAT>
 __device__ __forceinline__ bool leaf_step(const DevScene &S, const Ray &r, Hit &hit, TravState &ts,
                                           TraversalStats &st) {
     if (ts.pend != kNone) {
@@ -244,12 +244,14 @@ __device__ __forceinline__ bool leaf_step(const DevScene &S, const Ray &r, Hit &
             float t, b1, b2;
             if (tri_intersect(r, mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), t, b1, b2)) {
                 uint32_t id = __float_as_uint(c.y);
+                bool tested = MAT && __float_as_uint(c.z) != 0u;  // non-opaque material
                 if (ANY_HIT) {
+                    if (tested && !alpha_accept(S, id, b1, b2)) continue;
                     hit.t = t;
                     hit.id = id;
                     return true;
                 }
-                if (t < hit.t || (t == hit.t && id < hit.id)) {
+                if ((t < hit.t || (t == hit.t && id < hit.id)) && (!tested || alpha_accept(S, id, b1, b2))) {
                     hit.t = t;
                     hit.b1 = b1;
                     hit.b2 = b2;
@@ -265,7 +267,7 @@ __device__ __forceinline__ bool leaf_step(const DevScene &S, const Ray &r, Hit &
 // ray has finished takes the next queued ray (one atomic per wave for all
 // refilling lanes), so a wave keeps tracing while any rays remain instead
 // of waiting for its slowest lane.
-template <int RING, bool ANY_HIT, bool COUNT, typename Load, typename Store>
+template <int RING, bool ANY_HIT, bool COUNT, bool MAT, typename Load, typename Store>
 __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counters, uint32_t count_slot,
                                             uint32_t fetch_slot, uint32_t part_cap, uint32_t *lds_col,
                                             uint32_t *spill_col, uint32_t spill_stride, uint32_t steps, Load load,
@@ -325,7 +327,7 @@ __device__ __forceinline__ void trace_queue(const DevScene &S, uint32_t *counter
             unsigned long long mb = __ballot(blocked), mm = __ballot(movable);
             bool leaf_phase = mb && (!mm || 2 * __popcll(mb) >= __popcll(mm));
             if (has) {
-                bool done = leaf_phase ? leaf_step<ANY_HIT, COUNT>(S, r, h, ts, st)
+                bool done = leaf_phase ? leaf_step<ANY_HIT, COUNT, MAT>(S, r, h, ts, st)
                                        : node_step<RING, ANY_HIT, COUNT>(S, r, slack, h, ts, stk, st);
                 if (done) {
                     store(k, h);
@@ -373,6 +375,7 @@ struct GenParams {
     uint32_t ext_slot, sh_slot;  // counter slots of the queues written
 };
 
+template <bool MAT>
 __global__ __launch_bounds__(kWfBlock, 6) void wf_generate(DevScene S, DevLayout Ly, WfBuffers W, GenParams P) {
     uint32_t stride = gridDim.x * kWfBlock;
     uint32_t *ext_count = queue_counter(W.counters, P.ext_slot, blockIdx.x % kParts);
@@ -392,7 +395,7 @@ __global__ __launch_bounds__(kWfBlock, 6) void wf_generate(DevScene S, DevLayout
             if (P.side != 0 || P.sample != 0) rng.state = W.rng[w];
             f3 n = world_flat_normal(S, ts.tri);
             Shading sd;
-            shading_data(S, ts.tri, ts.b1, ts.b2, P.side ? -n : n, sd);
+            shading_data<MAT>(S, ts.tri, ts.b1, ts.b2, P.side ? -n : n, sd);
             float ux = rng.next_1d(), uy = rng.next_1d();
             ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
             float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
@@ -425,7 +428,7 @@ __global__ __launch_bounds__(kWfBlock, 6) void wf_generate(DevScene S, DevLayout
     }
 }
 
-template <int RING, bool COUNT>
+template <int RING, bool COUNT, bool MAT>
 __global__ __launch_bounds__(kTravBlock, 8) void wf_extend(DevScene S, WfBuffers W, uint32_t q, uint32_t count_slot,
                                                           uint32_t fetch_slot, uint32_t steps,
                                                           unsigned long long *counters) {
@@ -434,7 +437,7 @@ __global__ __launch_bounds__(kTravBlock, 8) void wf_extend(DevScene S, WfBuffers
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
     const float4 *ro = W.ray_o[q], *rd = W.ray_d[q];
     float4 *hits = W.hits;
-    trace_queue<RING, false, COUNT>(
+    trace_queue<RING, false, COUNT, MAT>(
         S, W.counters, count_slot, fetch_slot, W.qcap, stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
         [&](uint32_t k, Ray &r, uint32_t &root, float &slack) {
             float4 o = ro[k], d = rd[k];
@@ -455,14 +458,14 @@ __global__ __launch_bounds__(kTravBlock, 8) void wf_extend(DevScene S, WfBuffers
     }
 }
 
-template <int RING, bool COUNT>
+template <int RING, bool COUNT, bool MAT>
 __global__ __launch_bounds__(kTravBlock, 8) void wf_shadow(DevScene S, WfBuffers W, uint32_t count_slot,
                                                           uint32_t fetch_slot, uint32_t steps,
                                                           unsigned long long *counters) {
     __shared__ uint32_t stack[RING * kTravBlock];
     uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
     uint32_t n_rays = 0, n_nodes = 0, n_tris = 0;
-    trace_queue<RING, true, COUNT>(
+    trace_queue<RING, true, COUNT, MAT>(
         S, W.counters, count_slot, fetch_slot, 2 * W.qcap, stack + threadIdx.x, W.spill + gid, W.spill_lanes, steps,
         [&](uint32_t k, Ray &r, uint32_t &root, float &slack) {
             float4 o = W.sh_o[k], d = W.sh_d[k];
@@ -508,6 +511,7 @@ struct ShadeParams {
 };
 
 // one iteration of trace_radiance's loop body (texture_space_path_tracer.slang:60-83)
+template <bool MAT>
 __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W, ShadeParams P) {
     uint32_t cnt[kParts];
     queue_counts(W.counters, P.in_slot, W.qcap, cnt);
@@ -543,7 +547,7 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W,
                 rng.state = W.rng[p];
                 f3 d = mk(rd.x, rd.y, rd.z);
                 Shading sd;
-                shading_data(S, id, hv.y, hv.z, -d, sd);
+                shading_data<MAT>(S, id, hv.y, hv.z, -d, sd);
                 if (!first_train) {
                     rad = rad + T * sd.emissive;
                 } else if (P.train == 1) {
@@ -653,6 +657,7 @@ struct LearnedGenParams {
 
 // Texel samples of one side: the sun shadow ray, then the first bounce ray
 // (its query is filled at the hit) or, at_x0, the cache query at the texel.
+template <bool MAT>
 __global__ __launch_bounds__(kWfBlock, 6) void wf_learned_generate(DevScene S, DevLayout Ly, WfBuffers W,
                                                                    LearnedIO io, LearnedGenParams P) {
     uint32_t stride = gridDim.x * kWfBlock;
@@ -673,7 +678,7 @@ __global__ __launch_bounds__(kWfBlock, 6) void wf_learned_generate(DevScene S, D
             if (P.side != 0) rng.state = W.rng[w];
             f3 n = world_flat_normal(S, ts.tri);
             Shading sd;
-            shading_data(S, ts.tri, ts.b1, ts.b2, P.side ? -n : n, sd);
+            shading_data<MAT>(S, ts.tri, ts.b1, ts.b2, P.side ? -n : n, sd);
             float ux = rng.next_1d(), uy = rng.next_1d();
             ld = cone_sample(L.sun_dir, L.cos_half, ux, uy);
             float ndl = fmaxf(0.0f, dot(sd.frame.n, ld));
@@ -719,6 +724,7 @@ __global__ __launch_bounds__(kWfBlock, 6) void wf_learned_generate(DevScene S, D
 
 // First hits of the learned bounce: the hit's emission (or the sky on a
 // miss) aside in rad.xyz, the hit's cache query at the path's query index.
+template <bool MAT>
 __global__ __launch_bounds__(kWfBlock, 5) void wf_learned_hit(DevScene S, WfBuffers W, LearnedIO io,
                                                               uint32_t in_slot) {
     uint32_t cnt[kParts];
@@ -741,7 +747,7 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_learned_hit(DevScene S, WfBuff
             term = miss_radiance(L, d);
         } else {
             Shading y1;
-            shading_data(S, id, hv.y, hv.z, -d, y1);
+            shading_data<MAT>(S, id, hv.y, hv.z, -d, y1);
             term = y1.emissive;
             put3(io.q_pos, i, y1.origin_front);
             put3(io.q_nrm, i, y1.frame.n);
@@ -790,6 +796,7 @@ __global__ __launch_bounds__(kWfBlock) void wf_learned_finish(DevLayout Ly, WfBu
 
 // Training paths: record j starts like learned_train_paths in nan_learned.hip
 // (texel, side, own RNG stream, cosine bounce, no sun term at x0).
+template <bool MAT>
 __global__ __launch_bounds__(kWfBlock, 6) void wf_train_generate(DevScene S, DevLayout Ly, WfBuffers W,
                                                                  LearnedIO io, uint32_t n, uint32_t iteration,
                                                                  uint32_t at_x0, uint32_t ext_slot) {
@@ -812,7 +819,7 @@ __global__ __launch_bounds__(kWfBlock, 6) void wf_train_generate(DevScene S, Dev
             scene = ts.scene;
             f3 nf = world_flat_normal(S, ts.tri);
             Shading sd0;
-            shading_data(S, ts.tri, ts.b1, ts.b2, back ? -nf : nf, sd0);
+            shading_data<MAT>(S, ts.tri, ts.b1, ts.b2, back ? -nf : nf, sd0);
             float ex = rng.next_1d(), ey = rng.next_1d();
             f3 sdir;
             float pdf;
@@ -909,11 +916,11 @@ void wavefront_release(NanTracer *t) {
     t->wf = nullptr;
 }
 
-template <int RING>
+template <int RING, bool MAT>
 static int occupancy_blocks(int *per_cu) {
     int a = 0, b = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_extend<RING, false>, kTravBlock, 0), nullptr, 0);
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_shadow<RING, false>, kTravBlock, 0), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_extend<RING, false, MAT>, kTravBlock, 0), nullptr, 0);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_shadow<RING, false, MAT>, kTravBlock, 0), nullptr, 0);
     *per_cu = std::max(1, std::min(a, b));
     return 0;
 }
@@ -957,14 +964,22 @@ static int wavefront_alloc(NanTracer *t) {
     int dev = 0, cus = 0, per_cu = 1;
     HIP_TRY(hipGetDevice(&dev), nullptr, 0);
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), nullptr, 0);
-    int rc = W->ring == 16 ? occupancy_blocks<16>(&per_cu)
-             : W->ring == 24 ? occupancy_blocks<24>(&per_cu) : occupancy_blocks<32>(&per_cu);
+    // the kernel variants the scene runs (MAT: textured or alpha-tested materials)
+    const bool mat = t->scene->dev.mat_features != 0u;
+    int rc = W->ring == 16 ? (mat ? occupancy_blocks<16, true>(&per_cu) : occupancy_blocks<16, false>(&per_cu))
+             : W->ring == 24 ? (mat ? occupancy_blocks<24, true>(&per_cu) : occupancy_blocks<24, false>(&per_cu))
+                             : (mat ? occupancy_blocks<32, true>(&per_cu) : occupancy_blocks<32, false>(&per_cu));
     if (rc) return rc;
     if (tu.blocks_per_cu) per_cu = std::min(per_cu, (int)tu.blocks_per_cu);
     W->trav_blocks = (uint32_t)(cus * per_cu);
     int og = 1, os = 1, of = 1;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&og, wf_generate, kWfBlock, 0), nullptr, 0);
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os, wf_shade, kWfBlock, 0), nullptr, 0);
+    if (mat) {
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&og, wf_generate<true>, kWfBlock, 0), nullptr, 0);
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os, wf_shade<true>, kWfBlock, 0), nullptr, 0);
+    } else {
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&og, wf_generate<false>, kWfBlock, 0), nullptr, 0);
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os, wf_shade<false>, kWfBlock, 0), nullptr, 0);
+    }
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&of, wf_finish<false>, kWfBlock, 0), nullptr, 0);
     auto parts_multiple = [](int blocks) { return (uint32_t)std::max<int>(kParts, blocks / kParts * kParts); };
     W->gen_blocks = parts_multiple(cus * std::max(1, og));
@@ -1031,29 +1046,46 @@ static uint32_t trav_grid(const WfBuffers &W, uint64_t max_rays) {
     return (uint32_t)std::min<uint64_t>(W.trav_blocks, std::max<uint64_t>(need, kParts));
 }
 
+// Launches kernel<MAT> with MAT = the scene's material features.
+#define NAN_LAUNCH_MAT(kernel, grid, block, stream, ...)                                 \
+    do {                                                                                 \
+        if (S.mat_features) hipLaunchKernelGGL(kernel<true>, grid, block, 0, stream, __VA_ARGS__);  \
+        else hipLaunchKernelGGL(kernel<false>, grid, block, 0, stream, __VA_ARGS__);     \
+    } while (0)
+
 template <int RING>
 static void launch_extend(const WfBuffers &W, const DevScene &S, uint32_t q, uint32_t count_slot, uint32_t fetch,
                           uint32_t steps, unsigned long long *counters, hipStream_t stream,
                           uint64_t max_rays = ~0ull) {
     uint32_t blocks = trav_grid(W, max_rays);
-    if (counters)
-        hipLaunchKernelGGL((wf_extend<RING, true>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, q, count_slot,
-                           fetch, steps, counters);
-    else
-        hipLaunchKernelGGL((wf_extend<RING, false>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, q, count_slot,
-                           fetch, steps, counters);
+#define NAN_EXTEND(C, M)                                                                                    \
+    hipLaunchKernelGGL((wf_extend<RING, C, M>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, q, count_slot, \
+                       fetch, steps, counters)
+    if (counters) {
+        if (S.mat_features) NAN_EXTEND(true, true);
+        else NAN_EXTEND(true, false);
+    } else {
+        if (S.mat_features) NAN_EXTEND(false, true);
+        else NAN_EXTEND(false, false);
+    }
+#undef NAN_EXTEND
 }
 
 template <int RING>
 static void launch_shadow(const WfBuffers &W, const DevScene &S, uint32_t count_slot, uint32_t fetch, uint32_t steps,
                           unsigned long long *counters, hipStream_t stream, uint64_t max_rays = ~0ull) {
     uint32_t blocks = trav_grid(W, max_rays);
-    if (counters)
-        hipLaunchKernelGGL((wf_shadow<RING, true>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, count_slot,
-                           fetch, steps, counters);
-    else
-        hipLaunchKernelGGL((wf_shadow<RING, false>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, count_slot,
-                           fetch, steps, counters);
+#define NAN_SHADOW(C, M)                                                                                    \
+    hipLaunchKernelGGL((wf_shadow<RING, C, M>), dim3(blocks), dim3(kTravBlock), 0, stream, S, W, count_slot, \
+                       fetch, steps, counters)
+    if (counters) {
+        if (S.mat_features) NAN_SHADOW(true, true);
+        else NAN_SHADOW(true, false);
+    } else {
+        if (S.mat_features) NAN_SHADOW(false, true);
+        else NAN_SHADOW(false, false);
+    }
+#undef NAN_SHADOW
 }
 
 // Queue counter slots one sample of one sweep takes: the first ray and
@@ -1093,7 +1125,7 @@ static int run_sweeps(NanTracer *t, WfBuffers &W, float4 *payload, const uint32_
         GenParams gp{t->device_frame ? W.frame : nullptr, list, count, side, k, iteration, spp, ext_slot, sh_slot};
         {
             StageTimer tm(t, stream, kStageGenerate);
-            hipLaunchKernelGGL(wf_generate, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, Ly, W, gp);
+            NAN_LAUNCH_MAT(wf_generate, dim3(gen_blocks), dim3(kWfBlock), stream, S, Ly, W, gp);
         }
         uint32_t q = 0;
         for (uint32_t b = 1; b <= bounces; ++b) {
@@ -1108,7 +1140,7 @@ static int run_sweeps(NanTracer *t, WfBuffers &W, float4 *payload, const uint32_
             ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces};
             {
                 StageTimer tm(t, stream, kStageShade);
-                hipLaunchKernelGGL(wf_shade, dim3(shade_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
+                NAN_LAUNCH_MAT(wf_shade, dim3(shade_blocks), dim3(kWfBlock), stream, S, W, sp);
             }
             uint32_t sfetch = slot++;
             {
@@ -1248,8 +1280,8 @@ int wavefront_learned_train_paths(NanTracer *t, const LearnedIO &io, uint32_t it
     uint32_t slot = 0;
     uint32_t ext_slot = slot;
     slot += 2;  // the generator's ray queue and an unused second queue
-    hipLaunchKernelGGL(wf_train_generate, dim3(gen_blocks), dim3(kWfBlock), 0, stream, S, Ly, W, io, n, iteration,
-                       at_x0 ? 1u : 0u, ext_slot);
+    NAN_LAUNCH_MAT(wf_train_generate, dim3(gen_blocks), dim3(kWfBlock), stream, S, Ly, W, io, n, iteration,
+                   at_x0 ? 1u : 0u, ext_slot);
     uint32_t sh_slot = slot++;
     uint32_t q = 0;
     for (uint32_t b = 1; b <= bounces; ++b) {
@@ -1257,7 +1289,7 @@ int wavefront_learned_train_paths(NanTracer *t, const LearnedIO &io, uint32_t it
         launch_extend_any(W, S, q, ext_slot, fetch, steps, stream, n);
         uint32_t next_ext = slot++;
         ShadeParams sp{q, 1u - q, ext_slot, next_ext, sh_slot, b, bounces, at_x0 ? 2u : 1u, io};
-        hipLaunchKernelGGL(wf_shade, dim3(shade_blocks), dim3(kWfBlock), 0, stream, S, W, sp);
+        NAN_LAUNCH_MAT(wf_shade, dim3(shade_blocks), dim3(kWfBlock), stream, S, W, sp);
         uint32_t sfetch = slot++;
         launch_shadow_any(W, S, sh_slot, sfetch, steps, stream, n);
         sh_slot = slot++;
@@ -1283,12 +1315,12 @@ int wavefront_learned_side(NanTracer *t, const LearnedIO &io, uint32_t iteration
     StageTimer tm(t, stream, kStageFirstBounce);
     hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream, W.counters, kCounterWords);
     LearnedGenParams gp{side ? t->ds_list : nullptr, count, side, iteration, 0u, 1u, at_x0 ? 1u : 0u};
-    hipLaunchKernelGGL(wf_learned_generate, dim3(std::min<uint32_t>(W.gen_blocks, round8)), dim3(kWfBlock), 0, stream,
-                       S, Ly, W, io, gp);
+    NAN_LAUNCH_MAT(wf_learned_generate, dim3(std::min<uint32_t>(W.gen_blocks, round8)), dim3(kWfBlock), stream, S, Ly,
+                   W, io, gp);
     if (!at_x0) {
         launch_extend_any(W, S, 0u, 0u, 2u, steps, stream, count);
-        hipLaunchKernelGGL(wf_learned_hit, dim3(std::min<uint32_t>(W.shade_blocks, round8)), dim3(kWfBlock), 0,
-                           stream, S, W, io, 0u);
+        NAN_LAUNCH_MAT(wf_learned_hit, dim3(std::min<uint32_t>(W.shade_blocks, round8)), dim3(kWfBlock), stream, S,
+                       W, io, 0u);
     }
     launch_shadow_any(W, S, 1u, 3u, steps, stream, count);
     return check_launch("wavefront learned samples");

@@ -48,6 +48,8 @@ class Scene:
             _f32(a["transforms"]), _f32(a["inv_transposes"]), a["transforms"].shape[0],
             _u32(a["materials"]), a["materials"].shape[0],
             _f32(a["lighting"]),
+            _u32(a["textures"]), a["textures"].shape[0],
+            _u32(a["texels"]), a["texels"].size,
         )
         opts = _native.NanBvhOptions(sah_bins, max_leaf_size, 0)
         handle = ctypes.c_void_p()
@@ -105,8 +107,17 @@ class Scene:
         return int(self.info.triangle_count)
 
     def update(self) -> None:
-        """Per-frame scene update (scene.py:1213-1232): the frame counter."""
-        self.frame_index += 1
+        """Per-frame scene update (scene.py:1213-1232): advances the frame
+        index that the stochastic alpha test of blend materials hashes."""
+        self.set_frame_index(self.frame_index + 1)
+
+    def set_frame_index(self, frame_index: int) -> None:
+        """Sets the blend alpha test's frame index (scene.slang:146-148),
+        written on the device's stream."""
+        self.frame_index = int(frame_index) & 0xFFFFFFFF
+        _native.check(_native.lib().nan_scene_set_frame_index(self.handle, self.frame_index,
+                                                               self.device.stream_handle),
+                      "nan_scene_set_frame_index")
 
     def set_lighting(self, lighting) -> None:
         self.scene_node.lighting = lighting

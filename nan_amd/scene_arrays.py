@@ -7,15 +7,20 @@ library consumes (the buffers scene.py:330-437 of the reference uploads).
 * ``instance_descs``  uint32[I,4]   mesh_id, material_id, transform_id, flags
 * ``transforms``      float32[X,16] row-major object-to-world
 * ``inv_transposes``  float32[X,16] row-major transpose(inverse(M))
-* ``materials``       uint32[Mat,8] see ``Material.pack``
+* ``materials``       uint32[Mat,12] see ``Material.pack``
 * ``lighting``        float32[16]   see ``Lighting.pack``
+* ``textures``        uint32[Tx,4]  first texel, width, height, flags (``textures.pack_textures``)
+* ``texels``          uint32[N]     RGBA8 texels of every texture
 """
 
 from __future__ import annotations
 
+from pathlib import Path
+
 import numpy as np
 
-from .material import MATERIAL_RECORD_WORDS
+from .material import MATERIAL_RECORD_WORDS, TEXTURE_SLOTS
+from .textures import NO_TEXTURE, load_image, pack_textures
 from .scene_node import SceneNode
 from .transform import inverse_transpose, is_odd_negative
 
@@ -41,10 +46,7 @@ def flatten_scene(scene_node: SceneNode) -> dict:
     instance_descs = np.zeros((len(inst), 4), dtype=np.uint32)
     instance_descs[:, 0:3] = inst
     instance_descs[:, 3] = np.where(odd[inst[:, 2]], INSTANCE_FLAG_ODD_NEGATIVE_TRANSFORM, 0)
-    if scene_node.materials:
-        materials = np.stack([m.pack() for m in scene_node.materials]).astype(np.uint32)
-    else:
-        materials = np.zeros((0, MATERIAL_RECORD_WORDS), dtype=np.uint32)
+    materials, textures, texels = _pack_materials(scene_node.materials)
     return {
         "vertices": np.ascontiguousarray(vertices),
         "indices": np.ascontiguousarray(indices),
@@ -55,7 +57,48 @@ def flatten_scene(scene_node: SceneNode) -> dict:
         "materials": materials,
         "lighting": scene_node.lighting.pack(),
         "russian_roulette": bool(getattr(scene_node, "russian_roulette", False)),
+        "textures": textures,
+        "texels": texels,
     }
+
+
+def _texture_key(src, srgb: bool):
+    if isinstance(src, bytes):
+        key = src
+    elif isinstance(src, str) or hasattr(src, "__fspath__"):
+        key = str(Path(src).resolve())
+    else:
+        key = id(src)  # an array: one texture per object
+    return key, srgb
+
+
+def _pack_materials(material_list) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Material records and the scene's textures.  A texture source used
+    by several materials is loaded once per colour space (the reference's
+    TextureManager caches by path, texture_manager.py:84-89)."""
+    index: dict = {}
+    images, srgb = [], []
+    records = []
+    for m in material_list:
+        ids = []
+        for name, is_srgb in TEXTURE_SLOTS:
+            src = getattr(m, name, None)
+            if src is None:
+                ids.append(NO_TEXTURE)
+                continue
+            key = _texture_key(src, is_srgb)
+            if key not in index:
+                index[key] = len(images)
+                images.append(load_image(src))
+                srgb.append(is_srgb)
+            ids.append(index[key])
+        records.append(m.pack(tuple(ids)))
+    if records:
+        materials = np.stack(records).astype(np.uint32)
+    else:
+        materials = np.zeros((0, MATERIAL_RECORD_WORDS), dtype=np.uint32)
+    texels, textures = pack_textures(images, srgb)
+    return materials, textures, texels
 
 
 def instance_triangle_offsets(arrays: dict) -> np.ndarray:

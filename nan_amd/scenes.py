@@ -14,6 +14,10 @@
   (the 4M-triangle scene uses ``scale=3.3``).
 * :func:`batch_scene` - a small seeded mix of the same primitives with a
   requested triangle count, for many-scene batches.
+* :func:`textured_scene` - the batch mix with textured materials: a noise
+  base-colour texture and a bump normal map on the terrain, checker boxes,
+  alpha-masked double-sided cards with uvs that wrap, blend panes, spheres
+  with emissive textures, and texture sizes from 1x1 to non-powers of two.
 """
 
 from __future__ import annotations
@@ -273,6 +277,126 @@ def batch_scene(seed: int, triangles: int) -> SceneNode:
         y = _terrain_height(terrain, quads, extent, x, z) + r + rng.uniform(0.0, 1.5)
         node.add_instance(sphere, material(k), node.add_transform(Transform((x, y, z), (0, 0, 0), (r, r, r))))
         k += 1
+    node.camera.position = np.array([0.0, 6.0, 0.6 * extent])
+    node.camera.target = np.array([0.0, 0.0, 0.0])
+    node.camera.recompute()
+    return node
+
+
+# ---------------------------------------------------------------------------
+# textured scene
+# ---------------------------------------------------------------------------
+def _noise_texture(rng: np.random.Generator, w: int, h: int, alpha=None) -> np.ndarray:
+    """RGBA8 value noise, alpha 255 unless given (a scalar or an array)."""
+    ys, xs = np.meshgrid(np.linspace(0, 4, h), np.linspace(0, 4, w), indexing="ij")
+    img = np.zeros((h, w, 4), dtype=np.uint8)
+    for c in range(3):
+        v = _value_noise(rng, xs, ys, 3)
+        img[:, :, c] = np.clip(40 + 200 * v, 0, 255).astype(np.uint8)
+    img[:, :, 3] = 255 if alpha is None else alpha
+    return img
+
+
+def _checker(w: int, h: int, cells: int, a, b) -> np.ndarray:
+    ys, xs = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    on = ((xs * cells // w) + (ys * cells // h)) % 2 == 0
+    img = np.where(on[:, :, None], np.asarray(a, dtype=np.uint8), np.asarray(b, dtype=np.uint8))
+    return np.ascontiguousarray(img, dtype=np.uint8)
+
+
+def _bump_normal_map(rng: np.random.Generator, w: int, h: int, strength: float) -> np.ndarray:
+    """Tangent-space normal map (RGB = n * 0.5 + 0.5) of a noise height."""
+    ys, xs = np.meshgrid(np.linspace(0, 3, h), np.linspace(0, 3, w), indexing="ij")
+    height = _value_noise(rng, xs, ys, 2) * strength
+    gy, gx = np.gradient(height)
+    n = np.stack([-gx * w / 3, -gy * h / 3, np.ones_like(height)], axis=-1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    img = np.zeros((h, w, 4), dtype=np.uint8)
+    img[:, :, :3] = np.clip(np.rint((n * 0.5 + 0.5) * 255), 0, 255).astype(np.uint8)
+    img[:, :, 3] = 255
+    return img
+
+
+def _card_mesh(u_repeat: float) -> Mesh:
+    """XY quad (two triangles) whose uvs run over [-u_repeat/2, u_repeat/2]
+    x [0, 1], so a texture wraps across it."""
+    m = Mesh.create_quad_xy((1.0, 1.0))
+    v = m.vertices.copy()
+    v[:, 6] = (v[:, 6] - 0.5) * np.float32(u_repeat)
+    return Mesh(v, m.indices.copy())
+
+
+def textured_scene(seed: int, triangles: int = 6000) -> SceneNode:
+    """A seeded textured scene of about ``triangles`` triangles: the terrain
+    of :func:`batch_scene` with a base-colour texture and a normal map,
+    checker boxes (some with normal maps), alpha-masked double-sided cards,
+    blend panes (textured and untextured), spheres with emissive textures,
+    an untextured blend material (always kept) and a mask material whose
+    cutoff exceeds 1 (never kept)."""
+    from .material import ALPHA_MODE_BLEND, ALPHA_MODE_MASK
+
+    rng = np.random.default_rng(seed)
+    node = SceneNode()
+    node.lighting = Lighting.sun_and_sky(float(rng.uniform(25, 60)), float(rng.uniform(0, 360)), sky=0.3)
+    extent = 16.0
+    quads = max(4, int(math.sqrt(max(8.0, 0.5 * triangles / 2))))
+    terrain = heightfield_mesh(rng, quads, extent, 1.0)
+    identity = node.add_transform(Transform())
+    terrain_mat = node.add_material(Material(
+        base_color=(0.5, 0.5, 0.5), base_color_texture=_noise_texture(rng, 64, 48),
+        normal_texture=_bump_normal_map(rng, 37, 23, 2.0)))
+    node.add_instance(node.add_mesh(terrain), terrain_mat, identity)
+
+    checker = _checker(16, 16, 4, (200, 180, 60, 255), (40, 90, 160, 255))
+    bumps = _bump_normal_map(rng, 8, 8, 1.5)
+    box_mats = [
+        node.add_material(Material(base_color_texture=checker)),
+        node.add_material(Material(base_color_texture=checker, normal_texture=bumps)),
+        node.add_material(Material(base_color=(0.6, 0.3, 0.2))),
+        node.add_material(Material(base_color_texture=np.array([[[230, 230, 230, 255]]], dtype=np.uint8))),
+    ]
+    holes = _checker(32, 32, 8, (60, 160, 50, 255), (60, 160, 50, 0))
+    leaf_alpha = np.clip(rng.normal(0.55, 0.35, size=(24, 40)) * 255, 0, 255).astype(np.uint8)
+    card_mats = [
+        node.add_material(Material(base_color_texture=holes, alpha_mode=ALPHA_MODE_MASK, alpha_cutoff=0.5,
+                                   double_sided=True)),
+        node.add_material(Material(base_color_texture=_noise_texture(rng, 40, 24, leaf_alpha),
+                                   alpha_mode=ALPHA_MODE_MASK, alpha_cutoff=0.3, double_sided=True)),
+        node.add_material(Material(base_color_texture=_noise_texture(rng, 13, 29, np.tile(leaf_alpha, (2, 1))[:29, :13]),
+                                   alpha_mode=ALPHA_MODE_BLEND, double_sided=True)),
+        node.add_material(Material(base_color=(0.8, 0.8, 0.9), alpha_mode=ALPHA_MODE_BLEND, double_sided=True)),
+        node.add_material(Material(base_color=(0.9, 0.1, 0.1), alpha_mode=ALPHA_MODE_MASK, alpha_cutoff=1.5)),
+    ]
+    glow = _noise_texture(rng, 9, 5)
+    sphere_mats = [
+        node.add_material(Material(base_color=(0.7, 0.7, 0.7), emissive=(3.0, 2.0, 1.0), emissive_texture=glow)),
+        node.add_material(Material(base_color_texture=_noise_texture(rng, 20, 20))),
+    ]
+    n_boxes = max(2, int(0.2 * triangles / 12))
+    n_cards = max(5, int(0.1 * triangles / 2))
+    n_spheres = max(2, int(0.2 * triangles / 320))
+    box = node.add_mesh(box_mesh())
+    for k in range(n_boxes):
+        x, z = rng.uniform(-0.45 * extent, 0.45 * extent, size=2)
+        sx, sz = rng.uniform(0.2, 0.8, size=2)
+        h = float(min(rng.lognormal(0.0, 0.6), 4.0))
+        y = _terrain_height(terrain, quads, extent, x, z) - 0.1
+        node.add_instance(box, box_mats[k % len(box_mats)],
+                          node.add_transform(Transform((x, y, z), (0, 0, 0), (sx, h, sz))))
+    cards = [node.add_mesh(_card_mesh(r)) for r in (1.0, 2.5)]
+    for k in range(n_cards):
+        x, z = rng.uniform(-0.45 * extent, 0.45 * extent, size=2)
+        w, h = rng.uniform(0.5, 2.0, size=2)
+        y = _terrain_height(terrain, quads, extent, x, z) + 0.5 * h + rng.uniform(0.0, 0.5)
+        rot = (float(rng.uniform(-20, 20)), float(rng.uniform(0, 360)), 0.0)
+        node.add_instance(cards[k % 2], card_mats[k % len(card_mats)],
+                          node.add_transform(Transform((x, y, z), rot, (w, h, 1.0))))
+    sphere = node.add_mesh(icosphere_mesh(2))
+    for k in range(n_spheres):
+        r = rng.uniform(0.2, 0.8)
+        x, z = rng.uniform(-0.45 * extent, 0.45 * extent, size=2)
+        y = _terrain_height(terrain, quads, extent, x, z) + r + rng.uniform(0.0, 1.5)
+        node.add_instance(sphere, sphere_mats[k % 2], node.add_transform(Transform((x, y, z), (0, 0, 0), (r, r, r))))
     node.camera.position = np.array([0.0, 6.0, 0.6 * extent])
     node.camera.target = np.array([0.0, 0.0, 0.0])
     node.camera.recompute()

@@ -34,6 +34,8 @@ class SceneDesc(ctypes.Structure):
         ("transforms", f32p), ("inv_transposes", f32p), ("transform_count", ctypes.c_uint32),
         ("materials", u32p), ("material_count", ctypes.c_uint32),
         ("lighting", f32p),
+        ("textures", u32p), ("texture_count", ctypes.c_uint32),
+        ("texels", u32p), ("texel_count", ctypes.c_uint64),
     ]
 
 
@@ -86,6 +88,10 @@ def liboracle():
     lib.orc_crosscheck_result.argtypes = [f32p]
     lib.orc_scene_set_sky_view_lut.argtypes = [ctypes.c_void_p, f32p]
     lib.orc_scene_set_russian_roulette.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_scene_set_frame_index.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.orc_sample_texture.argtypes = [ctypes.c_void_p, ctypes.c_uint32, f32p, ctypes.c_uint64, f32p]
+    lib.orc_alpha_hash.argtypes = [f32p, ctypes.c_uint64, ctypes.c_uint32, u32p]
+    lib.orc_texel_lut.argtypes = [f32p]
     lib.orc_sky_view_lookup.argtypes = [f32p, f32p, f32p, ctypes.c_uint64, f32p]
     lib.orc_atmo_scattering.argtypes = [f32p, f32p]
     lib.orc_atmo_sphere_hit.restype = ctypes.c_float
@@ -188,6 +194,23 @@ def sincos_2pi(u: np.ndarray):
     return s, c
 
 
+def texel_lut() -> np.ndarray:
+    """The oracle's texel decode table float32[512]: c / 255, then the sRGB
+    decode of c / 255."""
+    out = np.zeros(512, dtype=np.float32)
+    liboracle().orc_texel_lut(_p(out, f32p))
+    return out
+
+
+def alpha_hash(positions: np.ndarray, frame_index: int) -> np.ndarray:
+    """murmur_key_mix(position_hashing(p, 10000), frame_index) per
+    position float32[n, 3] (common.slang:540-577)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(len(pos), dtype=np.uint32)
+    liboracle().orc_alpha_hash(_p(pos, f32p), len(pos), int(frame_index) & 0xFFFFFFFF, _p(out, u32p))
+    return out
+
+
 class OracleScene:
     """One scene in the oracle: world-space triangles plus an optional BVH.
 
@@ -207,6 +230,8 @@ class OracleScene:
             _p(a["transforms"], f32p), _p(a["inv_transposes"], f32p), a["transforms"].shape[0],
             _p(a["materials"], u32p), a["materials"].shape[0],
             _p(a["lighting"], f32p),
+            _p(a["textures"], u32p), a["textures"].shape[0],
+            _p(a["texels"], u32p), a["texels"].size,
         )
         self.handle = self.lib.orc_scene_create(ctypes.byref(self.desc), 1 if use_bvh else 0)
         if bool(arrays.get("russian_roulette", False)):
@@ -215,6 +240,18 @@ class OracleScene:
     def set_russian_roulette(self, enable: bool) -> None:
         """Russian roulette on the scene's paths (the rule of trace_radiance)."""
         self.lib.orc_scene_set_russian_roulette(self.handle, 1 if enable else 0)
+
+    def set_frame_index(self, frame_index: int) -> None:
+        """Frame index of the blend alpha test (scene.slang:146-148)."""
+        self.lib.orc_scene_set_frame_index(self.handle, int(frame_index) & 0xFFFFFFFF)
+
+    def sample_texture(self, texture: int, uv: np.ndarray) -> np.ndarray:
+        """Bilinear, wrap-addressed samples float32[n, 4] of the scene's
+        texture ``texture`` at ``uv`` float32[n, 2]."""
+        uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), 4), dtype=np.float32)
+        self.lib.orc_sample_texture(self.handle, int(texture), _p(uv, f32p), len(uv), _p(out, f32p))
+        return out
 
     def set_sky_view_lut(self, lut) -> None:
         """Attaches a sky-view table float32[200, 200, 4] (None detaches)."""

@@ -311,10 +311,16 @@ typedef struct {
     const float *transforms;      /* [X][16] row-major object-to-world */
     const float *inv_transposes;  /* [X][16] row-major transpose(inverse(M)) */
     uint32_t transform_count;
-    const uint32_t *materials;    /* [Mat][8] base.rgb f32, flags, emissive.rgb f32, alpha */
+    const uint32_t *materials;    /* [Mat][12] base.rgb f32, flags, emissive.rgb f32, alpha_mode,
+                                     base/emissive/normal texture (0xFFFFFFFF none), alpha_cutoff f32 */
     uint32_t material_count;
     const float *lighting;        /* [16] see Lighting.pack */
+    const uint32_t *textures;     /* [Tx][4] first texel, width, height, flags (bit 0 sRGB) */
+    uint32_t texture_count;
+    const uint32_t *texels;       /* RGBA8, R in the low byte */
+    uint64_t texel_count;
 } orc_scene_desc;
+#define ORC_MATERIAL_WORDS 12u
 
 typedef struct { float lo[3], hi[3]; uint32_t a, b; /* internal: left,right ; leaf: first,count|LEAF */ } orc_bnode;
 #define ORC_LEAF 0x80000000u
@@ -333,6 +339,8 @@ typedef struct {
     const float *sky_lut; /* sky-view table [200][200][4] (caller-owned) or NULL */
     float slack;         /* absolute box-test slack, see box_hit */
     int roulette;        /* Russian roulette on paths (extension), see trace_radiance */
+    uint32_t frame_index; /* stochastic alpha test's frame index (scene.slang:146-148) */
+    float lut[512];      /* texel decode: c / 255, then the sRGB decode of c / 255 */
 } orc_scene;
 
 static void build_world_triangles(orc_scene *s) {
@@ -369,6 +377,158 @@ static void build_world_triangles(orc_scene *s) {
             }
         }
     }
+}
+
+
+/* ------------------------------------------------------------------ */
+/* textures and the alpha test                                          */
+/* ------------------------------------------------------------------ */
+/* Texel decode table: c / 255, then the inverse sRGB transfer function
+ * (IEC 61966-2-1) of c / 255, each in double, rounded once.  The reference
+ * loads base-colour and emissive textures as sRGB, the rest as UNORM
+ * (texture_manager.py:93-94, 120-127); the GPU decodes per texel. */
+static void make_texel_lut(float *lut) {
+    for (int c = 0; c < 256; ++c) {
+        double x = c / 255.0;
+        lut[c] = (float)x;
+        lut[256 + c] = (float)(x <= 0.04045 ? x / 12.92 : pow((x + 0.055) / 1.055, 2.4));
+    }
+}
+
+ORC_API void orc_texel_lut(float *out512) { make_texel_lut(out512); }
+
+/* Wrap addressing on one axis (the reference's default sampler,
+ * scene.py:116): u wrapped into [0, 1], texel centres at (i + 0.5) / n, the
+ * two texels a bilinear footprint straddles and the second one's weight. */
+static void wrap_axis(float u, uint32_t n, uint32_t *i0, uint32_t *i1, float *a) {
+    if (!(fabsf(u) <= FLT_MAX)) u = 0.0f;
+    float f = u - floorf(u);
+    float x = f * (float)n - 0.5f;
+    float fl = floorf(x);
+    *a = x - fl;
+    int k = (int)fl;
+    *i0 = k < 0 ? n - 1u : (uint32_t)k;
+    *i1 = *i0 + 1u == n ? 0u : *i0 + 1u;
+}
+
+typedef struct { float r, g, b, a; } orc_rgba;
+
+static orc_rgba texel_value(const float *lut, const float *clut, uint32_t w) {
+    orc_rgba t = {clut[w & 0xFFu], clut[(w >> 8) & 0xFFu], clut[(w >> 16) & 0xFFu], lut[w >> 24]};
+    return t;
+}
+
+static float lerp2(float ax, float ay, float p00, float p10, float p01, float p11) {
+    float r0 = fmaf(ax, p10 - p00, p00);
+    float r1 = fmaf(ax, p11 - p01, p01);
+    return fmaf(ay, r1 - r0, r0);
+}
+
+/* SampleLevel(linear_sampler, uv, 0) on mip 0 of an RGBA8 texture
+ * (scene.slang:124-131, 313-368): bilinear over the decoded texels, along
+ * x then y, one fmaf per lerp. */
+static orc_rgba sample_texture(const orc_scene *s, uint32_t tex, float u, float v) {
+    const uint32_t *d = s->d.textures + 4 * (size_t)tex;
+    const float *clut = s->lut + ((d[3] & 1u) ? 256 : 0);
+    uint32_t x0, x1, y0, y1;
+    float ax, ay;
+    wrap_axis(u, d[1], &x0, &x1, &ax);
+    wrap_axis(v, d[2], &y0, &y1, &ay);
+    const uint32_t *row0 = s->d.texels + d[0] + (size_t)y0 * d[1];
+    const uint32_t *row1 = s->d.texels + d[0] + (size_t)y1 * d[1];
+    orc_rgba t00 = texel_value(s->lut, clut, row0[x0]), t10 = texel_value(s->lut, clut, row0[x1]);
+    orc_rgba t01 = texel_value(s->lut, clut, row1[x0]), t11 = texel_value(s->lut, clut, row1[x1]);
+    orc_rgba out = {lerp2(ax, ay, t00.r, t10.r, t01.r, t11.r), lerp2(ax, ay, t00.g, t10.g, t01.g, t11.g),
+                    lerp2(ax, ay, t00.b, t10.b, t01.b, t11.b), lerp2(ax, ay, t00.a, t10.a, t01.a, t11.a)};
+    return out;
+}
+
+ORC_API void orc_sample_texture(const orc_scene *s, uint32_t tex, const float *uv, uint64_t n, float *out4) {
+    for (uint64_t i = 0; i < n; ++i) {
+        orc_rgba t = sample_texture(s, tex, uv[2 * i], uv[2 * i + 1]);
+        out4[4 * i] = t.r, out4[4 * i + 1] = t.g, out4[4 * i + 2] = t.b, out4[4 * i + 3] = t.a;
+    }
+}
+
+/* murmur_add + murmur_mix (common.slang:520-545): MurmurHash3's block mix
+ * of k into h and its finaliser. */
+static uint32_t murmur_key_mix(uint32_t h, uint32_t k) {
+    k *= 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    h ^= k;
+    h = (h << 13) | (h >> 19);
+    h = h * 5u + 0xe6546b64u;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+/* int3(pos * scale) of the reference: truncation; out of range saturates and
+ * NaN gives 0 (what the GPU's conversion does) */
+static int32_t to_i32_sat(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+
+/* position_hashing(pos, 10000) (common.slang:547-564) */
+static uint32_t position_hash(f3 p) {
+    uint32_t vx = ((uint32_t)to_i32_sat(p.x * 10000.0f) + 0x3ffffu) & 0x7ffffu;
+    uint32_t vy = ((uint32_t)to_i32_sat(p.y * 10000.0f) + 0x3ffffu) & 0x7ffffu;
+    uint32_t vz = ((uint32_t)to_i32_sat(p.z * 10000.0f) + 0x3ffffu) & 0x7ffffu;
+    uint32_t k0 = ((vx & 0x7fffu) << 7) | (vy & 0x7fu);
+    uint32_t k1 = ((vy & 0xfffu) << 20) | ((vz & 0x7fffu) << 1);
+    return murmur_key_mix(k0, k1);
+}
+
+ORC_API void orc_alpha_hash(const float *pos, uint64_t n, uint32_t frame_index, uint32_t *out) {
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = murmur_key_mix(position_hash(mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2])), frame_index);
+}
+
+static inline float u32_bits_to_f(uint32_t w) { float f; memcpy(&f, &w, 4); return f; }
+
+/* eval_alpha_test on a candidate hit (scene.slang:124-149) as
+ * Scene.intersect / get_visibility apply it (scene.slang:1017-1172):
+ * opaque materials always keep the hit; mask keeps opacity >= cutoff;
+ * blend keeps opacity >= float(murmur_key_mix(position_hashing(world
+ * position, 10000), frame_index)) / 2^32.  Opacity is the base-colour
+ * texture's alpha at the interpolated uv, 1 without a texture.  World
+ * position and uv interpolate as get_vertex does (scene.slang:157-177). */
+static int alpha_accept(const orc_scene *s, uint32_t g, float b1, float b2) {
+    const orc_scene_desc *d = &s->d;
+    const uint32_t *inst = d->instance_descs + 4 * (size_t)s->tri_inst[g];
+    const uint32_t *mat = d->materials + ORC_MATERIAL_WORDS * (size_t)inst[1];
+    if (mat[7] == 0u) return 1;
+    const uint32_t *mesh = d->mesh_descs + 4 * (size_t)inst[0];
+    const uint32_t *idx = d->indices + mesh[3] + 3 * (size_t)s->tri_prim[g];
+    const float *V0 = d->vertices + 8 * (size_t)(mesh[2] + idx[0]);
+    const float *V1 = d->vertices + 8 * (size_t)(mesh[2] + idx[1]);
+    const float *V2 = d->vertices + 8 * (size_t)(mesh[2] + idx[2]);
+    float w0 = 1.0f - b1 - b2;
+    float opacity = 1.0f;
+    if (mat[8] != ORC_INVALID) {
+        float u = w0 * V0[6] + b1 * V1[6] + b2 * V2[6];
+        float v = w0 * V0[7] + b1 * V1[7] + b2 * V2[7];
+        opacity = sample_texture(s, mat[8], u, v).a;
+    }
+    if (mat[7] == 1u) return opacity >= u32_bits_to_f(mat[11]);
+    float p[3], w[3];
+    for (int k = 0; k < 3; ++k) p[k] = w0 * V0[k] + b1 * V1[k] + b2 * V2[k];
+    const float *M = d->transforms + 16 * (size_t)inst[2];
+    for (int r = 0; r < 3; ++r) w[r] = M[4 * r] * p[0] + M[4 * r + 1] * p[1] + M[4 * r + 2] * p[2] + M[4 * r + 3];
+    uint32_t seed = murmur_key_mix(position_hash(mk(w[0], w[1], w[2])), s->frame_index);
+    return opacity >= (float)seed * 0x1p-32f;
+}
+
+static inline int alpha_tested(const orc_scene *s, uint32_t g) {
+    const orc_scene_desc *d = &s->d;
+    return d->materials[ORC_MATERIAL_WORDS * (size_t)d->instance_descs[4 * (size_t)s->tri_inst[g] + 1] + 7] != 0u;
 }
 
 /* --- oracle BVH: binary, object-median split on the longest centroid axis --- */
@@ -469,6 +629,7 @@ ORC_API orc_scene *orc_scene_create(const orc_scene_desc *desc, int use_bvh) {
     orc_scene *s = (orc_scene *)calloc(1, sizeof(orc_scene));
     s->d = *desc;
     s->use_bvh = use_bvh;
+    make_texel_lut(s->lut);
     build_world_triangles(s);
     float m = 1.0f;
     for (size_t i = 0; i < 9 * (size_t)s->tri_count; ++i) m = fmaxf(m, fabsf(s->tri[i]));
@@ -479,6 +640,7 @@ ORC_API orc_scene *orc_scene_create(const orc_scene_desc *desc, int use_bvh) {
 
 ORC_API void orc_scene_set_sky_view_lut(orc_scene *s, const float *sky_lut) { s->sky_lut = sky_lut; }
 ORC_API void orc_scene_set_russian_roulette(orc_scene *s, int enable) { s->roulette = enable != 0; }
+ORC_API void orc_scene_set_frame_index(orc_scene *s, uint32_t frame_index) { s->frame_index = frame_index; }
 
 ORC_API void orc_scene_destroy(orc_scene *s) {
     if (!s) return;
@@ -600,7 +762,9 @@ static orc_hit closest_hit(const orc_scene *s, const orc_ray *r, uint64_t *tri_t
             float t, b1, b2;
             ++tests;
             if (tri_intersect(r, s->tri + 9 * (size_t)g, &t, &b1, &b2)) {
-                if (t < h.t || (t == h.t && g < h.id)) { h.valid = 1; h.t = t; h.b1 = b1; h.b2 = b2; h.id = g; }
+                if ((t < h.t || (t == h.t && g < h.id)) && (!alpha_tested(s, g) || alpha_accept(s, g, b1, b2))) {
+                    h.valid = 1; h.t = t; h.b1 = b1; h.b2 = b2; h.id = g;
+                }
             }
         }
     } else if (s->tri_count) {
@@ -618,7 +782,10 @@ static orc_hit closest_hit(const orc_scene *s, const orc_ray *r, uint64_t *tri_t
                     float t, b1, b2;
                     ++tests;
                     if (tri_intersect(r, s->tri + 9 * (size_t)g, &t, &b1, &b2)) {
-                        if (t < h.t || (t == h.t && g < h.id)) { h.valid = 1; h.t = t; h.b1 = b1; h.b2 = b2; h.id = g; }
+                        if ((t < h.t || (t == h.t && g < h.id)) &&
+                            (!alpha_tested(s, g) || alpha_accept(s, g, b1, b2))) {
+                            h.valid = 1; h.t = t; h.b1 = b1; h.b2 = b2; h.id = g;
+                        }
                     }
                 }
             } else {
@@ -635,7 +802,9 @@ static int any_hit(const orc_scene *s, const orc_ray *r) {
     if (!s->use_bvh) {
         for (uint32_t g = 0; g < s->tri_count; ++g) {
             float t, b1, b2;
-            if (tri_intersect(r, s->tri + 9 * (size_t)g, &t, &b1, &b2)) return 1;
+            if (tri_intersect(r, s->tri + 9 * (size_t)g, &t, &b1, &b2) &&
+                (!alpha_tested(s, g) || alpha_accept(s, g, b1, b2)))
+                return 1;
         }
         return 0;
     }
@@ -650,7 +819,10 @@ static int any_hit(const orc_scene *s, const orc_ray *r) {
             uint32_t cnt = nd->b & ~ORC_LEAF;
             for (uint32_t i = 0; i < cnt; ++i) {
                 float t, b1, b2;
-                if (tri_intersect(r, s->tri + 9 * (size_t)s->order[nd->a + i], &t, &b1, &b2)) return 1;
+                uint32_t g = s->order[nd->a + i];
+                if (tri_intersect(r, s->tri + 9 * (size_t)g, &t, &b1, &b2) &&
+                    (!alpha_tested(s, g) || alpha_accept(s, g, b1, b2)))
+                    return 1;
             }
         } else {
             stack[sp++] = nd->b;
@@ -830,15 +1002,46 @@ static void shading_data(const orc_scene *s, uint32_t instance, uint32_t prim, f
     f3 ni = add(add(muls(n0, bc.x), muls(n1, bc.y)), muls(n2, bc.z));
     f3 geometry_normal = normalize(mul33(IT, ni));
     f3 face_normal = world_normal;
-    const uint32_t *mat = d->materials + 8 * (size_t)inst[1];
+    const uint32_t *mat = d->materials + ORC_MATERIAL_WORDS * (size_t)inst[1];
     float mf[8];
     memcpy(mf, mat, sizeof(mf));
+    f3 diffuse = mk(mf[0], mf[1], mf[2]), emissive = mk(mf[4], mf[5], mf[6]);
+    f3 shading_normal = geometry_normal;
+    if (mat[8] != ORC_INVALID || mat[9] != ORC_INVALID || mat[10] != ORC_INVALID) {
+        /* textures at the interpolated uv (scene.slang:297, 313-368): the
+         * base-colour texture replaces the factor, the emissive texture
+         * scales it, the normal map tilts the interpolated normal in a
+         * tangent frame from object-space dp/duv (before the two-sided flip) */
+        float u = bc.x * V0[6] + bc.y * V1[6] + bc.z * V2[6];
+        float v = bc.x * V0[7] + bc.y * V1[7] + bc.z * V2[7];
+        if (mat[8] != ORC_INVALID) {
+            orc_rgba t = sample_texture(s, mat[8], u, v);
+            diffuse = mk(t.r, t.g, t.b);
+        }
+        if (mat[9] != ORC_INVALID) {
+            orc_rgba t = sample_texture(s, mat[9], u, v);
+            emissive = mk(t.r * mf[4], t.g * mf[5], t.b * mf[6]);
+        }
+        if (mat[10] != ORC_INVALID) {
+            orc_rgba t = sample_texture(s, mat[10], u, v);
+            f3 tn = normalize(mk(t.r * 2.0f - 1.0f, t.g * 2.0f - 1.0f, t.b * 2.0f - 1.0f));
+            float d1u = V1[6] - V0[6], d1v = V1[7] - V0[7], d2u = V2[6] - V0[6], d2v = V2[7] - V0[7];
+            float det = d1u * d2v - d1v * d2u;
+            if (fabsf(det) > 1e-6f) {
+                f3 tg = sub(muls(e1, d2v), muls(e2, d1v));
+                tg = normalize(mk(tg.x / det, tg.y / det, tg.z / det));
+                tg = mul33(IT, tg);
+                tg = normalize(sub(tg, muls(geometry_normal, dot(tg, geometry_normal))));
+                f3 bt = cross(geometry_normal, tg);
+                shading_normal = normalize(add(add(muls(tg, tn.x), muls(bt, tn.y)), muls(geometry_normal, tn.z)));
+            }
+        }
+    }
     int double_sided = (mat[3] & 1u) != 0;
     int front = dot(face_normal, view) > 0.0f;
     float sign = (double_sided && !front) ? -1.0f : 1.0f;
     face_normal = muls(face_normal, sign);
-    geometry_normal = muls(geometry_normal, sign);
-    f3 shading_normal = geometry_normal;
+    shading_normal = muls(shading_normal, sign);
     /* adjust_shading_normal (common.slang:475-490) with D = -view */
     f3 D = neg(view);
     float dn = 2.0f * dot(shading_normal, D);
@@ -851,8 +1054,8 @@ static void shading_data(const orc_scene *s, uint32_t instance, uint32_t prim, f
     sd->face_normal = face_normal;
     sd->view = view;
     sd->frame = make_frame(shading_normal);
-    sd->diffuse = mk(mf[0], mf[1], mf[2]);
-    sd->emissive = mk(mf[4], mf[5], mf[6]);
+    sd->diffuse = diffuse;
+    sd->emissive = emissive;
 }
 
 /* world flat normal (texture_space_path_tracer.slang:39-55) */

@@ -54,7 +54,7 @@ static int g_failures = 0;
 
 struct RandomScene {
     std::vector<float> vertices;  // [V][8]
-    std::vector<uint32_t> indices, mesh_descs, instance_descs, materials;
+    std::vector<uint32_t> indices, mesh_descs, instance_descs, materials, textures, texels;
     std::vector<float> transforms, inv_transposes, lighting;
     uint32_t triangles = 0;
 };
@@ -83,7 +83,8 @@ static RandomScene make_scene(uint32_t seed) {
                 }
                 if (t % 23 == 0) p[1] = cy;  // axis-aligned triangle
                 float n[3] = {0.0f, 1.0f, 0.0f};
-                s.vertices.insert(s.vertices.end(), {p[0], p[1], p[2], n[0], n[1], n[2], 0.0f, 0.0f});
+                float uv[2] = {3.0f * u(gen), 3.0f * u(gen)};  // wraps, negative too
+                s.vertices.insert(s.vertices.end(), {p[0], p[1], p[2], n[0], n[1], n[2], uv[0], uv[1]});
             }
             for (uint32_t k = 0; k < 3; ++k) s.indices.push_back(3 * t + k);
         }
@@ -105,9 +106,16 @@ static RandomScene make_scene(uint32_t seed) {
         s.triangles += s.mesh_descs[4 * mesh + 1] / 3;
     }
     auto fbits = [](float f) { uint32_t b; std::memcpy(&b, &f, 4); return b; };
-    // material 0 single-sided grey, material 1 double-sided and emissive
+    // material 0 single-sided, opaque, base-colour and normal textures; material 1
+    // double-sided, emissive, blend alpha from the base-colour texture
+    const uint32_t none = 0xFFFFFFFFu;
     s.materials = {fbits(0.7f), fbits(0.7f), fbits(0.7f), 0u, fbits(0.0f), fbits(0.0f), fbits(0.0f), 0u,
-                   fbits(0.5f), fbits(0.2f), fbits(0.1f), 1u, fbits(2.0f), fbits(1.0f), fbits(0.5f), 0u};
+                   0u, none, 1u, fbits(0.5f),
+                   fbits(0.5f), fbits(0.2f), fbits(0.1f), 1u, fbits(2.0f), fbits(1.0f), fbits(0.5f), 2u,
+                   0u, none, none, fbits(0.5f)};
+    // texture 0: 5x3 sRGB colour, texture 1: 2x7 normals
+    s.textures = {0u, 5u, 3u, 1u, 15u, 2u, 7u, 0u};
+    for (int k = 0; k < 29; ++k) s.texels.push_back((uint32_t)gen());
     float sun[16] = {0.3f, 0.8f, 0.2f, 0.9999f, 3.0f, 3.0f, 3.0f, 0.0f, 1.0f, 1.0f, 1.0f, 0.0f, 0.2f, 0.3f, 0.4f, 0.0f};
     float len = std::sqrt(sun[0] * sun[0] + sun[1] * sun[1] + sun[2] * sun[2]);
     sun[0] /= len, sun[1] /= len, sun[2] /= len;
@@ -129,8 +137,12 @@ static NanSceneDesc desc_of(const RandomScene &s) {
     d.inv_transposes = s.inv_transposes.data();
     d.transform_count = (uint32_t)(s.transforms.size() / 16);
     d.materials = s.materials.data();
-    d.material_count = (uint32_t)(s.materials.size() / 8);
+    d.material_count = (uint32_t)(s.materials.size() / 12);
     d.lighting = s.lighting.data();
+    d.textures = s.textures.data();
+    d.texture_count = (uint32_t)(s.textures.size() / 4);
+    d.texels = s.texels.data();
+    d.texel_count = s.texels.size();
     return d;
 }
 

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(_native.LIB_PATH))
     for name in sorted(_declared_symbols()):
         assert hasattr(lib, name), name
-    assert _native.lib().nan_version() == 2
+    assert _native.lib().nan_version() == _native.ABI_VERSION == 3
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
