@@ -23,21 +23,25 @@ What is kept from the reference:
   moved vertices (scene_node.py:364-389).
 * Camera framing: target = bounds centre, position = centre +
   (0, 0.3 d, 1.5 d), d = |max - min|.
-* glTF texture coordinates are flipped to ``1 - v`` (scene_node.py:90-95).
+* glTF texture coordinates reach the shader as the file stores them:
+  trimesh's glTF reader turns v into 1 - v (its origin is bottom-left) and
+  scene_node.py:90-95 turns it back for glTF files; OBJ, PLY and STL
+  coordinates pass unchanged.
 * Materials: base colour, emission (values above one kept), roughness,
-  metallic, alpha mode and cutoff, double-sidedness and texture paths
+  metallic, alpha mode and cutoff, double-sidedness and textures
   (scene_node.py:120-330); a geometry without a material takes
-  ``default_material`` when one is given.
+  ``default_material`` when one is given.  Textures are file paths, or
+  the encoded bytes of an image a GLB or data URI embeds, decoded when the
+  scene is flattened (``textures.load_image``).
 
 Deliberate differences, all in what trimesh would have done:
 
 * OBJ ``Kd`` is read at full precision (trimesh keeps it as 8-bit colour)
   and ``Ke`` is honoured (trimesh's OBJ material drops it), so an emitter
   written to OBJ stays an emitter.
-* The texture-space tracer reads constant materials only and treats every
-  surface as opaque (scene.slang:1037-1041 commits every candidate of the
-  reference's opaque scenes); alpha-masked or blended materials are
-  imported as opaque and keep their source mode in ``source_alpha_mode``.
+* A texture whose file is missing stays unset (the reference substitutes
+  a default texture, texture_manager.py:129-140, which for base colour is
+  the factor's look).
 * Missing vertex normals are the angle-weighted mean of the adjacent face
   normals; files without shared vertices (STL) have equal positions merged
   first, as trimesh's processing does.
@@ -165,7 +169,10 @@ def _parse_mtl(path: str) -> dict:
             "name": name,
             "base_color": m.get("Kd", (0.8, 0.8, 0.8)),
             "emissive": m.get("Ke", (0.0, 0.0, 0.0)),
-            "alpha_mode": ALPHA_MODE_BLEND if alpha < 1.0 else ALPHA_MODE_OPAQUE,
+            # trimesh's OBJ materials carry no alpha mode, so the reference
+            # traces them opaque (scene_node.py:233-241 reads alphaMode only)
+            "alpha_mode": ALPHA_MODE_OPAQUE,
+            "dissolve": alpha,
             "base_color_texture": m.get("map_Kd"),
             "normal_texture": m.get("map_Bump") or m.get("bump") or m.get("norm"),
             "emissive_texture": m.get("map_Ke"),
@@ -515,17 +522,30 @@ class _Gltf:
             return np.maximum(arr.astype(np.float64) / _GLTF_NORMALIZE[ct], -1.0)
         return arr.astype(np.int64)
 
-    def texture_path(self, info) -> str | None:
+    def texture_path(self, info):
+        """The image of a texture reference: a file path, or the encoded
+        bytes of an image held in a bufferView or a data URI; None when
+        there is none or its file is missing."""
         if not info or info.get("index") is None:
             return None
         tex = self.doc.get("textures", [])[info["index"]]
         src = tex.get("source")
         if src is None:
             return None
-        uri = self.doc.get("images", [])[src].get("uri")
-        if not uri or uri.startswith("data:"):
+        image = self.doc.get("images", [])[src]
+        if "bufferView" in image:
+            view = self.doc["bufferViews"][image["bufferView"]]
+            buf = self.buffer(view["buffer"])
+            start = view.get("byteOffset", 0)
+            return bytes(buf[start:start + view["byteLength"]])
+        uri = image.get("uri")
+        if not uri:
             return None
-        return os.path.normpath(uri if os.path.isabs(uri) else os.path.join(self.base, uri))
+        if uri.startswith("data:"):
+            return base64.b64decode(uri.split(",", 1)[1])
+        from urllib.parse import unquote
+        path = os.path.normpath(unquote(uri) if os.path.isabs(uri) else os.path.join(self.base, unquote(uri)))
+        return path if os.path.exists(path) else None
 
     def material(self, i: int) -> dict:
         m = self.doc["materials"][i]
@@ -677,39 +697,43 @@ def _make_material(desc: dict | None, colors, default: Material | None) -> Mater
         base = (0.8, 0.8, 0.8)
         if colors is not None and len(colors):
             base = tuple(float(x) for x in np.clip(colors[:, :3].mean(axis=0), 0.0, 1.0))
-        mat = Material(base_color=base)
-        mat.source_alpha_mode = ALPHA_MODE_OPAQUE
-        return mat
+        return Material(base_color=base)
     base = np.asarray(desc.get("base_color", (0.8, 0.8, 0.8)), dtype=np.float64)
+
+    def existing(key):
+        src = desc.get(key)
+        return None if isinstance(src, str) and not os.path.exists(src) else src
+
     mat = Material(
         base_color=tuple(float(x) for x in base),
         emissive=desc.get("emissive", (0.0, 0.0, 0.0)),
         specular_color=desc.get("specular_color", (1.0, 1.0, 1.0)),
         roughness=desc.get("roughness", 0.5),
         metallic=desc.get("metallic", 0.0),
-        alpha_mode=ALPHA_MODE_OPAQUE,
+        alpha_mode=int(desc.get("alpha_mode", ALPHA_MODE_OPAQUE)),
         alpha_cutoff=desc.get("alpha_cutoff", 0.5),
         double_sided=desc.get("double_sided", False),
+        base_color_texture=existing("base_color_texture"),
+        emissive_texture=existing("emissive_texture"),
+        normal_texture=existing("normal_texture"),
+        roughness_texture=existing("roughness_texture"),
+        metallic_texture=existing("metallic_texture"),
+        specular_color_texture=existing("specular_color_texture"),
     )
-    mat.source_alpha_mode = int(desc.get("alpha_mode", ALPHA_MODE_OPAQUE))
     mat.name = desc.get("name")
-    for key in ("base_color_texture", "normal_texture", "roughness_texture", "metallic_texture",
-                "emissive_texture", "specular_color_texture"):
-        setattr(mat, key, desc.get(key))
     return mat
 
 
-def _make_mesh(geo: _Geometry, positions: np.ndarray, flip_uv_y: bool) -> Mesh:
+def _make_mesh(geo: _Geometry, positions: np.ndarray) -> Mesh:
     """float32[V, 8] vertices (position, normal, uv) and uint32 indices
-    (scene_node.py:73-117)."""
+    (scene_node.py:73-117); uvs as the file stores them (see the module
+    docstring on glTF's v)."""
     normals = geo.normals if geo.normals is not None else angle_weighted_normals(positions, geo.faces)
     verts = np.zeros((len(positions), 8), dtype=np.float32)
     verts[:, 0:3] = positions
     verts[:, 3:6] = normals
     if geo.uvs is not None:
         verts[:, 6:8] = geo.uvs
-        if flip_uv_y:
-            verts[:, 7] = np.float32(1.0) - geo.uvs[:, 1]
     return Mesh(verts, geo.faces.astype(np.uint32))
 
 
@@ -737,7 +761,6 @@ def load_asset(cls, path: str, default_material: Material | None = None, scale: 
         loaded = loaders[ext](path)
     except (OSError, KeyError, IndexError, struct.error, json.JSONDecodeError, UnicodeDecodeError) as e:
         raise ValueError(f"Failed to load asset: {e}") from e
-    is_gltf = ext in (".gltf", ".glb")
     node = cls()
     node.asset_path = path
     scale = float(scale)
@@ -752,7 +775,7 @@ def load_asset(cls, path: str, default_material: Material | None = None, scale: 
         if scale != 1.0:
             p = p * scale
         pos = p.astype(np.float32)
-        mesh_id = node.add_mesh(_make_mesh(geo, pos, is_gltf))
+        mesh_id = node.add_mesh(_make_mesh(geo, pos))
         mat_id = node.add_material(_make_material(geo.material, geo.colors, default_material))
         node.add_instance(mesh_id, mat_id, node.add_transform(Transform()))
         _frame_camera(node, p.min(axis=0), p.max(axis=0))
@@ -765,7 +788,7 @@ def load_asset(cls, path: str, default_material: Material | None = None, scale: 
         geo = loaded.geometries[gname]
         if len(geo.positions) == 0 or len(geo.faces) == 0:
             continue
-        mesh_id = node.add_mesh(_make_mesh(geo, geo.positions, is_gltf))
+        mesh_id = node.add_mesh(_make_mesh(geo, geo.positions))
         mat_id = node.add_material(_make_material(geo.material, geo.colors, default_material))
         ids[gname] = (mesh_id, mat_id)
     lo = np.full(3, np.inf)

@@ -202,17 +202,21 @@ class TextureSpacePathTracingRenderer:
                 self.save_rgb9e5_button.enabled = True
             return False
 
-    def _status(self) -> str:
-        common = (f"iterations: {self.texture_iteration}; surface: {self.layout.total_surface_texels:,}; "
-                  f"back: {self.layout.total_payload_count - self.layout.total_surface_texels:,}")
-        if self._frozen_rgb9e5 is not None:
-            size = self.layout.total_payload_count * 4 / (1024 * 1024)
-            return f"RGB9E5 frozen; {common}; cache: {size:.1f} MiB"
-        if self._save_requested:
-            return f"RGB9E5 save requested; {common}"
-        working = self.layout.total_payload_count * MESH_COLORS_PAYLOAD_SIZE / (1024 * 1024)
-        prefix = f"RGB9E5 save failed: {self._save_error}; " if self._save_error else "Iterating; "
-        return f"{prefix}{common}; cache: {working:.1f} MiB"
+    def cache_state(self) -> dict:
+        """Where the cache stands, for a headless caller: iterations done,
+        slot counts, whether it is frozen (and its RGB9E5 size) or still
+        iterating (and its working size), and the last save error."""
+        slots = self.layout.total_payload_count
+        frozen = self._frozen_rgb9e5 is not None
+        return {
+            "iterations": self.texture_iteration,
+            "surface_slots": self.layout.total_surface_texels,
+            "back_slots": slots - self.layout.total_surface_texels,
+            "frozen": frozen,
+            "save_pending": bool(self._save_requested),
+            "save_error": self._save_error,
+            "cache_bytes": slots * (4 if frozen else MESH_COLORS_PAYLOAD_SIZE),
+        }
 
     # -- per frame -----------------------------------------------------------
     def render(self, command_encoder: CommandEncoder, output: torch.Tensor, frame: int, device: Device,

@@ -92,6 +92,8 @@ def liboracle():
     lib.orc_sample_texture.argtypes = [ctypes.c_void_p, ctypes.c_uint32, f32p, ctypes.c_uint64, f32p]
     lib.orc_alpha_hash.argtypes = [f32p, ctypes.c_uint64, ctypes.c_uint32, u32p]
     lib.orc_texel_lut.argtypes = [f32p]
+    lib.orc_murmur_key_mix.restype = ctypes.c_uint32
+    lib.orc_murmur_key_mix.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     lib.orc_sky_view_lookup.argtypes = [f32p, f32p, f32p, ctypes.c_uint64, f32p]
     lib.orc_atmo_scattering.argtypes = [f32p, f32p]
     lib.orc_atmo_sphere_hit.restype = ctypes.c_float

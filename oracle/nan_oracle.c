@@ -486,6 +486,8 @@ static uint32_t position_hash(f3 p) {
     return murmur_key_mix(k0, k1);
 }
 
+ORC_API uint32_t orc_murmur_key_mix(uint32_t h, uint32_t k) { return murmur_key_mix(h, k); }
+
 ORC_API void orc_alpha_hash(const float *pos, uint64_t n, uint32_t frame_index, uint32_t *out) {
     for (uint64_t i = 0; i < n; ++i)
         out[i] = murmur_key_mix(position_hash(mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2])), frame_index);

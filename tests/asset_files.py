@@ -17,9 +17,9 @@ def _col_major(m: np.ndarray) -> list:
     return [float(x) for x in np.asarray(m, dtype=np.float32).T.reshape(-1)]
 
 
-def gltf_document(node, *, mesh_names=None, node_names=None, flip_v=True):
+def gltf_document(node, *, mesh_names=None, node_names=None):
     """(doc, blob): one glTF mesh per SceneNode mesh (positions, normals,
-    uv with v flipped as glTF stores it, uint32 indices), one material per
+    uv as the scene holds it, uint32 indices), one material per
     mesh taken from the first instance that uses it, one node per instance
     carrying the instance's float32 matrix."""
     mesh_mat = {}
@@ -44,8 +44,6 @@ def gltf_document(node, *, mesh_names=None, node_names=None, flip_v=True):
     for mi, mesh in enumerate(node.meshes):
         v = mesh.vertices
         uv = v[:, 6:8].copy()
-        if flip_v:
-            uv[:, 1] = np.float32(1.0) - uv[:, 1]
         prim = {
             "attributes": {
                 "POSITION": add(v[:, 0:3].copy(), 5126, "VEC3", 34962),
@@ -175,3 +173,49 @@ def write_ply(path: str, positions, normals, faces, *, binary: bool) -> None:
                 fh.write((" ".join(repr(float(x)) for x in (*p, *n)) + "\n").encode())
             for f in faces:
                 fh.write((f"{len(f)} " + " ".join(str(int(i)) for i in f) + "\n").encode())
+
+
+def png_bytes(img: np.ndarray) -> bytes:
+    """PNG encoding of an RGBA8 image (PIL)."""
+    import io
+
+    from PIL import Image
+
+    out = io.BytesIO()
+    Image.fromarray(np.ascontiguousarray(img, dtype=np.uint8), "RGBA").save(out, format="PNG")
+    return out.getvalue()
+
+
+def add_textures(doc: dict, blob: bytes, textures: dict, *, where: str = "bufferView", folder: str | None = None):
+    """Attaches images to the glTF materials of ``doc``: ``textures`` maps
+    (material index, slot) -> RGBA8 image, the slot one of
+    "baseColorTexture", "emissiveTexture", "normalTexture".  Images go into
+    the binary buffer (``where="bufferView"``), a data URI, or PNG files in
+    ``folder`` (``where="file"``).  Returns the new blob."""
+    blob = bytearray(blob)
+    doc.setdefault("images", [])
+    doc.setdefault("textures", [])
+    for (mat, slot), img in textures.items():
+        data = png_bytes(img)
+        k = len(doc["images"])
+        if where == "bufferView":
+            while len(blob) % 4:
+                blob.append(0)
+            doc["bufferViews"].append({"buffer": 0, "byteOffset": len(blob), "byteLength": len(data)})
+            blob.extend(data)
+            doc["images"].append({"bufferView": len(doc["bufferViews"]) - 1, "mimeType": "image/png"})
+        elif where == "data":
+            doc["images"].append({"uri": "data:image/png;base64," + base64.b64encode(data).decode()})
+        else:
+            name = f"tex{k}.png"
+            with open(os.path.join(folder, name), "wb") as fh:
+                fh.write(data)
+            doc["images"].append({"uri": name})
+        doc["textures"].append({"source": k})
+        m = doc["materials"][mat]
+        if slot == "baseColorTexture":
+            m.setdefault("pbrMetallicRoughness", {})[slot] = {"index": len(doc["textures"]) - 1}
+        else:
+            m[slot] = {"index": len(doc["textures"]) - 1}
+    doc["buffers"][0]["byteLength"] = len(blob)
+    return bytes(blob)

@@ -54,8 +54,8 @@ def test_demo_round_trip_keeps_meshes_instances_and_layout(tmp_path, form):
     for a, b in zip(node.meshes, demo.meshes):
         np.testing.assert_array_equal(a.vertices[:, 0:6].view(np.uint32), b.vertices[:, 0:6].view(np.uint32))
         np.testing.assert_array_equal(a.indices, b.indices)
-        # glTF stores v flipped; reading flips it back
-        np.testing.assert_allclose(a.vertices[:, 6:8], b.vertices[:, 6:8], atol=1e-7)
+        # uvs reach the scene as the file stores them
+        np.testing.assert_array_equal(a.vertices[:, 6:8].view(np.uint32), b.vertices[:, 6:8].view(np.uint32))
     for (m, k, t), (dm, dk, dt) in zip(node.instances, demo.instances):
         assert m == dm
         np.testing.assert_array_equal(node.transforms[t].matrix.view(np.uint32),
@@ -276,7 +276,7 @@ def test_gltf_hierarchy_strip_sparse_stride_and_materials(tmp_path):
     lamp, leaf = node.materials
     assert lamp.emissive == pytest.approx((8.0, 4.0, 2.0)) and lamp.base_color == pytest.approx((0.2, 0.3, 0.4))
     assert lamp.metallic == 0.0 and not lamp.double_sided
-    assert leaf.double_sided and leaf.source_alpha_mode == ALPHA_MODE_MASK and leaf.alpha_mode == 0
+    assert leaf.double_sided and leaf.alpha_mode == ALPHA_MODE_MASK and leaf.alpha_cutoff == pytest.approx(0.3)
     assert leaf.base_color == (1.0, 1.0, 1.0)
     # the layout of a 2-instance scene: the double-sided leaf gets back slots
     layout = MeshColorsLayout.build(node, texels_per_unit=1.0, min_resolution=2, max_resolution=2)
@@ -351,3 +351,57 @@ def test_add_scene_node_and_load_json(tmp_path):
     with pytest.raises(ValueError, match="objects"):
         (tmp_path / "bad.json").write_text("{}")
         SceneNode.load_json(str(tmp_path / "bad.json"))
+
+
+@pytest.mark.parametrize("where", ["bufferView", "data", "file"])
+def test_gltf_textures_and_alpha_modes_are_imported(tmp_path, where):
+    rng = np.random.default_rng(3)
+    demo = SceneNode.demo()
+    doc, blob = W.gltf_document(demo)
+    base = rng.integers(0, 256, size=(6, 5, 4), dtype=np.uint8)
+    normal = rng.integers(0, 256, size=(3, 4, 4), dtype=np.uint8)
+    emit = rng.integers(0, 256, size=(2, 2, 4), dtype=np.uint8)
+    blob = W.add_textures(doc, blob, {(0, "baseColorTexture"): base, (0, "normalTexture"): normal,
+                                      (1, "emissiveTexture"): emit}, where=where, folder=str(tmp_path))
+    doc["materials"][0]["alphaMode"] = "MASK"
+    doc["materials"][0]["alphaCutoff"] = 0.25
+    doc["materials"][1]["alphaMode"] = "BLEND"
+    path = tmp_path / ("t.glb" if where == "bufferView" else "t.gltf")
+    if where == "bufferView":
+        W.write_glb(str(path), doc, blob)
+    else:
+        W.write_gltf(str(path), doc, blob, embed=True)
+    node = SceneNode.load_asset(str(path))
+    m0 = next(m for m in node.materials if m.name == "mat0")
+    m1 = next(m for m in node.materials if m.name == "mat1")
+    assert m0.alpha_mode == ALPHA_MODE_MASK and m0.alpha_cutoff == pytest.approx(0.25)
+    assert m1.alpha_mode == 2 and m0.textured and m1.textured
+    arrays = flatten_scene(node)
+    tex = arrays["textures"]
+    assert len(tex) == 3
+    # the decoded pixels, R in the low byte, rows top to bottom as stored
+    words = {}
+    for first, w, h, flags in tex:
+        words[(w, h)] = (arrays["texels"][first:first + w * h].reshape(h, w), flags)
+    for img, srgb in ((base, 1), (normal, 0), (emit, 1)):
+        got, flags = words[(img.shape[1], img.shape[0])]
+        want = (img[..., 0].astype(np.uint32) | img[..., 1].astype(np.uint32) << 8 |
+                img[..., 2].astype(np.uint32) << 16 | img[..., 3].astype(np.uint32) << 24)
+        np.testing.assert_array_equal(got, want)
+        assert flags == srgb
+    mats = arrays["materials"]
+    k0 = node.materials.index(m0)
+    assert mats[k0, 7] == ALPHA_MODE_MASK and mats[k0, 8] != 0xFFFFFFFF and mats[k0, 10] != 0xFFFFFFFF
+    assert mats[k0, 9] == 0xFFFFFFFF and mats[k0].view(np.float32)[11] == np.float32(0.25)
+
+
+def test_missing_texture_file_leaves_the_slot_empty(tmp_path):
+    demo = SceneNode.demo()
+    doc, blob = W.gltf_document(demo)
+    doc["images"] = [{"uri": "nowhere.png"}]
+    doc["textures"] = [{"source": 0}]
+    doc["materials"][0].setdefault("pbrMetallicRoughness", {})["baseColorTexture"] = {"index": 0}
+    W.write_gltf(str(tmp_path / "m.gltf"), doc, blob, embed=True)
+    node = SceneNode.load_asset(str(tmp_path / "m.gltf"))
+    assert all(m.base_color_texture is None for m in node.materials)
+    assert flatten_scene(node)["textures"].shape == (0, 4)

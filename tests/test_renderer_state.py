@@ -89,3 +89,17 @@ def test_camera_move_resets():
     assert r.reset_accumulator and not r.reset_texture_space and r.texture_iteration == 5
     r.on_camera_move({"texture_space": True})
     assert r.reset_texture_space and r.texture_iteration == 0 and r._release_frozen_buffer
+
+
+def test_cache_state_follows_save_and_discard():
+    r = _renderer()
+    r.layout = SimpleNamespace(total_payload_count=123, total_surface_texels=100)
+    st = r.cache_state()
+    assert st["iterations"] == 3 and st["surface_slots"] == 100 and st["back_slots"] == 23
+    assert not st["frozen"] and st["save_pending"] and st["cache_bytes"] == 123 * 16
+    r.surface_filter = None
+    r.filter_before_save_check_box = SimpleNamespace(value=False)
+    r.rgb9e5_packer = SimpleNamespace(execute=lambda *a: None)
+    assert r._execute_rgb9e5_save(object(), object(), None, object())
+    st = r.cache_state()
+    assert st["frozen"] and not st["save_pending"] and st["cache_bytes"] == 123 * 4
