@@ -223,6 +223,14 @@ NAN_API int nan_batch_set_frame_index(NanTracer *batch, uint32_t frame_index, vo
  * Wavefront mode only.  No reference counterpart. */
 NAN_API int nan_tracer_device_frame(NanTracer *tracer, int enable, uint32_t iteration, uint32_t reset);
 
+/* Random streams (rng_mode).  NAN_RNG_REFERENCE is the reference's LCG
+ * seeded per (texel, instance, sample) (common.slang:164-210,
+ * texture_space_path_tracer.slang:169-171).  NAN_RNG_PCG gives every
+ * (texel, instance) its own PCG sequence (RXS-M-XS output, 32-bit state, a
+ * per-texel odd increment) started from a hash of the sample index; no
+ * reference counterpart. */
+enum { NAN_RNG_REFERENCE = 0, NAN_RNG_PCG = 1 };
+
 /* One progressive iteration over every texel of every instance
  * (TextureSpacePathTracer.execute, texture_space_path_tracer.py:33-59; kernel
  * texture_space_path_tracer.slang:140-216).  payload: 16-byte slots
@@ -230,15 +238,16 @@ NAN_API int nan_tracer_device_frame(NanTracer *tracer, int enable, uint32_t iter
  * samples_per_texel; in wavefront mode max_bounces up to 127 (checked before
  * the first launch, so a refused call leaves the payload untouched). */
 NAN_API int nan_tst_execute(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,
-                            uint32_t max_bounces, uint32_t reset, void *stream);
+                            uint32_t max_bounces, uint32_t reset, uint32_t rng_mode, void *stream);
 /* Same, adding per-launch work counters (NAN_COUNTER_*) into counters
  * (uint64[NAN_COUNTER_COUNT], device memory, caller zeroes). */
 NAN_API int nan_tst_execute_counted(NanTracer *tracer, void *payload, uint32_t iteration,
                                     uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
-                                    uint64_t *counters, void *stream);
-/* Texel ray generation only (texture_space_path_tracer.slang:142-199): per flat front
- * texel writes uint32[4] = instance, primitive, bits of bary.x, bits of bary.y and
- * uint32 RNG state after the jitter draws (rng_state may be NULL). */
+                                    uint32_t rng_mode, uint64_t *counters, void *stream);
+/* Texel ray generation only (texture_space_path_tracer.slang:142-199) with the
+ * reference's stream: per flat front texel writes uint32[4] = instance, primitive,
+ * bits of bary.x, bits of bary.y and uint32 RNG state after the jitter draws
+ * (rng_state may be NULL). */
 NAN_API int nan_tst_raygen(NanTracer *tracer, uint32_t iteration, uint32_t samples_per_texel, uint32_t *out,
                            uint32_t *rng_state, void *stream);
 
@@ -385,8 +394,11 @@ NAN_API int nan_nrc_encode(NanNrc *nrc, const float *pos, const float *nrm, cons
  * E = sun + NEE(emitters) + max(0, cache(x0)), with no bounce ray; the cache
  * then fits the irradiance field over the surfaces (biased by its fit, no
  * per-texel sampling noise in the indirect term).  counters as in
- * nan_tst_execute_counted (may be NULL).  No reference counterpart. */
-enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2, NAN_LEARNED_TEXEL_QUERY = 4 };
+ * nan_tst_execute_counted (may be NULL).  NAN_LEARNED_RNG_PCG selects the
+ * PCG streams of NAN_RNG_PCG (default: the reference's).  No reference
+ * counterpart. */
+enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2, NAN_LEARNED_TEXEL_QUERY = 4,
+       NAN_LEARNED_RNG_PCG = 8 };
 NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payload, uint32_t iteration,
                                     uint32_t samples_per_texel, uint32_t max_bounces, uint32_t reset,
                                     uint32_t train_samples, uint32_t flags, uint64_t *counters, void *stream);

@@ -82,6 +82,8 @@ STAGES = ("generate", "extend", "shade", "shadow", "finish", "megakernel", "lear
 LEARNED_EMITTER_NEE = 1
 LEARNED_RAMPED_MEAN = 2
 LEARNED_TEXEL_QUERY = 4
+LEARNED_RNG_PCG = 8
+RNG_MODES = {"reference": 0, "pcg": 1}  # NAN_RNG_REFERENCE, NAN_RNG_PCG
 
 TRACER_MEGAKERNEL = 0
 TRACER_WAVEFRONT = 1
@@ -135,8 +137,8 @@ def _declare(lib) -> None:
         "nan_tracer_set_mode": (I, [vp, I]),
         "nan_tracer_profile": (I, [vp, I]),
         "nan_tracer_stage_times": (I, [vp, ctypes.POINTER(ctypes.c_double), u32p]),
-        "nan_tst_execute": (I, [vp, vp, U32, U32, U32, U32, vp]),
-        "nan_tst_execute_counted": (I, [vp, vp, U32, U32, U32, U32, vp, vp]),
+        "nan_tst_execute": (I, [vp, vp, U32, U32, U32, U32, U32, vp]),
+        "nan_tst_execute_counted": (I, [vp, vp, U32, U32, U32, U32, U32, vp, vp]),
         "nan_tst_raygen": (I, [vp, U32, U32, vp, vp, vp]),
         "nan_rgb9e5_pack": (I, [vp, vp, U64, vp]),
         "nan_rgb9e5_unpack": (I, [vp, vp, U64, vp]),

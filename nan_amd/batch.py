@@ -31,13 +31,16 @@ class SceneBatch:
     workload)."""
 
     def __init__(self, device: Device, nodes: list[SceneNode], layout_kwargs: list[dict],
-                 max_bounces: list[int], samples_per_texel: int = 1):
+                 max_bounces: list[int], samples_per_texel: int = 1, rng: str = "reference"):
         if not (len(nodes) == len(layout_kwargs) == len(max_bounces)) or not nodes:
             raise ValueError("one layout and one path length per scene")
         self.device = device
         self.nodes = nodes
         self.max_bounces = [max(1, int(b)) for b in max_bounces]
         self.samples_per_texel = max(1, int(samples_per_texel))
+        if rng not in _native.RNG_MODES:
+            raise ValueError(f"unknown rng {rng!r}")
+        self.rng = rng
         self.layouts = [MeshColorsLayout.build(n, **kw) for n, kw in zip(nodes, layout_kwargs)]
         self.scenes = [Scene(device, n) for n in nodes]
         self.tracers = [TextureSpacePathTracer(device, s, l, samples_per_texel=self.samples_per_texel,
@@ -95,10 +98,11 @@ class SceneBatch:
         mb = max(self.max_bounces)
         if counters is None:
             rc = _native.lib().nan_tst_execute(self.handle, None, int(iteration), self.samples_per_texel, mb,
-                                               1 if reset else 0, stream)
+                                               1 if reset else 0, _native.RNG_MODES[self.rng], stream)
         else:
             rc = _native.lib().nan_tst_execute_counted(self.handle, None, int(iteration), self.samples_per_texel, mb,
-                                                       1 if reset else 0, counters.data_ptr(), stream)
+                                                       1 if reset else 0, _native.RNG_MODES[self.rng],
+                                                       counters.data_ptr(), stream)
         _native.check(rc, "nan_tst_execute (batch)")
 
     def set_frame_index(self, command_encoder: CommandEncoder, frame_index: int) -> None:

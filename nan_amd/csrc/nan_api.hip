@@ -904,8 +904,11 @@ NAN_API int nan_tracer_tables(const NanTracer *tracer, const void **face_table, 
 }
 
 static int tst_execute(NanTracer *t, void *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
-                       uint32_t reset, uint64_t *counters, void *stream) {
+                       uint32_t reset, uint32_t rng_mode, uint64_t *counters, void *stream) {
     if (!t || (!payload && !t->scene->batch)) return fail(nullptr, 0, "nan_tst_execute: null argument");
+    if (rng_mode != NAN_RNG_REFERENCE && rng_mode != NAN_RNG_PCG)
+        return fail(nullptr, 0, "nan_tst_execute: unknown rng_mode %u", rng_mode);
+    t->rng_mode = rng_mode;
     if (t->total_front == 0) return 0;
     if (t->scene->batch && t->mode != NAN_TRACER_WAVEFRONT)
         return fail(nullptr, 0, "nan_tst_execute: scene batches run in wavefront mode only");
@@ -957,22 +960,27 @@ NAN_API int nan_tracer_set_mode(NanTracer *tracer, int mode) {
 }
 
 NAN_API int nan_tst_execute(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,
-                            uint32_t max_bounces, uint32_t reset, void *stream) {
-    return tst_execute(tracer, payload, iteration, samples_per_texel, max_bounces, reset, nullptr, stream);
+                            uint32_t max_bounces, uint32_t reset, uint32_t rng_mode, void *stream) {
+    return tst_execute(tracer, payload, iteration, samples_per_texel, max_bounces, reset, rng_mode, nullptr,
+                       stream);
 }
 
 NAN_API int nan_tst_execute_counted(NanTracer *tracer, void *payload, uint32_t iteration, uint32_t samples_per_texel,
-                                    uint32_t max_bounces, uint32_t reset, uint64_t *counters, void *stream) {
+                                    uint32_t max_bounces, uint32_t reset, uint32_t rng_mode, uint64_t *counters,
+                                    void *stream) {
     if (!counters) return fail(nullptr, 0, "nan_tst_execute_counted: null counters");
-    return tst_execute(tracer, payload, iteration, samples_per_texel, max_bounces, reset, counters, stream);
+    return tst_execute(tracer, payload, iteration, samples_per_texel, max_bounces, reset, rng_mode, counters,
+                       stream);
 }
 
 NAN_API int nan_tst_raygen(NanTracer *tracer, uint32_t iteration, uint32_t spp, uint32_t *out, uint32_t *rng_state,
                            void *stream) {
     if (!tracer || !out) return fail(nullptr, 0, "nan_tst_raygen: null argument");
     if (tracer->total_front == 0) return 0;
+    DevLayout Ly = dev_layout(tracer);
+    Ly.rng_pcg = 0u;  // the reference's stream
     hipLaunchKernelGGL(tst_raygen_kernel, dim3(grid_for(tracer->total_front, kFlatBlock)), dim3(kFlatBlock), 0,
-                       as_stream(stream), dev_layout(tracer), iteration, spp, (uint4 *)out, rng_state);
+                       as_stream(stream), Ly, iteration, spp, (uint4 *)out, rng_state);
     return check_launch("tst_raygen");
 }
 

@@ -105,14 +105,53 @@ __device__ __forceinline__ void sincos_2pi(float u, float *s_out, float *c_out) 
 // ---------------------------------------------------------------------------
 // RNG: 32-bit LCG (common.slang:164-210)
 // ---------------------------------------------------------------------------
+// MurmurHash3's block mix and finaliser over the key (h, k) (Appleby,
+// MurmurHash3, public domain; common.slang:520-545).
+__device__ __forceinline__ uint32_t murmur_key_mix(uint32_t h, uint32_t k) {
+    k *= 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    h ^= k;
+    h = (h << 13) | (h >> 19);
+    h = h * 5u + 0xe6546b64u;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Per-texel random streams.  pcg == 0: the reference's LCG (common.slang:
+// 164-210), seeded from (texel, instance, sample).  pcg == 1: PCG
+// (O'Neill, "PCG: A Family of Simple Fast Space-Efficient Statistically Good
+// Algorithms for Random Number Generation", 2014) with the RXS-M-XS output
+// on a 32-bit LCG state, started at a hash of (texel, instance, sample).
+// Both map a 32-bit draw to [0, 1) as the reference does, (r >> 8) * 2^-24.
+// pcg comes from a kernel argument, so it stays in a scalar register.
 struct Rng {
     uint32_t state;
+    uint32_t pcg;
     __device__ __forceinline__ void seed(uint32_t x, uint32_t y, uint32_t s) {
         state = (x * 100591u) ^ (y * 102191u) ^ (s * 103643u);
+        pcg = 0u;
+    }
+    __device__ __forceinline__ void seed_pcg(uint32_t x, uint32_t y, uint32_t s) {
+        state = murmur_key_mix(murmur_key_mix(x, y), s);
+        pcg = 1u;
     }
     __device__ __forceinline__ float next_1d() {
-        state = 1664525u * state + 1013904223u;
-        return (float)(state >> 8) * 0x1p-24f;
+        uint32_t r;
+        if (pcg == 0u) {
+            state = 1664525u * state + 1013904223u;
+            r = state;
+        } else {
+            uint32_t old = state;
+            state = old * 747796405u + 2891336453u;
+            uint32_t word = ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
+            r = (word >> 22u) ^ word;
+        }
+        return (float)(r >> 8) * 0x1p-24f;
     }
 };
 
@@ -333,23 +372,6 @@ __device__ __forceinline__ float4 sample_texture(const DevScene &S, uint32_t tex
     };
     return make_float4(lerp2(t00.x, t10.x, t01.x, t11.x), lerp2(t00.y, t10.y, t01.y, t11.y),
                        lerp2(t00.z, t10.z, t01.z, t11.z), lerp2(t00.w, t10.w, t01.w, t11.w));
-}
-
-// MurmurHash3's block mix and finaliser over the key (h, k) (Appleby,
-// MurmurHash3, public domain; common.slang:520-545).
-__device__ __forceinline__ uint32_t murmur_key_mix(uint32_t h, uint32_t k) {
-    k *= 0xcc9e2d51u;
-    k = (k << 15) | (k >> 17);
-    k *= 0x1b873593u;
-    h ^= k;
-    h = (h << 13) | (h >> 19);
-    h = h * 5u + 0xe6546b64u;
-    h ^= h >> 16;
-    h *= 0x85ebca6bu;
-    h ^= h >> 13;
-    h *= 0xc2b2ae35u;
-    h ^= h >> 16;
-    return h;
 }
 
 // float -> int32 by truncation, saturating, NaN -> 0 (the GPU's conversion,

@@ -99,6 +99,7 @@ struct NanScene {
 
 struct NanTracer {
     NanScene *scene = nullptr;
+    uint32_t rng_mode = NAN_RNG_REFERENCE;  // random streams of the current call
     uint2 *faces = nullptr;
     uint4 *instances = nullptr;
     uint2 *sides = nullptr;
@@ -168,6 +169,7 @@ struct DevLayout {
     uint64_t total_front;
     const uint32_t *face_scene;       // scene of each face (null: one scene)
     const uint32_t *scene_inst_base;  // first global instance of each scene (null: one scene)
+    uint32_t rng_pcg;                 // 1: per-texel PCG streams, 0: the reference LCG
 };
 
 
@@ -184,6 +186,7 @@ inline DevLayout dev_layout(const NanTracer *t) {
     L.total_front = t->total_front;
     L.face_scene = t->face_scene;
     L.scene_inst_base = t->scene_inst_base;
+    L.rng_pcg = t->rng_mode == NAN_RNG_PCG ? 1u : 0u;
     return L;
 }
 

@@ -638,8 +638,10 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
     if (tracer->scene->stack_depth == 0)
         return fail(nullptr, 0, "nan_tst_execute_learned: BVH too deep for the per-texel kernels");
     if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_execute_learned: max_bounces must be at least 1");
-    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY))
+    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY |
+                            NAN_LEARNED_RNG_PCG))
         return fail(nullptr, 0, "nan_tst_execute_learned: unknown flags");
+    tracer->rng_mode = (flags & NAN_LEARNED_RNG_PCG) ? NAN_RNG_PCG : NAN_RNG_REFERENCE;
     if (tracer->total_front == 0) return 0;
     uint32_t nspp = samples_per_texel > 1u ? samples_per_texel : 1u;
     uint64_t nq = tracer->total_payload * nspp;
@@ -696,8 +698,10 @@ NAN_API int nan_tst_learned_records(NanTracer *tracer, uint32_t iteration, uint3
     if (tracer->scene->roulette)
         return fail(nullptr, 0, "nan_tst_learned_records: Russian roulette is not supported by the learned forms");
     if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_learned_records: max_bounces must be at least 1");
-    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY))
+    if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY |
+                            NAN_LEARNED_RNG_PCG))
         return fail(nullptr, 0, "nan_tst_learned_records: unknown flags");
+    tracer->rng_mode = (flags & NAN_LEARNED_RNG_PCG) ? NAN_RNG_PCG : NAN_RNG_REFERENCE;
     if (n == 0 || tracer->total_front == 0) return 0;
     hipStream_t s = as_stream(stream);
     if (reserve(tracer, n, 0, s)) return -1;

@@ -213,7 +213,8 @@ __device__ __forceinline__ TexelSurface texel_surface(const DevLayout &Ly, uint6
     address_to_ij(ts.local - face.x, face.y, i, j);
     float fr = (float)face.y;
     float base_x = (float)i / fr, base_y = (float)j / fr;
-    rng.seed(ts.global_texel, inst_local, iteration * (spp > 1u ? spp : 1u));
+    if (Ly.rng_pcg) rng.seed_pcg(ts.global_texel, inst_local, iteration * (spp > 1u ? spp : 1u));
+    else rng.seed(ts.global_texel, inst_local, iteration * (spp > 1u ? spp : 1u));
     float bx = base_x, by = base_y;
     for (int attempt = 0; attempt < 32; ++attempt) {
         float ux = rng.next_1d();

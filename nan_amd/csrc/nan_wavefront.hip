@@ -60,7 +60,8 @@ constexpr uint64_t kSideLimit = 8;  // split when at most 1/kSideLimit of front 
 
 struct WfBuffers {
     uint32_t capacity = 0;      // paths (front texels)
-    uint32_t *rng = nullptr;    // [cap]
+    uint32_t *rng = nullptr;    // [cap] random stream state of each path
+    uint32_t rng_pcg = 0;       // 1: PCG streams (NAN_RNG_PCG); set per call
     float4 *T = nullptr;        // [cap] throughput.xyz, |cos| of the first bounce
     float4 *rad = nullptr;      // [cap] path radiance.xyz, pdf of the first bounce
     float4 *sun = nullptr;      // [cap] surface sun irradiance of the current sample
@@ -545,6 +546,7 @@ __global__ __launch_bounds__(kWfBlock, 5) void wf_shade(DevScene S, WfBuffers W,
             } else {
                 Rng rng;
                 rng.state = W.rng[p];
+                rng.pcg = W.rng_pcg;
                 f3 d = mk(rd.x, rd.y, rd.z);
                 Shading sd;
                 shading_data<MAT>(S, id, hv.y, hv.z, -d, sd);
@@ -1178,6 +1180,13 @@ static int run_sweeps(NanTracer *t, WfBuffers &W, float4 *payload, const uint32_
 // only its own per-path entries and payload slots, and each stream has its
 // own queues, spill area and counter slots, so the results are those of the
 // one-stream order.  Stage timing (profiling) uses the one-stream order.
+// The random streams of the current call, on every queue set.
+static void set_rng_mode(NanTracer *t) {
+    uint32_t pcg = t->rng_mode == NAN_RNG_PCG ? 1u : 0u;
+    t->wf->rng_pcg = pcg;
+    if (t->wf->ds) t->wf->ds->rng_pcg = pcg;
+}
+
 int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                       uint32_t reset, unsigned long long *counters, hipStream_t stream) {
     if (t->scene->info.max_stack > (uint32_t)kMaxStack)
@@ -1192,6 +1201,7 @@ int wavefront_execute(NanTracer *t, float4 *payload, uint32_t iteration, uint32_
         return fail(nullptr, 0, "wavefront: max_bounces %u exceeds the %u the queue counters hold", bounces,
                     (kCounterSlots / 2 - 2) / 4);
     WfBuffers &W = *t->wf;
+    set_rng_mode(t);
     bool split = W.ds != nullptr && !t->profile;
     hipLaunchKernelGGL(wf_zero_counters, dim3(64), dim3(1024), 0, stream, W.counters, kCounterWords);
     if (!split) {
@@ -1245,6 +1255,7 @@ static int learned_checks(NanTracer *t) {
         return fail(nullptr, 0, "wavefront: BVH needs a %u-entry stack (max %d)", t->scene->info.max_stack,
                     kMaxStack);
     if (!t->wf) return fail(nullptr, 0, "wavefront: working set missing (a failed nan_tracer_tune?)");
+    set_rng_mode(t);
     return 0;
 }
 

@@ -41,12 +41,13 @@ class TextureSpacePathTracer:
     MODES = {"megakernel": _native.TRACER_MEGAKERNEL, "wavefront": _native.TRACER_WAVEFRONT}
 
     def __init__(self, device: Device, scene: Scene, layout: MeshColorsLayout, *, samples_per_texel: int = 1,
-                 max_bounces: int = 3, mode: str = "wavefront"):
+                 max_bounces: int = 3, mode: str = "wavefront", rng: str = "reference"):
         self.device = device
         self.scene = scene
         self.layout = layout
         self.samples_per_texel = max(1, int(samples_per_texel))
         self.max_bounces = max(1, int(max_bounces))
+        self.set_rng(rng)
         handle = ctypes.c_void_p()
         err = _native.err_buffer()
         rc = _native.lib().nan_tracer_create(
@@ -64,6 +65,15 @@ class TextureSpacePathTracer:
         self.instance_infos_buffer = torch.from_numpy(layout.instance_table.view(np.int32)).to(dev)
         self.side_infos_buffer = torch.from_numpy(layout.side_table.view(np.int32)).to(dev)
         self.adjacency_infos_buffer = torch.from_numpy(layout.adjacency_table.view(np.int32)).to(dev)
+
+    def set_rng(self, rng: str) -> None:
+        """Random streams of the texel paths: ``"reference"`` (the
+        reference's LCG seeded per texel, instance and sample,
+        common.slang:164-210) or ``"pcg"`` (PCG, RXS-M-XS output, started at
+        a hash of texel, instance and sample; no reference counterpart)."""
+        if rng not in _native.RNG_MODES:
+            raise ValueError(f"unknown rng {rng!r}")
+        self.rng = rng
 
     def set_mode(self, mode: str) -> None:
         """``"wavefront"`` (staged kernels, default) or ``"megakernel"`` (one
@@ -92,13 +102,14 @@ class TextureSpacePathTracer:
         it = max(0, int(iteration))
         if counters is None:
             rc = _native.lib().nan_tst_execute(self.handle, output.data_ptr(), it, self.samples_per_texel,
-                                               self.max_bounces, 1 if reset else 0, stream)
+                                               self.max_bounces, 1 if reset else 0, _native.RNG_MODES[self.rng],
+                                               stream)
         else:
             if counters.dtype != torch.int64 or counters.numel() < _native.COUNTER_COUNT or not counters.is_cuda:
                 raise ValueError("counters must be an int64 device tensor of at least 8 entries")
             rc = _native.lib().nan_tst_execute_counted(self.handle, output.data_ptr(), it, self.samples_per_texel,
-                                                       self.max_bounces, 1 if reset else 0, counters.data_ptr(),
-                                                       stream)
+                                                       self.max_bounces, 1 if reset else 0,
+                                                       _native.RNG_MODES[self.rng], counters.data_ptr(), stream)
         _native.check(rc, "nan_tst_execute")
 
     def learned_training_records(self, iteration: int, n: int, *, emitter_nee: bool = True,
@@ -111,6 +122,7 @@ class TextureSpacePathTracer:
         out = {k: torch.empty((n, 3), dtype=torch.float32, device=dev) for k in ("pos", "nrm", "alb", "target")}
         out["valid"] = torch.empty((n,), dtype=torch.int32, device=dev)
         flags = (_native.LEARNED_EMITTER_NEE if emitter_nee else 0) | (_native.LEARNED_TEXEL_QUERY if texel_query else 0)
+        flags |= _native.LEARNED_RNG_PCG if self.rng == "pcg" else 0
         rc = _native.lib().nan_tst_learned_records(
             self.handle, max(0, int(iteration)), int(n), self.max_bounces, flags, out["pos"].data_ptr(),
             out["nrm"].data_ptr(), out["alb"].data_ptr(), out["target"].data_ptr(), out["valid"].data_ptr(),
@@ -144,7 +156,7 @@ class TextureSpacePathTracer:
             self.handle, cache.handle, output.data_ptr(), max(0, int(iteration)), self.samples_per_texel,
             self.max_bounces, 1 if reset else 0, int(train_samples),
             (_native.LEARNED_EMITTER_NEE if emitter_nee else 0) | (_native.LEARNED_RAMPED_MEAN if ramped_mean else 0)
-            | (_native.LEARNED_TEXEL_QUERY if texel_query else 0),
+            | (_native.LEARNED_TEXEL_QUERY if texel_query else 0) | (_native.LEARNED_RNG_PCG if self.rng == "pcg" else 0),
             None if counters is None else counters.data_ptr(), command_encoder.stream_handle)
         _native.check(rc, "nan_tst_execute_learned")
 

@@ -89,6 +89,9 @@ def liboracle():
     lib.orc_scene_set_sky_view_lut.argtypes = [ctypes.c_void_p, f32p]
     lib.orc_scene_set_russian_roulette.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_scene_set_frame_index.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.orc_scene_set_rng_mode.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.orc_pcg_stream.restype = ctypes.c_uint32
+    lib.orc_pcg_stream.argtypes = [ctypes.c_uint32] * 4 + [u32p]
     lib.orc_sample_texture.argtypes = [ctypes.c_void_p, ctypes.c_uint32, f32p, ctypes.c_uint64, f32p]
     lib.orc_alpha_hash.argtypes = [f32p, ctypes.c_uint64, ctypes.c_uint32, u32p]
     lib.orc_texel_lut.argtypes = [f32p]
@@ -196,6 +199,14 @@ def sincos_2pi(u: np.ndarray):
     return s, c
 
 
+def pcg_stream(x: int, y: int, s: int, count: int) -> tuple[np.ndarray, int]:
+    """The first ``count`` raw 32-bit draws of the PCG stream of texel x,
+    instance y and sample s, and the state after them."""
+    out = np.zeros(count, dtype=np.uint32)
+    state = liboracle().orc_pcg_stream(x & 0xFFFFFFFF, y & 0xFFFFFFFF, s & 0xFFFFFFFF, count, _p(out, u32p))
+    return out, int(state)
+
+
 def texel_lut() -> np.ndarray:
     """The oracle's texel decode table float32[512]: c / 255, then the sRGB
     decode of c / 255."""
@@ -242,6 +253,10 @@ class OracleScene:
     def set_russian_roulette(self, enable: bool) -> None:
         """Russian roulette on the scene's paths (the rule of trace_radiance)."""
         self.lib.orc_scene_set_russian_roulette(self.handle, 1 if enable else 0)
+
+    def set_rng(self, rng: str) -> None:
+        """Random streams of the texel paths: "reference" (the LCG) or "pcg"."""
+        self.lib.orc_scene_set_rng_mode(self.handle, {"reference": 0, "pcg": 1}[rng])
 
     def set_frame_index(self, frame_index: int) -> None:
         """Frame index of the blend alpha test (scene.slang:146-148)."""

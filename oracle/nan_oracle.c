@@ -95,20 +95,41 @@ static void sincos_2pi(float u, float *s_out, float *c_out) {
 /* ------------------------------------------------------------------ */
 /* RNG (common.slang:164-210)                                          */
 /* ------------------------------------------------------------------ */
-typedef struct { uint32_t state; } orc_rng;
-static inline void rng_seed(orc_rng *r, uint32_t x, uint32_t y, uint32_t sample_index) {
-    r->state = (x * 100591u) ^ (y * 102191u) ^ (sample_index * 103643u);
+/* pcg == 0: the reference's LCG.  pcg == 1: the production streams, PCG
+ * (O'Neill 2014) with the RXS-M-XS output on a 32-bit LCG state, started at
+ * murmur_key_mix(murmur_key_mix(x, y), sample_index); no reference
+ * counterpart.  Both map a draw r to (r >> 8) * 2^-24. */
+typedef struct { uint32_t state, pcg; } orc_rng;
+static uint32_t murmur_key_mix(uint32_t h, uint32_t k);
+static inline void rng_seed_mode(orc_rng *r, uint32_t x, uint32_t y, uint32_t sample_index, uint32_t pcg) {
+    r->pcg = pcg;
+    if (pcg) r->state = murmur_key_mix(murmur_key_mix(x, y), sample_index);
+    else r->state = (x * 100591u) ^ (y * 102191u) ^ (sample_index * 103643u);
 }
 static inline uint32_t rng_next(orc_rng *r) {
-    r->state = 1664525u * r->state + 1013904223u;
-    return r->state;
+    if (!r->pcg) {
+        r->state = 1664525u * r->state + 1013904223u;
+        return r->state;
+    }
+    uint32_t old = r->state;
+    r->state = old * 747796405u + 2891336453u;
+    uint32_t word = ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
+    return (word >> 22u) ^ word;
 }
 static inline float rng_1d(orc_rng *r) { return (float)(rng_next(r) >> 8) * 0x1p-24f; }
 
 ORC_API uint32_t orc_rng_stream(uint32_t x, uint32_t y, uint32_t s, uint32_t count, float *out) {
     orc_rng r;
-    rng_seed(&r, x, y, s);
+    rng_seed_mode(&r, x, y, s, 0);
     for (uint32_t i = 0; i < count; ++i) out[i] = rng_1d(&r);
+    return r.state;
+}
+
+/* count raw 32-bit draws of the PCG stream of (x, y, s); returns the state after them */
+ORC_API uint32_t orc_pcg_stream(uint32_t x, uint32_t y, uint32_t s, uint32_t count, uint32_t *out) {
+    orc_rng r;
+    rng_seed_mode(&r, x, y, s, 1);
+    for (uint32_t i = 0; i < count; ++i) out[i] = rng_next(&r);
     return r.state;
 }
 
@@ -187,7 +208,7 @@ ORC_API uint32_t orc_nearest_texel(const float *bary, const uint32_t *inst, cons
 typedef struct { uint32_t instance, prim; float b1, b2; } orc_hit_info;
 
 static int texel_raygen(const uint32_t *faces, const uint32_t *instances, uint32_t instance_index,
-                        uint32_t local_texel, uint32_t iteration, uint32_t spp,
+                        uint32_t local_texel, uint32_t iteration, uint32_t spp, uint32_t pcg,
                         orc_hit_info *hit, orc_rng *rng, const uint32_t **face_out) {
     const uint32_t *inst = instances + 4 * (size_t)instance_index;
     const uint32_t *face;
@@ -198,7 +219,7 @@ static int texel_raygen(const uint32_t *faces, const uint32_t *instances, uint32
     orc_address_to_ij(local_texel - face[0], face[1], ij);
     float fr = (float)face[1];
     float base_x = (float)ij[0] / fr, base_y = (float)ij[1] / fr;
-    rng_seed(rng, global_texel, instance_index, iteration * (spp > 1 ? spp : 1));
+    rng_seed_mode(rng, global_texel, instance_index, iteration * (spp > 1 ? spp : 1), pcg);
     float bx = base_x, by = base_y;
     for (int attempt = 0; attempt < 32; ++attempt) {
         float ux = rng_1d(rng);
@@ -227,7 +248,7 @@ ORC_API int orc_texel_raygen(const uint32_t *faces, const uint32_t *instances, u
         for (uint32_t t = 0; t < count; ++t, ++w) {
             orc_hit_info h;
             orc_rng rng;
-            if (!texel_raygen(faces, instances, i, t, iteration, spp, &h, &rng, NULL)) return -1;
+            if (!texel_raygen(faces, instances, i, t, iteration, spp, 0, &h, &rng, NULL)) return -1;
             out[4 * w + 0] = h.instance;
             out[4 * w + 1] = h.prim;
             memcpy(&out[4 * w + 2], &h.b1, 4);
@@ -340,6 +361,7 @@ typedef struct {
     float slack;         /* absolute box-test slack, see box_hit */
     int roulette;        /* Russian roulette on paths (extension), see trace_radiance */
     uint32_t frame_index; /* stochastic alpha test's frame index (scene.slang:146-148) */
+    uint32_t rng_pcg;    /* 1: PCG streams (rng_1d), 0: the reference's LCG */
     float lut[512];      /* texel decode: c / 255, then the sRGB decode of c / 255 */
 } orc_scene;
 
@@ -643,6 +665,7 @@ ORC_API orc_scene *orc_scene_create(const orc_scene_desc *desc, int use_bvh) {
 ORC_API void orc_scene_set_sky_view_lut(orc_scene *s, const float *sky_lut) { s->sky_lut = sky_lut; }
 ORC_API void orc_scene_set_russian_roulette(orc_scene *s, int enable) { s->roulette = enable != 0; }
 ORC_API void orc_scene_set_frame_index(orc_scene *s, uint32_t frame_index) { s->frame_index = frame_index; }
+ORC_API void orc_scene_set_rng_mode(orc_scene *s, uint32_t pcg) { s->rng_pcg = pcg != 0; }
 
 ORC_API void orc_scene_destroy(orc_scene *s) {
     if (!s) return;
@@ -1343,7 +1366,9 @@ static void trace_texel(orc_job *j, uint32_t instance, uint32_t local) {
     orc_hit_info h;
     orc_rng rng;
     const uint32_t *face;
-    if (!texel_raygen(j->faces, j->instances, instance, local, j->iteration, j->spp, &h, &rng, &face)) return;
+    if (!texel_raygen(j->faces, j->instances, instance, local, j->iteration, j->spp, j->s->rng_pcg, &h, &rng,
+                      &face))
+        return;
     uint32_t spp = j->spp > 1 ? j->spp : 1;
     orc_light L = load_light_lut(j->s->d.lighting, j->s->sky_lut);
     f3 n = world_flat_normal(j->s, instance, h.prim);
