@@ -211,6 +211,18 @@ def _plain_frames_for(curve: dict, r: float) -> float:
     return float(np.exp(np.interp(-math.log(r), -np.log(rs), np.log(ks))))
 
 
+def cache_config(scene, layout, minibatches: int):
+    """The radiance cache's grid matched to the layout's texel density
+    (RadianceCacheConfig.for_texel_density over the scene's longest extent)."""
+    import numpy as np
+
+    from nan_amd.radiance_cache import RadianceCacheConfig
+
+    lo, hi = scene.bounds()
+    return RadianceCacheConfig.for_texel_density(float(np.max(hi - lo)), layout.texels_per_unit,
+                                                 minibatches=minibatches)
+
+
 def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, plain_curve_frames: int = 2048,
                         train_samples: int = 65536, mlp_queries: int = 1 << 22, steps: int = 10,
                         warmup: int = 2) -> dict:
@@ -227,7 +239,7 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
     import torch
 
     from nan_amd.mesh_colors import MeshColorsLayout
-    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+    from nan_amd.radiance_cache import RadianceCache
     from nan_amd.scene import Scene
     from nan_amd.scenes import cornell_box
     from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
@@ -263,7 +275,8 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
     torch.cuda.synchronize()
     plain_ms = ev0.elapsed_time(ev1) / 512
 
-    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=4))
+    cfg = cache_config(scene, layout, 4)
+    cache = RadianceCache.for_scene(device, scene, cfg)
     learned = torch.zeros_like(bake)
     out_frames = {}
     elapsed_ms = 0.0
@@ -314,10 +327,13 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
     last = out_frames[str(frames[-1])]
     return {
         "workload": ("Cornell box (36 tris), texels_per_unit 16, res 4-64, 1 spp, 3 bounces; learned cache: "
-                     "16-level hash grid x 2 FP16 features, 2^19 entries/level, normal + albedo, 4x64 ReLU MLP "
+                     f"{cfg.levels}-level hash grid (16 to {cfg.max_resolution:g} cells, matched to the texel "
+                     "density) x 2 FP16 features, 2^19 entries/level, normal + albedo, 4x64 ReLU MLP "
                      f"FP16 MFMA / FP32 accumulate, Adam lr 1e-2, relative L2, {train_samples} training suffixes "
                      "per frame in 4 Adam steps, paths end in the cache after 1 bounce, emitter light sampling "
                      "with MIS, ramped mean"),
+        "cache_levels": cfg.levels,
+        "cache_max_resolution": cfg.max_resolution,
         "bake": f"{bake_frames}-frame plain bake (independent RNG streams)",
         "payload_slots": slots,
         "plain_ms_per_frame": plain_ms,
@@ -366,7 +382,7 @@ def learned_forms_vs_plain(device, scene, tracer, rmse, curve: dict, plain_ms: f
     import numpy as np
     import torch
 
-    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+    from nan_amd.radiance_cache import RadianceCache
 
     enc = device.create_command_encoder()
     stream = torch.cuda.current_stream(device.torch_device)
@@ -374,7 +390,8 @@ def learned_forms_vs_plain(device, scene, tracer, rmse, curve: dict, plain_ms: f
     slots = tracer.layout.total_payload_count
     forms = {}
     for name, (texel_query, n_train, mb) in LEARNED_FORMS.items():
-        cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=mb))
+        cfg = cache_config(scene, tracer.layout, mb)
+        cache = RadianceCache.for_scene(device, scene, cfg)
         learned = torch.zeros((slots, 4), dtype=torch.float32, device=device.torch_device)
         res = {}
         elapsed = 0.0
@@ -409,8 +426,8 @@ def learned_forms_vs_plain(device, scene, tracer, rmse, curve: dict, plain_ms: f
                                    texel_query=texel_query)
         stages = {k: v[0] / 2 for k, v in tracer.stage_times().items()}
         tracer.profile(False)
-        forms[name] = {"form": LEARNED_FORM_NOTES[name], "train_samples": n_train, "minibatches": mb, "frames": res,
-                       "stage_ms_per_frame": stages}
+        forms[name] = {"form": LEARNED_FORM_NOTES[name], "train_samples": n_train, "minibatches": mb,
+                       "cache_levels": cfg.levels, "frames": res, "stage_ms_per_frame": stages}
         if graph:
             forms[name]["graph_replays"] = True
             forms[name]["latency_ms"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}
@@ -523,7 +540,7 @@ def combined_leg(args, device) -> dict:
     from nan_amd.autotune import autotune
     from nan_amd.batch import SceneBatch
     from nan_amd.passes import MeshColorsResolve
-    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+    from nan_amd.radiance_cache import RadianceCache
     from nan_amd.scenes import synthetic_scene
 
     dev = device.torch_device
@@ -603,7 +620,7 @@ def combined_leg(args, device) -> dict:
     # frame (the specified one-bounce form) of the same scene, each advancing
     # its own device frame per replay
     n_train, mb = LEARNED_FORMS["one_bounce"][1:]
-    cache = RadianceCache.for_scene(device, scene, RadianceCacheConfig(minibatches=mb))
+    cache = RadianceCache.for_scene(device, scene, cache_config(scene, tracer.layout, mb))
     learned = torch.zeros_like(bake)
     tracer.reserve_learned(cache, n_train)
     batch.set_device_frame(2_000_000, True)
