@@ -366,6 +366,11 @@ NAN_API int nan_nrc_backward_raw(NanNrc *nrc, const float *pos, const float *nrm
 /* Mean loss and valid record count of the last training step (host floats;
  * waits for the device). */
 NAN_API int nan_nrc_stats(NanNrc *nrc, float *loss_and_count2);
+/* Testing hook: the hash-table gradient of the mean loss as the Adam step
+ * receives it (packed FP16 atomic sums, float[table_params], layout as
+ * nan_nrc_get_params); parameters are not updated. */
+NAN_API int nan_nrc_table_grad(NanNrc *nrc, const float *pos, const float *nrm, const float *alb,
+                               const float *target, uint32_t n, float *table_grad, void *stream);
 /* Measurement hook: encodes n queries, then launches the fused MLP forward
  * kernel alone `repeats` times between two HIP events on `stream` and
  * returns the average milliseconds per launch (waits for the device). */

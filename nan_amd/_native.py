@@ -101,7 +101,7 @@ EXPORTED_SYMBOLS = (
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
     "nan_tst_learned_records", "nan_tst_learned_reserve", "nan_scene_set_russian_roulette", "nan_calibrate_valu",
-    "nan_scene_set_frame_index", "nan_batch_set_frame_index",
+    "nan_scene_set_frame_index", "nan_batch_set_frame_index", "nan_nrc_table_grad",
 )
 
 _LIB = None
@@ -122,6 +122,7 @@ def _declare(lib) -> None:
         "nan_scene_set_sky_view_lut": (I, [vp, vp]),
         "nan_scene_set_russian_roulette": (I, [vp, ctypes.c_uint32]),
         "nan_scene_set_frame_index": (I, [vp, ctypes.c_uint32, vp]),
+        "nan_nrc_table_grad": (I, [vp, vp, vp, vp, vp, U32, vp, vp]),
         "nan_batch_set_frame_index": (I, [vp, ctypes.c_uint32, vp]),
         "nan_atmosphere_transmittance_lut": (I, [vp, vp]),
         "nan_atmosphere_multiscattering_lut": (I, [vp, vp, vp]),

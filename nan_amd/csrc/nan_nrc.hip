@@ -609,6 +609,17 @@ __global__ __launch_bounds__(256) void nrc_sum_partials(const float *partial, ui
     if (threadIdx.x < kReduceWeights && k < (uint32_t)kNrcWeights) out[k] = g * grad_scale(stats);
 }
 
+// The accumulated hash-table gradient as a float pair per entry, on the
+// scale of the mean loss (what the table's Adam step uses), clearing it.
+__global__ void nrc_table_grad_out(__half2 *grad, uint64_t n, const float *stats, float2 *out) {
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    __half2 g = grad[k];
+    grad[k] = __halves2half2(__float2half(0.0f), __float2half(0.0f));
+    float sc = grad_scale(stats) / kTableGradScale;
+    out[k] = make_float2(__low2float(g) * sc, __high2float(g) * sc);
+}
+
 __global__ void nrc_scale_rows(const float *in, uint32_t n, const float *stats, float *out) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (uint64_t)n * kNrcIn) return;
@@ -1047,6 +1058,22 @@ NAN_API int nan_nrc_train(NanNrc *nrc, const float *pos, const float *nrm, const
     if (nrc_train_step(nrc, pos, nrm, alb, target, nullptr, n, s)) return -1;
     if (loss_out) hipLaunchKernelGGL(nrc_add_loss, dim3(1), dim3(64), 0, s, nrc->stats, loss_out);
     return check_launch("nrc train");
+}
+
+NAN_API int nan_nrc_table_grad(NanNrc *nrc, const float *pos, const float *nrm, const float *alb,
+                               const float *target, uint32_t n, float *table_grad, void *stream) {
+    if (!nrc || (n && (!pos || !nrm || !alb || !target || !table_grad)))
+        return fail(nullptr, 0, "nan_nrc_table_grad: null argument");
+    if (n == 0) return 0;
+    hipStream_t s = as_stream(stream);
+    if (nrc_reserve_train(nrc, n, s)) return -1;
+    forward_backward(nrc, pos, nrm, alb, target, nullptr, n, s);
+    hipLaunchKernelGGL(nrc_encode_bwd, dim3(grid_for((uint64_t)n * 8, 256), nrc->cfg.levels), dim3(256), 0, s,
+                       grid_of(nrc), pos, n, nrc->denc, nrc->tg);
+    uint64_t entries = nrc->table_params / 2;
+    hipLaunchKernelGGL(nrc_table_grad_out, dim3(grid_for(entries, 256)), dim3(256), 0, s, nrc->tg, entries,
+                       nrc->stats, (float2 *)table_grad);
+    return check_launch("nrc table grad");
 }
 
 NAN_API int nan_nrc_backward_raw(NanNrc *nrc, const float *pos, const float *nrm, const float *alb,

@@ -223,6 +223,17 @@ class RadianceCache:
                       "nan_nrc_backward_raw")
         return wg, ig
 
+    def table_grad_raw(self, pos, nrm, alb, target, encoder: CommandEncoder | None = None) -> torch.Tensor:
+        """The hash-table gradient of the mean loss as the Adam step receives
+        it (the packed FP16 atomic sums, rescaled), float32[levels, entries,
+        2]; parameters are not updated.  Test hook."""
+        n = self._inputs(pos, nrm, alb)
+        _check_rows(target, n, 3, "target", self.device.torch_device)
+        out = torch.empty((self.table_param_count,), dtype=torch.float32, device=self.device.torch_device)
+        _native.check(_native.lib().nan_nrc_table_grad(self.handle, _ptr(pos), _ptr(nrm), _ptr(alb), _ptr(target),
+                                                       n, _ptr(out), self._stream(encoder)), "nan_nrc_table_grad")
+        return out.view(self.config.levels, -1, 2)
+
     def last_step_stats(self) -> tuple[float, float]:
         """(mean loss, valid records) of the last Adam step; waits for the device."""
         out = (ctypes.c_float * 2)()

@@ -104,3 +104,29 @@ def relative_l2_sum(pred: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
     """Sum over records and RGB of (p - t)^2 / (sg(p)^2 + 0.01)."""
     p = pred[:, :3]
     return ((p - target) ** 2 / (p.detach() ** 2 + 0.01)).sum()
+
+
+def table_gradient(pos, input_grad, bbox_min, bbox_max, levels, entries, base, finest, log2_table):
+    """float64[levels, entries, 2]: each sample's gradient with respect to its
+    level features spread over the 8 cell corners with the trilinear
+    weights of ``encode`` and summed exactly (float64)."""
+    res, dense = level_grid(levels, log2_table, base, finest)
+    u = np.clip((pos.astype(np.float64) - bbox_min) / (bbox_max - bbox_min), 0.0, 1.0)
+    g = input_grad.astype(np.float64)
+    out = np.zeros((levels, entries, 2))
+    for level in range(levels):
+        r = res[level]
+        x = u * r
+        fl = np.minimum(np.floor(x), r - 1)
+        f = x - fl
+        base_i = fl.astype(np.int64)
+        for k in range(8):
+            d = np.array([k & 1, (k >> 1) & 1, (k >> 2) & 1])
+            c = base_i + d
+            if dense[level]:
+                idx = c[:, 0] + (r + 1) * (c[:, 1] + (r + 1) * c[:, 2])
+            else:
+                idx = hash_index(c[:, 0], c[:, 1], c[:, 2], entries)
+            w = np.prod(np.where(d == 1, f, 1.0 - f), axis=1)
+            np.add.at(out[level], idx, w[:, None] * g[:, 2 * level:2 * level + 2])
+    return out
