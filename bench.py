@@ -30,6 +30,9 @@ Beside it, in the same JSON line:
 * ``learned_vs_plain_at_scale``: both learned forms on the 1.21M-triangle
   scene against a 4096-frame bake (the texel-query form is a biased variant
   outside the specification).
+* ``batch``: 64 seeded scenes (3.06M triangles, 21.4M slots, Russian
+  roulette paths of 1-8 bounces) in one set of launches per frame, every
+  frame a replayed hipGraph, the launch shape autotuned first.
 * ``combined``: the 4.0M-triangle scene as a batch of one under hipGraph
   replay (latency percentiles, Mrays/s), a 3840x2160 resolve, the learned
   forms against a 4096-frame bake.
@@ -679,21 +682,18 @@ def combined_leg(args, device) -> dict:
     }
 
 
-def bench_batch(args) -> int:
+def batch_leg(args, device) -> dict:
     """64 seeded scenes (1k-200k triangles, 16k-2M slot budgets, Russian
     roulette paths of 1-8 bounces) traced by one set of wavefront launches
     captured in a hipGraph, after the autotuner picked the traversal launch
-    shape."""
-    import numpy as np
+    shape; ``args.steps`` replayed frames timed with HIP events."""
     import torch
 
     from nan_amd.autotune import autotune
     from nan_amd.batch import SceneBatch, batch_workload
-    from nan_amd.device import Device
 
     t0 = time.perf_counter()
     nodes, kws, bounces = batch_workload(0, 64)
-    device = Device(0)
     batch = SceneBatch(device, nodes, kws, bounces)
     log(f"batch of {len(nodes)} scenes, {sum(batch.triangle_counts)} tris, {batch.total_slots} slots "
         f"({time.perf_counter() - t0:.1f} s)")
@@ -717,30 +717,44 @@ def bench_batch(args) -> int:
     total_ms = start.elapsed_time(end)
     batch.clear_device_frame()
     slots = batch.total_slots
-    out = {
-        "metric": METRIC,
+    return {
+        "workload": "64-scene batch (seed 0): 1k-200k triangles and 16k-2M slot budgets log-uniform, "
+                    "Russian roulette paths of 1-8 bounces, one hipGraph per frame",
         "value": slots * args.steps / (total_ms * 1e-3),
         "unit": "texel-path samples/s",
+        "ms_per_step": total_ms / args.steps,
+        "steps": args.steps,
+        "scenes": len(nodes),
+        "russian_roulette": True,
+        "triangles": int(sum(batch.triangle_counts)),
+        "payload_slots": int(slots),
+        "bounces": bounces,
+        "tuning": tuned["best"],
+        "autotune": tuned["results"],
+    }
+
+
+def bench_batch(args) -> int:
+    """The 64-scene batch alone (batch_leg)."""
+    from nan_amd.device import Device
+
+    res = batch_leg(args, Device(0))
+    out = {
+        "metric": METRIC,
+        "value": res["value"],
+        "unit": res["unit"],
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": total_ms / args.steps,
+        "ms_per_step": res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": "64-scene batch (seed 0): 1k-200k triangles and 16k-2M slot budgets log-uniform, "
-                        "Russian roulette paths of 1-8 bounces, one hipGraph per frame",
-            "scenes": len(nodes),
-            "russian_roulette": True,
-            "triangles": int(sum(batch.triangle_counts)),
-            "payload_slots": int(slots),
-            "bounces": bounces,
-            "tuning": tuned["best"],
-        },
-        "autotune": tuned["results"],
+        "config": {k: res[k] for k in ("workload", "scenes", "russian_roulette", "triangles", "payload_slots",
+                                       "bounces", "tuning")},
+        "autotune": res["autotune"],
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -758,6 +772,7 @@ def main() -> int:
     ap.add_argument("--scale-bake-frames", type=int, default=4096,
                     help="plain bake length of the learned comparison on the workload's own scene")
     ap.add_argument("--no-combined", action="store_true", help="skip the combined full-size leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -1001,6 +1016,11 @@ def bench_default(args) -> int:
             t0 = time.perf_counter()
             at_scale = learned_at_scale(device, scene, layout, tracer, bake_frames=args.scale_bake_frames)
             log(f"learned at scale {time.perf_counter() - t0:.1f} s")
+    batch = None
+    if not args.no_batch and args.workload == "synthetic":
+        t0 = time.perf_counter()
+        batch = batch_leg(args, device)
+        log(f"batch leg {time.perf_counter() - t0:.1f} s: {batch['ms_per_step']:.2f} ms/frame")
     if not args.no_combined and args.workload == "synthetic":
         t0 = time.perf_counter()
         combined = combined_leg(args, device)
@@ -1047,6 +1067,8 @@ def bench_default(args) -> int:
     out["roofline"] = plain["roofline"]
     if at_scale is not None:
         out["learned_vs_plain_at_scale"] = at_scale
+    if batch is not None:
+        out["batch"] = batch
     if combined is not None:
         out["combined"] = combined
     out["cpu_baseline"] = None
