@@ -175,7 +175,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;  // not a node or leaf entry (leaves hol
 
 struct TravState {
     uint32_t cur;   // next entry to visit, kNone once the stack is exhausted (then sp == 0)
-    int sp;
+    uint32_t sp;    // unsigned, so the ring slot sp % RING of a power-of-two ring is one AND
     uint32_t pend;  // leaf parked for the next leaf phase, or kNone
 };
 
