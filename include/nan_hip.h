@@ -353,11 +353,19 @@ NAN_API int nan_nrc_infer_noalloc(NanNrc *nrc, const float *pos, const float *nr
  * last step's mean loss to *loss_out (device float, may be NULL). */
 NAN_API int nan_nrc_train(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, const float *target,
                           uint32_t n, float *loss_out, void *stream);
+/* Which kernels a training step runs: when every level of the cache is
+ * densely indexed ((res + 1)^3 entries fit the table), fused = 1 (the
+ * default) takes the two-launch step (forward, loss and backward in one
+ * launch; table-gradient sums per z-slab in LDS and both Adam updates in the
+ * other), fused = 0 the seven-launch step with packed FP16 table-gradient
+ * atomics.  Hashed caches always take the seven-launch step. */
+NAN_API int nan_nrc_set_fused_training(NanNrc *nrc, int fused);
 /* Testing hooks: the raw first four network outputs float[n][4], and the
  * gradients of the mean loss for targets float[n][3] with respect to the MLP
  * weights (float[mlp_weights], layout as nan_nrc_get_params) and to the
  * network input float[n][48] (hash features, normal, albedo, padding);
- * parameters are not updated. */
+ * parameters are not updated.  The backward runs the kernels a training step
+ * of this cache runs (see nan_nrc_set_fused_training). */
 NAN_API int nan_nrc_forward_raw(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, float *out,
                                 uint32_t n, void *stream);
 NAN_API int nan_nrc_backward_raw(NanNrc *nrc, const float *pos, const float *nrm, const float *alb,

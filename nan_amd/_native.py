@@ -101,7 +101,7 @@ EXPORTED_SYMBOLS = (
     "nan_scene_set_sky_view_lut", "nan_atmosphere_transmittance_lut", "nan_atmosphere_multiscattering_lut",
     "nan_atmosphere_sky_view_lut", "nan_atmosphere_view_transmittance", "nan_sky_view_lookup",
     "nan_tst_learned_records", "nan_tst_learned_reserve", "nan_scene_set_russian_roulette", "nan_calibrate_valu",
-    "nan_scene_set_frame_index", "nan_batch_set_frame_index", "nan_nrc_table_grad",
+    "nan_scene_set_frame_index", "nan_batch_set_frame_index", "nan_nrc_table_grad", "nan_nrc_set_fused_training",
 )
 
 _LIB = None
@@ -165,6 +165,7 @@ def _declare(lib) -> None:
         "nan_nrc_backward_raw": (I, [vp, vp, vp, vp, vp, U32, vp, vp, vp]),
         "nan_nrc_encode": (I, [vp, vp, vp, vp, vp, U32, vp]),
         "nan_nrc_stats": (I, [vp, f32p]),
+        "nan_nrc_set_fused_training": (I, [vp, I]),
         "nan_nrc_time_forward": (I, [vp, vp, vp, vp, U32, U32, f32p, vp]),
         "nan_tst_execute_learned": (I, [vp, vp, vp, U32, U32, U32, U32, U32, U32, vp, vp]),
         "nan_tst_learned_reserve": (I, [vp, vp, U32, U32]),

@@ -38,7 +38,7 @@ def build(verbose: bool = False) -> Path:
     bvh_obj = build_dir / "nan_bvh.o"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden", "-c",
           str(CSRC / "nan_bvh.cpp"), "-o", str(bvh_obj)])
-    sources = ("nan_api.hip", "nan_wavefront.hip", "nan_nrc.hip", "nan_nrc_noalloc.hip", "nan_learned.hip",
+    sources = ("nan_api.hip", "nan_wavefront.hip", "nan_nrc.hip", "nan_nrc_fused.hip", "nan_nrc_noalloc.hip", "nan_learned.hip",
                "nan_filter.hip", "nan_atmosphere.hip")
     objs = [str(build_dir / (Path(src).stem + ".o")) for src in sources]
     cmds = [[HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", obj]

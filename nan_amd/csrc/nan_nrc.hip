@@ -40,17 +40,15 @@
 
 using namespace nanx;
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef float floatx4_t __attribute__((ext_vector_type(4)));
-
 namespace {
+
+constexpr uint64_t kFusedStepRecords = 8192;  // largest minibatch the fused step takes
 
 constexpr int kWtPad = 8;           // LDS row padding (halves) of the transposed weights
 constexpr int kTPad = 8;            // LDS row padding (halves) of the backward transposes
 constexpr int kFwdBlock = 256;      // 4 waves
 constexpr int kEncGroup = 64 * 48;  // halves of encoded input per group
 constexpr int kActGroup = 64 * 64;  // halves of a 64-wide activation per group
-constexpr float kGradClamp = 16384.0f;
 
 __host__ __device__ constexpr int wt_lds_offset(int l) {
     int o = 0;
@@ -58,10 +56,6 @@ __host__ __device__ constexpr int wt_lds_offset(int l) {
     return o;
 }
 constexpr int kWtLds = wt_lds_offset(kNrcLayers);
-
-__device__ __forceinline__ floatx4_t mfma16(half4_t a, half4_t b, floatx4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
 
 __device__ __forceinline__ size_t frag_index(uint32_t s, uint32_t unit, uint32_t kb_count) {
     uint32_t g = s >> 6, t = (s >> 4) & 3u, r = s & 15u;
@@ -78,31 +72,6 @@ struct Corners {
     float w[8];
 };
 
-// The cell of a position in one level: lower corner and fractions.
-__device__ __forceinline__ void level_cell(const NrcGrid &E, uint32_t level, const float *pos, uint32_t (&base)[3],
-                                           float (&f)[3]) {
-    float fr = (float)E.res[level];
-    for (int a = 0; a < 3; ++a) {
-        float u = fminf(fmaxf((pos[a] - E.bmin[a]) * E.bscale[a], 0.0f), 1.0f);
-        float x = u * fr;
-        float fl = fminf(floorf(x), fr - 1.0f);
-        base[a] = (uint32_t)fl;
-        f[a] = x - fl;
-    }
-}
-
-// Corner k (bit 0: +x, bit 1: +y, bit 2: +z) of a cell: table entry (dense
-// indexing where (res+1)^3 fits, spatial hash otherwise) and trilinear weight.
-__device__ __forceinline__ void cell_corner(const NrcGrid &E, uint32_t level, const uint32_t (&base)[3],
-                                            const float (&f)[3], uint32_t k, uint32_t &idx, float &w) {
-    uint32_t r1 = E.res[level] + 1u;
-    uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
-    uint32_t x = base[0] + dx, y = base[1] + dy, z = base[2] + dz;
-    idx = E.dense[level] ? x + r1 * (y + r1 * z)
-                         : ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & (E.entries - 1u);
-    w = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
-}
-
 __device__ __forceinline__ Corners level_corners(const NrcGrid &E, uint32_t level, const float *pos) {
     Corners c;
     uint32_t base[3];
@@ -111,8 +80,6 @@ __device__ __forceinline__ Corners level_corners(const NrcGrid &E, uint32_t leve
     for (uint32_t k = 0; k < 8; ++k) cell_corner(E, level, base, f, k, c.idx[k], c.w[k]);
     return c;
 }
-
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 // One thread per (sample, pair of levels) over np (a multiple of 64) samples,
 // level-major (blockIdx.y = the pair, so a wave gathers 64 neighbouring
@@ -176,10 +143,6 @@ __global__ void nrc_encode_fwd(NrcGrid E, const float *pos, const float *nrm, co
     }
 }
 
-// Table gradients accumulate as packed FP16 pairs (one global_atomic_pk_add_f16
-// per corner carries both features), pre-scaled by kTableGradScale to keep
-// sums of many contributions inside FP16 range.
-constexpr float kTableGradScale = 1.0f / 16.0f;
 
 // One thread per (sample, corner) of one level (blockIdx.y = level): eight
 // consecutive lanes take the eight corners of a sample, one packed atomic
@@ -205,17 +168,6 @@ __global__ void nrc_encode_bwd(NrcGrid E, const float *pos, uint32_t n, const fl
 // ---------------------------------------------------------------------------
 // fused MLP forward
 // ---------------------------------------------------------------------------
-typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-
-// One K=32 MFMA over two K=16 fragments: hardware K index 8q+i of lane
-// quarter q stands for fragment 0's 4q+i (i < 4) and fragment 1's 4q+i-4,
-// the same map in both operands, so the product sums the same 32 terms.
-__device__ __forceinline__ floatx4_t mfma32(half4_t a0, half4_t a1, half4_t b0, half4_t b1, floatx4_t c) {
-    half8_t a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
-    half8_t b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
 // acc[t][ot] = W^T block (ot, .) x X^T tile t, over IN/16 input blocks taken
 // in pairs on the K=32 MFMA (an odd last block on K=16).
 template <int IN, int OUT>
@@ -484,28 +436,6 @@ __global__ __launch_bounds__(64) void nrc_mlp_bwd(const _Float16 *wn, const _Flo
 // ---------------------------------------------------------------------------
 // optimiser
 // ---------------------------------------------------------------------------
-struct AdamArgs {
-    float lr, b1, b2, eps;
-};
-
-// Adam's step count lives on the device, so a captured training step that is
-// replayed takes the next step's bias corrections 1 - beta^step each time.
-// nrc_adam_mlp reads `step` and publishes step + 1 and its corrections;
-// nrc_adam_table reads those and stores `step` (neither kernel writes a field
-// that it reads).
-struct AdamState {
-    uint32_t step, next_step;
-    float c1, c2;
-};
-
-__device__ __forceinline__ float bias_correction(float beta, uint32_t step) {
-    return 1.0f - (float)pow((double)beta, (double)step);
-}
-
-__device__ __forceinline__ float grad_scale(const float *stats) {
-    return 1.0f / (3.0f * fmaxf(stats[1], 1.0f));
-}
-
 // Fixed-order sum of the per-wave partials for weights [k0, k0 + 32):
 // thread (k, q) sums partials q, q + 8, ...; the 8 sums combine in order.
 constexpr int kReduceWeights = 32, kReduceSplit = 8;
@@ -646,7 +576,10 @@ __global__ void nrc_add_loss(const float *stats, float *loss_out) {
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
-NrcGrid grid_of(const NanNrc *c) {
+}  // namespace
+
+namespace nanx {
+NrcGrid nrc_grid(const NanNrc *c) {
     NrcGrid E;
     E.table = c->t16;
     E.levels = c->cfg.levels;
@@ -661,6 +594,11 @@ NrcGrid grid_of(const NanNrc *c) {
     }
     return E;
 }
+}  // namespace nanx
+
+namespace {
+
+NrcGrid grid_of(const NanNrc *c) { return nrc_grid(c); }
 
 uint32_t groups_for(uint32_t n) { return (n + 63u) / 64u; }
 
@@ -702,7 +640,7 @@ void forward_backward(NanNrc *c, const float *pos, const float *nrm, const float
 NanNrc::~NanNrc() {
     for (void *p : {(void *)w32, (void *)wm, (void *)wv, (void *)wpart, (void *)w16t, (void *)w16n, (void *)ts,
                     (void *)tg, (void *)t16, (void *)enc, (void *)save, (void *)dy,
-                    (void *)denc, (void *)pred, (void *)stats, adam})
+                    (void *)denc, (void *)pred, (void *)stats, (void *)lossp, adam})
         hipFree(p);
 }
 
@@ -781,17 +719,19 @@ int nrc_train_step(NanNrc *c, const float *pos, const float *nrm, const float *a
                    const uint32_t *valid, uint32_t n, hipStream_t s) {
     if (n == 0) return 0;
     uint32_t parts = std::max<uint32_t>(1, std::min<uint32_t>(c->cfg.minibatches, n));
-    if (parts > 1) {
-        // consecutive slices of the records, one Adam step each; counts are the slice sizes
-        for (uint32_t k = 0; k < parts; ++k) {
-            uint64_t a = (uint64_t)n * k / parts, b = (uint64_t)n * (k + 1) / parts;
-            if (nrc_train_one(c, pos + 3 * a, nrm + 3 * a, alb + 3 * a, target + 3 * a, valid ? valid + a : nullptr,
-                              (uint32_t)(b - a), s))
-                return -1;
-        }
-        return 0;
+    // consecutive slices of the records, one Adam step each; counts are the slice sizes
+    for (uint32_t k = 0; k < parts; ++k) {
+        uint64_t a = (uint64_t)n * k / parts, b = (uint64_t)n * (k + 1) / parts;
+        const uint32_t *v = valid ? valid + a : nullptr;
+        // the fused step is latency bound (one wave per 64 records): faster than the
+        // seven-launch step up to about 8K records per step (measured 38 vs 52 us
+        // at 4,096 and 52 vs 60 us at 8,192 records), slower at 16K (78 vs 73 us)
+        const bool fused = c->fused_train && b - a <= kFusedStepRecords && nrc_fused_eligible(c, (uint32_t)(b - a));
+        if (fused ? nrc_train_fused(c, pos + 3 * a, nrm + 3 * a, alb + 3 * a, target + 3 * a, v, (uint32_t)(b - a), s)
+                  : nrc_train_one(c, pos + 3 * a, nrm + 3 * a, alb + 3 * a, target + 3 * a, v, (uint32_t)(b - a), s))
+            return -1;
     }
-    return nrc_train_one(c, pos, nrm, alb, target, valid, n, s);
+    return 0;
 }
 
 int nrc_train_one(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
@@ -933,6 +873,7 @@ NAN_API int nan_nrc_create(const NanNrcConfig *cfg, NanNrc **out, char *err, siz
               upload(&c->t16, t16.data(), t16.size() * 2, err, cap) == 0 &&
               upload(&c->tg, nullptr, t.size() * 2, err, cap) == 0 &&
               upload(&c->stats, stats.data(), stats.size() * 4, err, cap) == 0 &&
+              upload(&c->lossp, nullptr, (size_t)kNrcBwdWaves * sizeof(float2), err, cap) == 0 &&
               upload(&c->adam, zw.data(), 16, err, cap) == 0 &&
               hipMemset(c->tg, 0, t.size() * 2) == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
               nrc_reserve_train(c, 65536, nullptr) == 0;
@@ -999,6 +940,12 @@ NAN_API int nan_nrc_forward_raw(NanNrc *nrc, const float *pos, const float *nrm,
     if (nrc_predict(nrc, pos, nrm, alb, n, nrc->pred, s)) return -1;
     hipLaunchKernelGGL(nrc_rgb, dim3(grid_for(n, 256)), dim3(256), 0, s, nrc->pred, n, out, 1);
     return check_launch("nrc forward");
+}
+
+NAN_API int nan_nrc_set_fused_training(NanNrc *nrc, int fused) {
+    if (!nrc) return fail(nullptr, 0, "nan_nrc_set_fused_training: null cache");
+    nrc->fused_train = fused != 0;
+    return 0;
 }
 
 NAN_API int nan_nrc_stats(NanNrc *nrc, float *out) {
@@ -1083,6 +1030,8 @@ NAN_API int nan_nrc_backward_raw(NanNrc *nrc, const float *pos, const float *nrm
         return fail(nullptr, 0, "nan_nrc_backward_raw: null argument");
     if (n == 0) return 0;
     hipStream_t s = as_stream(stream);
+    if (nrc->fused_train && nrc_fused_eligible(nrc, n))
+        return nrc_fused_grads(nrc, pos, nrm, alb, target, n, weight_grad, input_grad, s);
     if (nrc_reserve_train(nrc, n, s)) return -1;
     forward_backward(nrc, pos, nrm, alb, target, nullptr, n, s);
     hipLaunchKernelGGL(nrc_sum_partials, dim3(grid_for(kNrcWeights, kReduceWeights)), dim3(256), 0, s, nrc->wpart,

@@ -35,6 +35,86 @@ struct NrcGrid {
     float bmin[3], bscale[3];  // unit-cube coordinate = (p - bmin) * bscale
 };
 
+
+// ---------------------------------------------------------------------------
+// device helpers shared by the network kernels
+// ---------------------------------------------------------------------------
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+// One K=32 MFMA over two K=16 fragments: hardware K index 8q+i of lane
+// quarter q stands for fragment 0's 4q+i (i < 4) and fragment 1's 4q+i-4,
+// the same map in both operands, so the product sums the same 32 terms.
+__device__ __forceinline__ floatx4_t mfma32(half4_t a0, half4_t a1, half4_t b0, half4_t b1, floatx4_t c) {
+    half8_t a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    half8_t b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ floatx4_t mfma16(half4_t a, half4_t b, floatx4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// Per-record gradient bound of the relative L2 loss (before the 1/3n scale).
+constexpr float kGradClamp = 16384.0f;
+
+// Table gradients accumulate as packed FP16 pairs (one global_atomic_pk_add_f16
+// per corner carries both features), pre-scaled by kTableGradScale to keep
+// sums of many contributions inside FP16 range.
+constexpr float kTableGradScale = 1.0f / 16.0f;
+
+// The cell of a position in one level: lower corner and fractions.
+__device__ __forceinline__ void level_cell(const NrcGrid &E, uint32_t level, const float *pos, uint32_t (&base)[3],
+                                           float (&f)[3]) {
+    float fr = (float)E.res[level];
+    for (int a = 0; a < 3; ++a) {
+        float u = fminf(fmaxf((pos[a] - E.bmin[a]) * E.bscale[a], 0.0f), 1.0f);
+        float x = u * fr;
+        float fl = fminf(floorf(x), fr - 1.0f);
+        base[a] = (uint32_t)fl;
+        f[a] = x - fl;
+    }
+}
+
+// Corner k (bit 0: +x, bit 1: +y, bit 2: +z) of a cell: table entry (dense
+// indexing where (res+1)^3 fits, spatial hash otherwise) and trilinear weight.
+__device__ __forceinline__ void cell_corner(const NrcGrid &E, uint32_t level, const uint32_t (&base)[3],
+                                            const float (&f)[3], uint32_t k, uint32_t &idx, float &w) {
+    uint32_t r1 = E.res[level] + 1u;
+    uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    uint32_t x = base[0] + dx, y = base[1] + dy, z = base[2] + dz;
+    idx = E.dense[level] ? x + r1 * (y + r1 * z)
+                         : ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & (E.entries - 1u);
+    w = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
+}
+
+struct AdamArgs {
+    float lr, b1, b2, eps;
+};
+
+// Adam's step count lives on the device, so a captured training step that is
+// replayed takes the next step's bias corrections 1 - beta^step each time.
+// The first kernel of a step reads `step` and publishes step + 1 and its
+// corrections; a later kernel of the step reads those and stores `step` (no
+// kernel writes a field that it reads).
+struct AdamState {
+    uint32_t step, next_step;
+    float c1, c2;
+};
+
+// Adam's bias correction 1 - beta^step as -expm1(step * ln beta): FP32, no
+// cancellation for beta^step near 1, and cheap enough for one thread of a
+// training kernel (beta = 0 gives 1 for every step >= 1).
+__device__ __forceinline__ float bias_correction(float beta, uint32_t step) {
+    return -expm1f((float)step * logf(beta));
+}
+
+__device__ __forceinline__ float grad_scale(const float *stats) {
+    return 1.0f / (3.0f * fmaxf(stats[1], 1.0f));
+}
+
 }  // namespace nanx
 
 // Optimiser state of one hash-table entry (both features): FP32 master
@@ -67,11 +147,16 @@ struct NanNrc {
     float *denc = nullptr;     // [groups * 64][48]
     float4 *pred = nullptr;    // [groups * 64]
     float *stats = nullptr;    // [0] loss sum, [1] normalising count, [2] valid records (floats), [3] spare
+    float2 *lossp = nullptr;   // [kNrcBwdWaves] per-block loss sums of the fused step
+    bool fused_train = true;   // two-launch step when every level is densely indexed
     void *adam = nullptr;      // Adam step count and bias corrections (16 bytes, device)
     ~NanNrc();
 };
 
 namespace nanx {
+
+// The encoding's view of a cache (table, level resolutions, box).
+NrcGrid nrc_grid(const NanNrc *c);
 
 // Grow the prediction / training scratch to hold n samples (refused while
 // `stream` is being captured into a graph: growing allocates).
@@ -91,5 +176,15 @@ int nrc_infer_into(NanNrc *c, const float *pos, const float *nrm, const float *a
 // slice size.
 int nrc_train_step(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                    const uint32_t *valid, uint32_t n, hipStream_t stream);
+// One Adam step in two launches (nan_nrc_fused.hip), for caches whose levels
+// are all densely indexed and steps of at most kNrcBwdWaves * 64 records; the
+// same loss and update as the seven-launch step.
+bool nrc_fused_eligible(const NanNrc *c, uint32_t n);
+int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
+                    const uint32_t *valid, uint32_t n, hipStream_t stream);
+// The fused forward and backward alone: gradients of the mean loss with
+// respect to the weights and to the 48 inputs (test hook).
+int nrc_fused_grads(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target, uint32_t n,
+                    float *weight_grad, float *input_grad, hipStream_t stream);
 
 }  // namespace nanx

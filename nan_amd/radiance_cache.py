@@ -234,6 +234,14 @@ class RadianceCache:
                                                        n, _ptr(out), self._stream(encoder)), "nan_nrc_table_grad")
         return out.view(self.config.levels, -1, 2)
 
+    def set_fused_training(self, fused: bool) -> None:
+        """Training kernels of a cache whose levels are all densely indexed:
+        the two-launch fused step (default) or the seven-launch step with
+        packed FP16 table-gradient atomics (hashed caches always take that
+        one)."""
+        _native.check(_native.lib().nan_nrc_set_fused_training(self.handle, 1 if fused else 0),
+                      "nan_nrc_set_fused_training")
+
     def last_step_stats(self) -> tuple[float, float]:
         """(mean loss, valid records) of the last Adam step; waits for the device."""
         out = (ctypes.c_float * 2)()

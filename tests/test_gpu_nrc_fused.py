@@ -1,0 +1,196 @@
+"""The two-launch fused Adam step of a densely indexed radiance cache (taken
+for minibatches of up to 8,192 records) against the seven-launch step and
+the float references of tests/nrc_reference.py: its forward and backward
+against autograd, the first step's update (Adam's first step moves every
+parameter with a nonzero gradient by the learning rate against the
+gradient's sign), online training on a smooth function, and the learned
+texel tracer on the Cornell box."""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from tests import nrc_reference as ref  # noqa: E402
+
+BBOX_MIN = np.array([-1.0, 0.0, -1.0])
+BBOX_MAX = np.array([1.0, 2.0, 1.0])
+DENSE = dict(levels=4, base_resolution=16.0, max_resolution=32.0, log2_table_size=19)
+
+
+def _say(msg: str) -> None:
+    print(f"[gpu-nrc-fused] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _cache(device, fused=True, **kw):
+    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+
+    cache = RadianceCache(device, BBOX_MIN, BBOX_MAX, RadianceCacheConfig(**{**DENSE, **kw}))
+    cache.set_fused_training(fused)
+    return cache
+
+
+def _inputs(n, seed, device):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(BBOX_MIN, BBOX_MAX, size=(n, 3))
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    alb = rng.uniform(0.05, 0.9, size=(n, 3))
+    t = lambda a: torch.from_numpy(a.astype(np.float32)).to(device.torch_device)  # noqa: E731
+    return (pos, nrm, alb), (t(pos), t(nrm), t(alb))
+
+
+def test_every_level_of_the_grid_is_dense():
+    _, dense = ref.level_grid(DENSE["levels"], DENSE["log2_table_size"], DENSE["base_resolution"],
+                              DENSE["max_resolution"])
+    assert all(dense)
+
+
+@pytest.mark.parametrize("n", [64, 100, 5000, 32_768])
+def test_fused_backward_matches_autograd(device, n):
+    """The fused forward and backward (the backward_raw hook of a dense cache
+    runs them) against autograd through the float reference, which rounds
+    where the kernels round: weight gradients per layer and the gradients of
+    all 48 inputs."""
+    _say(f"fused backward n={n}")
+    cache = _cache(device)
+    rng = np.random.default_rng(5)
+    w, t = cache.get_params()
+    w = rng.uniform(-0.3, 0.3, size=w.shape).astype(np.float32)
+    t = rng.uniform(-0.5, 0.5, size=t.shape).astype(np.float32)
+    cache.set_params(w, t)
+    _, dev_in = _inputs(n, 6, device)
+    target = torch.from_numpy(rng.uniform(0.0, 2.0, size=(n, 3)).astype(np.float32)).to(device.torch_device)
+    enc = cache.encode(*dev_in)
+    wg, ig = cache.backward_raw(*dev_in, target)
+    out, x_leaf, ws = ref.mlp(enc.cpu(), w)
+    ref.relative_l2_sum(out, target.cpu()).backward()
+    scale = 1.0 / (3.0 * n)
+    want_w = torch.cat([x.grad.reshape(-1) for x in ws]).numpy() * scale
+    got_w = wg.cpu().numpy()
+    for k, (o, i) in enumerate(ref.LAYER_SHAPES):
+        off = sum(a * b for a, b in ref.LAYER_SHAPES[:k])
+        g, h = got_w[off:off + o * i], want_w[off:off + o * i]
+        assert np.linalg.norm(g - h) <= 2e-2 * np.linalg.norm(h) + 1e-9, f"layer {k}"
+    got_i = ig.cpu().numpy()
+    want_i = x_leaf.grad.numpy() * scale
+    rel_i = np.linalg.norm(got_i - want_i) / np.linalg.norm(want_i)
+    assert rel_i < 1e-2, f"input gradient relative error {rel_i}"
+
+
+@pytest.mark.parametrize("n", [100, 4096, 8192])
+def test_first_fused_step_moves_by_learning_rate(device, n):
+    """Adam's first step: m_hat = g and v_hat = g^2, so a parameter with a
+    nonzero gradient moves by exactly lr against its sign.  Weight gradients
+    come from the seven-launch backward (checked against autograd
+    elsewhere); table gradients from its input gradients spread over the cell
+    corners in float64.  Entries no record touches keep their values."""
+    _say(f"first fused step n={n}")
+    lr = 1e-2
+    cache = _cache(device, learning_rate=lr)
+    rng = np.random.default_rng(20)
+    w0, t0 = cache.get_params()
+    w0 = rng.uniform(-0.3, 0.3, size=w0.shape).astype(np.float32)
+    t0 = rng.uniform(-0.5, 0.5, size=t0.shape).astype(np.float32)
+    cache.set_params(w0, t0)
+    (pos, _, _), dev_in = _inputs(n, 21, device)
+    target = torch.from_numpy(rng.uniform(0.0, 2.0, size=(n, 3)).astype(np.float32)).to(device.torch_device)
+    g, ig = cache.backward_raw(*dev_in, target)
+    g = g.cpu().numpy()
+    want = ref.table_gradient(pos, ig.cpu().numpy(), BBOX_MIN, BBOX_MAX, DENSE["levels"], t0.shape[1],
+                              DENSE["base_resolution"], DENSE["max_resolution"], DENSE["log2_table_size"])
+    cache.train_step(*dev_in, target)
+    loss, count = cache.last_step_stats()
+    assert np.isfinite(loss) and loss > 0 and count == n
+    w1, t1 = cache.get_params()
+    dw = (w1 - w0).astype(np.float64)
+    big = np.abs(g) > 1e-3 * np.abs(g).max()
+    assert big.sum() > 1000
+    agree = np.isclose(dw[big], -lr * np.sign(g[big]), rtol=0, atol=1e-6)
+    assert agree.mean() > 0.999, agree.mean()
+    assert np.abs(dw).max() <= lr * 1.001
+    dt = (t1 - t0).astype(np.float64)
+    assert np.all(dt[want == 0.0] == 0.0)  # untouched entries keep their values
+    hot = np.abs(want) > 1e-3 * np.abs(want).max()
+    assert hot.sum() > 100
+    np.testing.assert_allclose(dt[hot], -lr * np.sign(want[hot]), rtol=0, atol=2e-6)
+    assert np.abs(dt).max() <= lr * 1.001
+
+
+def _smooth(pos, alb):
+    return np.stack([np.sin(3 * pos[:, 0]) + 1.2, np.cos(2 * pos[:, 1]) + 1.1, pos[:, 2] ** 2 + 0.5 * alb[:, 0]],
+                    axis=1)
+
+
+def test_fused_and_seven_launch_steps_train_alike(device):
+    """Online training on a smooth function (fresh records every call, four
+    Adam steps per call, 600 steps): the fused and the seven-launch step reach
+    the same held-out error, each below 5%.  The loss of one minibatch is
+    noisy, so the median of the last ten calls is compared."""
+    n = 16_384
+    results = {}
+    for fused in (True, False):
+        _say(f"training fused={fused}")
+        cache = _cache(device, fused=fused, minibatches=4)
+        losses = []
+        for step in range(150):
+            (pos, _, alb), dev_in = _inputs(n, 500 + step, device)
+            target = torch.from_numpy(_smooth(pos, alb).astype(np.float32)).to(device.torch_device)
+            losses.append(cache.train_step(*dev_in, target).item())
+        (pos2, _, alb2), dev_in2 = _inputs(4096, 13, device)
+        f2 = _smooth(pos2, alb2)
+        pred = cache.infer(*dev_in2).cpu().numpy()
+        late = float(np.median(losses[-10:]))
+        results[fused] = (losses[0], late, float(np.sqrt(np.mean((pred - f2) ** 2) / np.mean(f2 ** 2))))
+        _say(f"fused={fused}: loss {losses[0]:.4f} -> median of the last ten {late:.5f}, held-out relative RMSE "
+             f"{results[fused][2]:.4f}")
+    for fused, (first, late, rel) in results.items():
+        assert late < 0.02 * first, fused
+        assert rel < 0.05, fused
+    a, b = results[True][2], results[False][2]
+    assert a < 1.5 * b + 5e-3 and b < 1.5 * a + 5e-3
+
+
+def test_learned_tracer_with_fused_training_on_the_cornell_box(device, cornell):
+    """The learned form on the Cornell box with the density-matched dense
+    grid: after 64 frames the fused and the seven-launch training give the
+    same RMSE against a 512-frame plain bake (within 15%), unbiased in the
+    mean (within 5%)."""
+    from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
+    from nan_amd.scene import Scene
+    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
+    node, layout, _ = cornell
+    sc = Scene(device, node)
+    tracer = TextureSpacePathTracer(device, sc, layout, samples_per_texel=1, max_bounces=3, mode="megakernel")
+    enc = device.create_command_encoder()
+    slots = layout.total_payload_count
+    bake = torch.zeros((slots, 4), dtype=torch.float32, device=device.torch_device)
+    for it in range(512):
+        tracer.execute(enc, bake, 100_000 + it, reset=(it == 0))
+    b = bake[:, :3].cpu().numpy().astype(np.float64)
+    lo, hi = sc.bounds()
+    cfg = RadianceCacheConfig.for_texel_density(float(np.max(hi - lo)), layout.texels_per_unit, minibatches=4)
+    _, dense = ref.level_grid(cfg.levels, cfg.log2_table_size, cfg.base_resolution, cfg.max_resolution)
+    assert all(dense)
+    rmse = {}
+    for fused in (True, False):
+        _say(f"Cornell learned, fused={fused}")
+        cache = RadianceCache.for_scene(device, sc, cfg)
+        cache.set_fused_training(fused)
+        learned = torch.zeros_like(bake)
+        for it in range(64):
+            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0))
+        lq = learned[:, :3].cpu().numpy().astype(np.float64)
+        assert np.isfinite(lq).all()
+        assert abs(lq.mean() - b.mean()) < 0.05 * b.mean(), fused
+        rmse[fused] = float(np.sqrt(np.mean((lq - b) ** 2)))
+    _say(f"RMSE after 64 frames: fused {rmse[True]:.4f}, seven-launch {rmse[False]:.4f}")
+    assert abs(rmse[True] - rmse[False]) < 0.15 * rmse[False]
