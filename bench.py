@@ -226,6 +226,13 @@ def cache_config(scene, layout, minibatches: int):
                                                  minibatches=minibatches)
 
 
+# Adam steps per frame of the Cornell-box learned form: the training runs
+# beside the tracing, and more, smaller steps converge in fewer frames
+# (measured at 256 frames: 4 steps 0.33x, 8 steps 0.32-0.36x, 16 steps 0.25x
+# the plain tracer; at 64 frames 0.10x, 0.18-0.22x, 0.26x)
+LEARNED_MINIBATCHES = 8
+
+
 def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, plain_curve_frames: int = 2048,
                         train_samples: int = 65536, mlp_queries: int = 1 << 22, steps: int = 10,
                         warmup: int = 2) -> dict:
@@ -278,7 +285,7 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
     torch.cuda.synchronize()
     plain_ms = ev0.elapsed_time(ev1) / 512
 
-    cfg = cache_config(scene, layout, 4)
+    cfg = cache_config(scene, layout, LEARNED_MINIBATCHES)
     cache = RadianceCache.for_scene(device, scene, cfg)
     learned = torch.zeros_like(bake)
     out_frames = {}
@@ -288,7 +295,8 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
     for target in frames:
         ev0.record(stream)
         for it in range(done, target):
-            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=train_samples)
+            tracer.execute_learned(enc, learned, it, cache, reset=(it == 0), train_samples=train_samples,
+                                   overlap=True)
         ev1.record(stream)
         torch.cuda.synchronize()
         elapsed_ms += ev0.elapsed_time(ev1)
@@ -308,12 +316,12 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
     # the timed steps: learned frames continuing the run above
     it = frames[-1]
     for _ in range(warmup):
-        tracer.execute_learned(enc, learned, it, cache, train_samples=train_samples)
+        tracer.execute_learned(enc, learned, it, cache, train_samples=train_samples, overlap=True)
         it += 1
     torch.cuda.synchronize()
     ev0.record(stream)
     for _ in range(steps):
-        tracer.execute_learned(enc, learned, it, cache, train_samples=train_samples)
+        tracer.execute_learned(enc, learned, it, cache, train_samples=train_samples, overlap=True)
         it += 1
     ev1.record(stream)
     torch.cuda.synchronize()
@@ -333,8 +341,9 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
                      f"{cfg.levels}-level hash grid (16 to {cfg.max_resolution:g} cells, matched to the texel "
                      "density) x 2 FP16 features, 2^19 entries/level, normal + albedo, 4x64 ReLU MLP "
                      f"FP16 MFMA / FP32 accumulate, Adam lr 1e-2, relative L2, {train_samples} training suffixes "
-                     "per frame in 4 Adam steps, paths end in the cache after 1 bounce, emitter light sampling "
-                     "with MIS, ramped mean"),
+                     f"per frame in {LEARNED_MINIBATCHES} Adam steps, trained on a second stream beside the next "
+                     "frame's tracing (each frame's queries see the cache trained through the previous frame), "
+                     "paths end in the cache after 1 bounce, emitter light sampling with MIS, ramped mean"),
         "cache_levels": cfg.levels,
         "cache_max_resolution": cfg.max_resolution,
         "bake": f"{bake_frames}-frame plain bake (independent RNG streams)",

@@ -83,6 +83,7 @@ LEARNED_EMITTER_NEE = 1
 LEARNED_RAMPED_MEAN = 2
 LEARNED_TEXEL_QUERY = 4
 LEARNED_RNG_PCG = 8
+LEARNED_OVERLAP = 16
 RNG_MODES = {"reference": 0, "pcg": 1}  # NAN_RNG_REFERENCE, NAN_RNG_PCG
 
 TRACER_MEGAKERNEL = 0

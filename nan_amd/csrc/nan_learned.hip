@@ -19,6 +19,8 @@ using namespace nanx;
 struct LearnedDev {
     float *tr_pos, *tr_nrm, *tr_alb, *tr_target;  // training records [n][3]
     uint32_t *tr_valid;
+    float *pv_rec;  // NAN_LEARNED_OVERLAP: the previous call's records, trained on while this call
+                    // traces: pos, nrm, alb, target [cap][3] and valid u32 [cap], one after another
     float *q_pos, *q_nrm, *q_alb, *q_w;  // cache queries [slots * spp][3], weight (0: none)
     float4 *q_pred;
     float4 *partial;  // per payload slot: sum over samples of the terms outside the cache
@@ -29,11 +31,19 @@ struct LearnedBuffers {
     LearnedDev d{};
     uint32_t cap_train = 0;
     uint64_t cap_query = 0;
+    uint32_t cap_prev = 0;     // records pv_rec holds
+    // NAN_LEARNED_OVERLAP: the training of the previous call's records runs on
+    // side_stream, forked from and joined into the caller's stream
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_trained = nullptr, ev_traced = nullptr, ev_join = nullptr;
     ~LearnedBuffers() {
         for (void *p : {(void *)d.tr_pos, (void *)d.tr_nrm, (void *)d.tr_alb, (void *)d.tr_target,
                         (void *)d.tr_valid, (void *)d.q_pos, (void *)d.q_nrm, (void *)d.q_alb, (void *)d.q_w,
-                        (void *)d.q_pred, (void *)d.partial})
+                        (void *)d.q_pred, (void *)d.partial, (void *)d.pv_rec})
             hipFree(p);
+        for (hipEvent_t e : {ev_fork, ev_trained, ev_traced, ev_join})
+            if (e) hipEventDestroy(e);
+        if (side_stream) hipStreamDestroy(side_stream);
     }
 };
 
@@ -244,15 +254,9 @@ __device__ __forceinline__ bool first_bounce(const DevScene &S, const Shading &s
 // suffix radiance of y1 plus y1's emission (MIS-weighted against the light
 // sampling x0 does when emitters are light-sampled).
 template <int STACK, int BLOCK, bool COUNT, bool MAT>
-__global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayout Ly, uint32_t iteration,
-                                                              uint32_t spp, uint32_t max_bounces, uint32_t n,
-                                                              uint32_t flags, LearnedDev B,
-                                                              unsigned long long *counters) {
-    if (B.frame) iteration = B.frame[0];
-    __shared__ uint32_t stack[STACK * BLOCK];
-    uint32_t *col = stack + threadIdx.x;
-    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
-    uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void train_path(const DevScene &S, const DevLayout &Ly, uint32_t iteration, uint32_t spp,
+                                           uint32_t max_bounces, uint32_t n, uint32_t flags, const LearnedDev &B,
+                                           uint32_t j, uint32_t *col, PathCounters &pc) {
     if (j < n) {
         bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && S.emitter_count > 0;
         bool at_x0 = (flags & NAN_LEARNED_TEXEL_QUERY) != 0;
@@ -293,6 +297,18 @@ __global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayo
         put3(B.tr_target, j, target);
         B.tr_valid[j] = valid ? 1u : 0u;
     }
+}
+
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
+__global__ __launch_bounds__(BLOCK) void learned_train_paths(DevScene S, DevLayout Ly, uint32_t iteration,
+                                                              uint32_t spp, uint32_t max_bounces, uint32_t n,
+                                                              uint32_t flags, LearnedDev B,
+                                                              unsigned long long *counters) {
+    if (B.frame) iteration = B.frame[0];
+    __shared__ uint32_t stack[STACK * BLOCK];
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
+    train_path<STACK, BLOCK, COUNT, MAT>(S, Ly, iteration, spp, max_bounces, n, flags, B,
+                                         blockIdx.x * BLOCK + threadIdx.x, stack + threadIdx.x, pc);
     if (COUNT) flush_counters(pc, counters);
 }
 
@@ -312,14 +328,9 @@ __device__ __forceinline__ void texel_slots(const DevLayout &Ly, uint64_t w, uin
 // the first bounce; queries for the cache go to q = slot * spp + k (weight 0:
 // no query).
 template <int STACK, int BLOCK, bool COUNT, bool MAT>
-__global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLayout Ly, uint32_t iteration,
-                                                               uint32_t spp, uint32_t flags, LearnedDev B,
-                                                               unsigned long long *counters) {
-    if (B.frame) iteration = B.frame[0];
-    __shared__ uint32_t stack[STACK * BLOCK];
-    uint32_t *col = stack + threadIdx.x;
-    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
-    uint64_t w = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void texel_first_bounce(const DevScene &S, const DevLayout &Ly, uint32_t iteration,
+                                                   uint32_t spp, uint32_t flags, const LearnedDev &B, uint64_t w,
+                                                   uint32_t *col, PathCounters &pc) {
     if (w < Ly.total_front) {
         const Light &L = S.light;
         bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && S.emitter_count > 0;
@@ -364,7 +375,53 @@ __global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLay
             B.partial[slot] = make_float4(acc.x, acc.y, acc.z, 0.0f);
         }
     }
+}
+
+template <int STACK, int BLOCK, bool COUNT, bool MAT>
+__global__ __launch_bounds__(BLOCK) void learned_first_bounce(DevScene S, DevLayout Ly, uint32_t iteration,
+                                                               uint32_t spp, uint32_t flags, LearnedDev B,
+                                                               unsigned long long *counters) {
+    if (B.frame) iteration = B.frame[0];
+    __shared__ uint32_t stack[STACK * BLOCK];
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
+    texel_first_bounce<STACK, BLOCK, COUNT, MAT>(S, Ly, iteration, spp, flags, B,
+                                                 (uint64_t)blockIdx.x * BLOCK + threadIdx.x, stack + threadIdx.x, pc);
     if (COUNT) flush_counters(pc, counters);
+}
+
+// One launch for a call's texel samples (blocks [0, texel_blocks)) and its
+// training paths (the blocks after them): both are latency bound, and one
+// launch costs the longer of the two instead of their sum.
+template <int STACK, int BLOCK, bool MAT>
+__global__ __launch_bounds__(BLOCK) void learned_trace(DevScene S, DevLayout Ly, uint32_t iteration, uint32_t spp,
+                                                        uint32_t max_bounces, uint32_t n_train, uint32_t flags,
+                                                        uint32_t texel_blocks, LearnedDev B) {
+    if (B.frame) iteration = B.frame[0];
+    __shared__ uint32_t stack[STACK * BLOCK];
+    PathCounters pc{0, 0, 0, 0, 0, 0, 0};
+    if (blockIdx.x < texel_blocks)
+        texel_first_bounce<STACK, BLOCK, false, MAT>(S, Ly, iteration, spp, flags, B,
+                                                     (uint64_t)blockIdx.x * BLOCK + threadIdx.x, stack + threadIdx.x,
+                                                     pc);
+    else
+        train_path<STACK, BLOCK, false, MAT>(S, Ly, iteration, spp, max_bounces, n_train, flags, B,
+                                             (blockIdx.x - texel_blocks) * BLOCK + threadIdx.x, stack + threadIdx.x,
+                                             pc);
+}
+
+// The call's training records into the previous-frame buffer (same layout,
+// cap records per array) for the next call's training.
+__global__ void learned_keep_records(LearnedDev B, uint32_t n, uint32_t cap) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float *o = B.pv_rec;
+    for (int a = 0; a < 3; ++a) {
+        o[3 * (size_t)j + a] = B.tr_pos[3 * (size_t)j + a];
+        o[3 * ((size_t)cap + j) + a] = B.tr_nrm[3 * (size_t)j + a];
+        o[3 * (2 * (size_t)cap + j) + a] = B.tr_alb[3 * (size_t)j + a];
+        o[3 * (3 * (size_t)cap + j) + a] = B.tr_target[3 * (size_t)j + a];
+    }
+    reinterpret_cast<uint32_t *>(o + 12 * (size_t)cap)[j] = B.tr_valid[j];
 }
 
 // Texel-query form: every texel sample takes the sun term at x0 and (with
@@ -492,6 +549,32 @@ int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query, hipStream_t s) {
     return 0;
 }
 
+// NAN_LEARNED_OVERLAP's working set: the previous call's records (zeroed, so
+// the first call trains on nothing) and the side stream with its events.
+int reserve_overlap(NanTracer *t, uint32_t n_train, hipStream_t s) {
+    LearnedBuffers &B = *t->learned;
+    if (n_train > B.cap_prev) {
+        if (stream_capturing(s))
+            return fail(nullptr, 0, "learned forms: %u training paths exceed the %u reserved for overlapped "
+                                    "training; call nan_tst_learned_reserve before graph capture", n_train, B.cap_prev);
+        HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
+        B.cap_prev = 0;
+        if (grow(&B.d.pv_rec, (size_t)n_train * 13 * 4)) return -1;
+        HIP_TRY(hipMemset(B.d.pv_rec, 0, (size_t)n_train * 13 * 4), nullptr, 0);
+        HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
+        B.cap_prev = n_train;
+    }
+    if (!B.side_stream) {
+        if (stream_capturing(s))
+            return fail(nullptr, 0, "learned forms: overlap stream not created; call nan_tst_learned_reserve before "
+                                    "graph capture");
+        HIP_TRY(hipStreamCreateWithFlags(&B.side_stream, hipStreamNonBlocking), nullptr, 0);
+        for (hipEvent_t *e : {&B.ev_fork, &B.ev_trained, &B.ev_traced, &B.ev_join})
+            HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming), nullptr, 0);
+    }
+    return 0;
+}
+
 LearnedIO learned_io(const LearnedDev &B) {
     return LearnedIO{B.tr_pos, B.tr_nrm, B.tr_alb, B.tr_target, B.tr_valid, B.q_pos, B.q_nrm, B.q_alb, B.q_w, B.q_pred,
                      B.frame};
@@ -547,6 +630,57 @@ int launch_wavefront(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteratio
     return 0;
 }
 
+// NAN_LEARNED_OVERLAP: the cache trains on the previous call's records on
+// the side stream while this call traces its texel samples and training
+// paths in one launch; the cache queries wait for the training.
+template <int STACK, bool MAT>
+int launch_overlap(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t nspp, uint32_t bounces,
+                   uint32_t reset, uint32_t n_train, uint32_t flags, hipStream_t s) {
+    const DevScene &S = t->scene->dev;
+    DevLayout Ly = dev_layout(t);
+    LearnedBuffers &LB = *t->learned;
+    const LearnedDev &B = LB.d;
+    hipStream_t side = LB.side_stream;
+    HIP_TRY(hipEventRecord(LB.ev_fork, s), nullptr, 0);
+    HIP_TRY(hipStreamWaitEvent(side, LB.ev_fork, 0), nullptr, 0);
+    // the previous call's records; zeros (no valid record, so a step that
+    // changes no parameter) before the first overlapped call
+    const uint32_t cap = LB.cap_prev;
+    if (n_train) {
+        StageTimer tm(t, side, kStageCacheTrain);
+        const float *pv = B.pv_rec;
+        if (nrc_train_step(c, pv, pv + 3 * (size_t)cap, pv + 6 * (size_t)cap, pv + 9 * (size_t)cap,
+                           reinterpret_cast<const uint32_t *>(pv + 12 * (size_t)cap), n_train, side))
+            return -1;
+    }
+    HIP_TRY(hipEventRecord(LB.ev_trained, side), nullptr, 0);
+    {
+        StageTimer tm(t, s, kStageFirstBounce);
+        const uint32_t texel_blocks = grid_for(t->total_front, kTraceBlock);
+        const uint32_t train_blocks = grid_for(n_train, kTraceBlock);
+        hipLaunchKernelGGL((learned_trace<STACK, kTraceBlock, MAT>), dim3(texel_blocks + train_blocks),
+                           dim3(kTraceBlock), 0, s, S, Ly, iteration, nspp, bounces, n_train, flags, texel_blocks, B);
+    }
+    HIP_TRY(hipEventRecord(LB.ev_traced, s), nullptr, 0);
+    HIP_TRY(hipStreamWaitEvent(side, LB.ev_traced, 0), nullptr, 0);
+    if (n_train)
+        hipLaunchKernelGGL(learned_keep_records, dim3(grid_for(n_train, 256)), dim3(256), 0, side, B, n_train, cap);
+    HIP_TRY(hipEventRecord(LB.ev_join, side), nullptr, 0);
+    HIP_TRY(hipStreamWaitEvent(s, LB.ev_trained, 0), nullptr, 0);
+    {
+        StageTimer tm(t, s, kStageCacheQuery);
+        uint64_t nq = t->total_payload * nspp;
+        if (nrc_predict(c, B.q_pos, B.q_nrm, B.q_alb, (uint32_t)nq, B.q_pred, s)) return -1;
+    }
+    {
+        StageTimer tm(t, s, kStageFinish);
+        hipLaunchKernelGGL((learned_finish<false>), dim3(grid_for(t->total_front, kFlatBlock)), dim3(kFlatBlock), 0,
+                           s, Ly, B, payload, nspp, reset, flags, (unsigned long long *)nullptr);
+    }
+    HIP_TRY(hipStreamWaitEvent(s, LB.ev_join, 0), nullptr, 0);
+    return check_launch("learned tracer (overlapped training)");
+}
+
 template <int STACK, bool MAT>
 int launch_mat(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uint32_t spp, uint32_t bounces,
                uint32_t reset, uint32_t n_train, uint32_t flags, unsigned long long *counters, hipStream_t s) {
@@ -554,6 +688,8 @@ int launch_mat(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration, uin
     DevLayout Ly = dev_layout(t);
     const LearnedDev &B = t->learned->d;
     uint32_t nspp = spp > 1u ? spp : 1u;
+    if ((flags & NAN_LEARNED_OVERLAP) && !counters && !(flags & NAN_LEARNED_TEXEL_QUERY))
+        return launch_overlap<STACK, MAT>(t, c, payload, iteration, nspp, bounces, reset, n_train, flags, s);
     {
         StageTimer tm(t, s, kStageLearnedPaths);
         if (n_train) {
@@ -639,8 +775,10 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
         return fail(nullptr, 0, "nan_tst_execute_learned: BVH too deep for the per-texel kernels");
     if (max_bounces == 0) return fail(nullptr, 0, "nan_tst_execute_learned: max_bounces must be at least 1");
     if (flags & ~(uint32_t)(NAN_LEARNED_EMITTER_NEE | NAN_LEARNED_RAMPED_MEAN | NAN_LEARNED_TEXEL_QUERY |
-                            NAN_LEARNED_RNG_PCG))
+                            NAN_LEARNED_RNG_PCG | NAN_LEARNED_OVERLAP))
         return fail(nullptr, 0, "nan_tst_execute_learned: unknown flags");
+    if ((flags & NAN_LEARNED_OVERLAP) && (flags & NAN_LEARNED_TEXEL_QUERY))
+        return fail(nullptr, 0, "nan_tst_execute_learned: overlapped training is a one-bounce-form option");
     tracer->rng_mode = (flags & NAN_LEARNED_RNG_PCG) ? NAN_RNG_PCG : NAN_RNG_REFERENCE;
     if (tracer->total_front == 0) return 0;
     uint32_t nspp = samples_per_texel > 1u ? samples_per_texel : 1u;
@@ -648,7 +786,8 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
     if (nq >= (1ull << 31)) return fail(nullptr, 0, "nan_tst_execute_learned: too many cache queries per frame");
     hipStream_t s = as_stream(stream);
     if (reserve(tracer, train_samples, nq, s) || nrc_reserve(nrc, (uint32_t)nq, s) ||
-        nrc_reserve_train(nrc, train_samples, s))
+        nrc_reserve_train(nrc, train_samples, s) ||
+        ((flags & NAN_LEARNED_OVERLAP) && reserve_overlap(tracer, train_samples, s)))
         return -1;
     unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
     // device-frame mode: iteration and reset come from the tracer's device
@@ -660,7 +799,8 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
     // counters, training paths within the tracer's path slots
     bool nee = (flags & NAN_LEARNED_EMITTER_NEE) && tracer->scene->dev.emitter_count > 0;
     int rc;
-    if (tracer->mode == NAN_TRACER_WAVEFRONT && nspp == 1 && !nee && !c && train_samples <= tracer->total_front)
+    if (tracer->mode == NAN_TRACER_WAVEFRONT && nspp == 1 && !nee && !c && !(flags & NAN_LEARNED_OVERLAP) &&
+        train_samples <= tracer->total_front)
         rc = launch_wavefront(tracer, nrc, (float4 *)payload, iteration, max_bounces, reset, train_samples, flags, s);
     else if (tracer->scene->stack_depth <= 32)
         rc = launch<32>(tracer, nrc, (float4 *)payload, iteration, nspp, max_bounces, reset, train_samples, flags, c,
@@ -681,7 +821,8 @@ NAN_API int nan_tst_learned_reserve(NanTracer *tracer, NanNrc *nrc, uint32_t sam
     if (stream_capturing(nullptr))
         return fail(nullptr, 0, "nan_tst_learned_reserve: called during graph capture");
     if (reserve(tracer, train_samples, nq, nullptr) || nrc_reserve(nrc, (uint32_t)std::max<uint64_t>(nq, 1), nullptr) ||
-        nrc_reserve_train(nrc, std::max(train_samples, 1u), nullptr))
+        nrc_reserve_train(nrc, std::max(train_samples, 1u), nullptr) ||
+        reserve_overlap(tracer, std::max(train_samples, 1u), nullptr))
         return -1;
     return 0;
 }

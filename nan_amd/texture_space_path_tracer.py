@@ -132,7 +132,7 @@ class TextureSpacePathTracer:
 
     def execute_learned(self, command_encoder: CommandEncoder, output: torch.Tensor, iteration: int, cache, *,
                         reset: bool = False, train_samples: int = 65536, emitter_nee: bool = True,
-                        ramped_mean: bool = True, texel_query: bool = False,
+                        ramped_mean: bool = True, texel_query: bool = False, overlap: bool = False,
                         counters: torch.Tensor | None = None) -> None:
         """One iteration of the learned form: trains ``cache`` (a
         :class:`~nan_amd.radiance_cache.RadianceCache`) on ``train_samples``
@@ -144,8 +144,13 @@ class TextureSpacePathTracer:
         less trained cache fades out.  ``texel_query`` ends paths in the
         cache at the texel itself: the cache learns the indirect irradiance
         field and each texel adds only its sun and light-sampled emitter
-        terms (no bounce ray; use a cache trained in this form only).  No
-        counterpart in the reference."""
+        terms (no bounce ray; use a cache trained in this form only).
+        ``overlap`` (one-bounce form) trains the cache on the previous
+        call's path suffixes on a second stream while this call traces, so
+        the queries see the cache trained through the previous frame, as
+        online neural radiance caching does; before the first such call the
+        kept records are zeros (none valid), so its step leaves the
+        parameters unchanged.  No counterpart in the reference."""
         as_payload_tensor(output, self.layout.total_payload_count)
         if counters is not None and (counters.dtype != torch.int64 or counters.numel() < _native.COUNTER_COUNT
                                      or not counters.is_cuda):
@@ -156,7 +161,8 @@ class TextureSpacePathTracer:
             self.handle, cache.handle, output.data_ptr(), max(0, int(iteration)), self.samples_per_texel,
             self.max_bounces, 1 if reset else 0, int(train_samples),
             (_native.LEARNED_EMITTER_NEE if emitter_nee else 0) | (_native.LEARNED_RAMPED_MEAN if ramped_mean else 0)
-            | (_native.LEARNED_TEXEL_QUERY if texel_query else 0) | (_native.LEARNED_RNG_PCG if self.rng == "pcg" else 0),
+            | (_native.LEARNED_TEXEL_QUERY if texel_query else 0) | (_native.LEARNED_RNG_PCG if self.rng == "pcg" else 0)
+            | (_native.LEARNED_OVERLAP if overlap else 0),
             None if counters is None else counters.data_ptr(), command_encoder.stream_handle)
         _native.check(rc, "nan_tst_execute_learned")
 
