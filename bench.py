@@ -226,11 +226,13 @@ def cache_config(scene, layout, minibatches: int):
                                                  minibatches=minibatches)
 
 
-# Adam steps per frame of the Cornell-box learned form: the training runs
-# beside the tracing, and more, smaller steps converge in fewer frames
-# (measured at 256 frames: 4 steps 0.33x, 8 steps 0.32-0.36x, 16 steps 0.25x
-# the plain tracer; at 64 frames 0.10x, 0.18-0.22x, 0.26x)
-LEARNED_MINIBATCHES = 8
+# Adam steps per frame of the Cornell-box learned form, trained beside the
+# next frame's tracing.  Mean of three cache seeds against the plain tracer
+# at equal RMSE: 4 steps (358 us per frame) 0.37x at 256 frames, 0.15x at 64;
+# 8 steps (448 us) 0.31x and 0.20x; 4 steps without the overlap (415 us)
+# 0.33x and 0.14x; one run varies by about +-0.04 at 256 frames and +-0.07 at
+# 64.  More, smaller steps converge in fewer frames but cost more per frame.
+LEARNED_MINIBATCHES = 4
 
 
 def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, plain_curve_frames: int = 2048,
