@@ -42,7 +42,7 @@ using namespace nanx;
 
 namespace {
 
-constexpr uint64_t kFusedStepRecords = 8192;  // largest minibatch the fused step takes
+constexpr uint64_t kFusedStepRecords = 32768;  // largest minibatch the fused step takes
 
 constexpr int kWtPad = 8;           // LDS row padding (halves) of the transposed weights
 constexpr int kTPad = 8;            // LDS row padding (halves) of the backward transposes
@@ -723,9 +723,8 @@ int nrc_train_step(NanNrc *c, const float *pos, const float *nrm, const float *a
     for (uint32_t k = 0; k < parts; ++k) {
         uint64_t a = (uint64_t)n * k / parts, b = (uint64_t)n * (k + 1) / parts;
         const uint32_t *v = valid ? valid + a : nullptr;
-        // the fused step is latency bound (one wave per 64 records): faster than the
-        // seven-launch step up to about 8K records per step (measured 38 vs 52 us
-        // at 4,096 and 52 vs 60 us at 8,192 records), slower at 16K (78 vs 73 us)
+        // the fused step (measured against the seven-launch step: 38 vs 54 us at
+        // 4,096 records, 50 vs 64 at 8,192, 67 vs 78 at 16,384, 105 vs 111 at 32,768)
         const bool fused = c->fused_train && b - a <= kFusedStepRecords && nrc_fused_eligible(c, (uint32_t)(b - a));
         if (fused ? nrc_train_fused(c, pos + 3 * a, nrm + 3 * a, alb + 3 * a, target + 3 * a, v, (uint32_t)(b - a), s)
                   : nrc_train_one(c, pos + 3 * a, nrm + 3 * a, alb + 3 * a, target + 3 * a, v, (uint32_t)(b - a), s))

@@ -1,20 +1,22 @@
 // Fused Adam step of the learned radiance cache, for caches whose hash-grid
 // levels are all densely indexed ((res+1)^3 entries fit the table: the grids
-// of small scenes).  Two launches per step instead of seven:
+// of small scenes).  Three launches per step instead of seven:
 //
-//   nrc_fused_fb      one wave per 64-record group (four 16-record tiles):
-//                     hash-grid encoding, the 48 -> 64x4 -> 16 ReLU MLP
-//                     forward (the layers' inputs kept in LDS for the
-//                     backward), the relative L2 loss, and the backward.  The
-//                     wave's weight gradients over its 64 records go to its
-//                     own partial in MFMA fragment order (no cross-wave
-//                     reduction); the hash-feature gradients go straight to
-//                     the table's packed FP16 gradient pairs (one atomic per
-//                     record, level and corner).
-//   nrc_fused_update  the weights' Adam on the fixed-order sum of the
-//                     partials (sixteen threads per element), the table's lazy
-//                     Adam over the dense lattices only, and the step's loss
-//                     statistics.
+//   nrc_fused_fb          one wave per 64-record group (four 16-record
+//                         tiles): hash-grid encoding, the 48 -> 64x4 -> 16
+//                         ReLU MLP forward (the layers' inputs kept in LDS for
+//                         the backward), the relative L2 loss, and the
+//                         backward.  The wave's weight gradients over its 64
+//                         records go to its own partial in MFMA fragment order
+//                         (no cross-wave reduction); each record's gradient
+//                         with respect to its hash features goes to a
+//                         per-level array.
+//   nrc_fused_update      the weights' Adam on the fixed-order sum of the
+//                         partials (sixteen threads per element) and the
+//                         step's loss statistics, beside the table-gradient
+//                         scatter: one thread per (record, level, corner), one
+//                         packed FP16 atomic each.
+//   nrc_fused_table_adam  the table's lazy Adam over the dense lattices.
 //
 // The step computes what nrc_train_one does: the same loss, gradient clamp,
 // rounding points (activations and dY in FP16, FP32 accumulation), FP16
@@ -26,7 +28,10 @@
 // computed wrong forward values at -O3 on gfx950 although the device reports
 // 160 KB per block.  LDS float atomics were measured too: about 200 cycles
 // per ds_add_f32 wave instruction, which made summing table gradients per
-// z-slab in LDS slower than the global packed atomics used here.
+// z-slab in LDS slower than the global packed atomics used here.  The
+// atomics run from their own grid of one thread per (record, corner): issued
+// from the forward-backward kernel's few waves they reached about 13 G/s,
+// against about 30 G/s spread over many threads.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -49,9 +54,9 @@ struct FbArgs {
     const _Float16 *w16t, *w16n;  // per layer [out][in] and [in][out]
     const float *pos, *nrm, *alb, *target;
     const uint32_t *valid;  // nullable
-    uint32_t n, groups;
-    float *partial;    // [gridDim.x][kNrcWeights], weight order [out][in]
-    __half2 *tgrad;    // table gradient pairs (x kTableGradScale); null: none
+    uint32_t n, groups, cap;
+    float *partial;    // [gridDim.x][kNrcWeights], MFMA fragment order
+    float2 *gtab;      // [level][cap]: each record's gradient wrt its two features of the level; null: none
     float2 *lossp;     // [gridDim.x]: loss sum, valid records
     AdamState *as;     // null: no Adam bookkeeping
     float b1, b2;
@@ -60,6 +65,10 @@ struct FbArgs {
 };
 
 struct UpdArgs {
+    NrcGrid E;
+    const float *pos;
+    const float2 *gtab;
+    uint32_t cap, scatter_blocks;
     uint32_t n, parts;
     const float *partial;
     float *w32, *wm, *wv;
@@ -327,26 +336,13 @@ __global__ __launch_bounds__(64) void nrc_fused_fb(FbArgs A) {
                         make_float4(dh[t][it][0] * A.input_scale, dh[t][it][1] * A.input_scale,
                                     dh[t][it][2] * A.input_scale, dh[t][it][3] * A.input_scale);
             }
-            if (!A.tgrad) continue;
-            float p[3];
-            for (int a = 0; a < 3; ++a) p[a] = A.pos[3 * (size_t)s + a];
+            if (!A.gtab) continue;
 #pragma unroll
             for (int it = 0; it < 2; ++it)
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const uint32_t lv = (uint32_t)(it * 8 + 2 * q + h);
-                    const float g0 = dh[t][it][2 * h] * kTableGradScale, g1 = dh[t][it][2 * h + 1] * kTableGradScale;
-                    if (lv >= A.E.levels || (g0 == 0.0f && g1 == 0.0f)) continue;
-                    uint32_t base[3];
-                    float f[3];
-                    level_cell(A.E, lv, p, base, f);
-                    __half2 *tg = A.tgrad + (size_t)lv * A.E.entries;
-                    for (uint32_t k = 0; k < 8; ++k) {
-                        uint32_t idx;
-                        float w;
-                        cell_corner(A.E, lv, base, f, k, idx, w);
-                        unsafeAtomicAdd(&tg[idx], __halves2half2(__float2half(w * g0), __float2half(w * g1)));
-                    }
+                    if (lv < A.E.levels) A.gtab[(size_t)lv * A.cap + s] = make_float2(dh[t][it][2 * h], dh[t][it][2 * h + 1]);
                 }
         }
     }
@@ -378,13 +374,35 @@ __device__ __forceinline__ void adam2(float (&w)[2], float (&m)[2], float (&v)[2
     }
 }
 
+// Hash-table gradients: one thread per (record, corner) of one level
+// (blockIdx.y), eight neighbouring lanes the eight corners of a record, one
+// packed FP16 atomic each (many threads keep the device's atomic rate busy).
+__device__ __forceinline__ void scatter_corner(const UpdArgs &U, uint32_t gid, uint32_t level) {
+    const uint32_t s = gid >> 3, k = gid & 7u;
+    if (s >= U.n) return;
+    const float2 g = U.gtab[(size_t)level * U.cap + s];
+    if (g.x == 0.0f && g.y == 0.0f) return;
+    uint32_t base[3], idx;
+    float f[3], w;
+    level_cell(U.E, level, U.pos + 3 * (size_t)s, base, f);
+    cell_corner(U.E, level, base, f, k, idx, w);
+    unsafeAtomicAdd(&U.tgrad[(size_t)level * U.entries + idx],
+                    __halves2half2(__float2half(w * g.x * kTableGradScale), __float2half(w * g.y * kTableGradScale)));
+}
+
+// Blocks [0, mlp_blocks): the weights' Adam and the step's loss statistics;
+// blocks after them (blockIdx.y = level): the table-gradient atomics.
 __global__ __launch_bounds__(kUpdBlock) void nrc_fused_update(UpdArgs U) {
     __shared__ float red[kUpdBlock];
+    const uint32_t tid = threadIdx.x;
+    if (blockIdx.x >= U.mlp_blocks) {
+        scatter_corner(U, (blockIdx.x - U.mlp_blocks) * kUpdBlock + tid, blockIdx.y);
+        return;
+    }
+    if (blockIdx.y != 0) return;
     const float scale = 1.0f / (3.0f * fmaxf((float)U.n, 1.0f));
     const float c1 = U.as->c1, c2 = U.as->c2;
-    const uint32_t tid = threadIdx.x;
     if (blockIdx.x == 0 && tid == 0) {
-        U.as->step = U.as->next_step;
         float l = 0.0f, v = 0.0f;  // the step's loss statistics (fixed order)
         for (uint32_t p = 0; p < U.parts; ++p) {
             l += U.lossp[p].x;
@@ -393,27 +411,6 @@ __global__ __launch_bounds__(kUpdBlock) void nrc_fused_update(UpdArgs U) {
         U.stats[0] = l;
         U.stats[1] = (float)U.n;
         U.stats[2] = v;
-    }
-    if (blockIdx.x >= U.mlp_blocks) {  // table: one lattice entry per thread, lazy Adam
-        const uint32_t i = (blockIdx.x - U.mlp_blocks) * kUpdBlock + tid;
-        if (i >= U.lattice_total) return;
-        uint32_t l = 0;
-        while (i >= U.lattice_begin[l + 1]) ++l;
-        const size_t k = (size_t)l * U.entries + (i - U.lattice_begin[l]);
-        const __half2 g2 = U.tgrad[k];
-        float g[2] = {__low2float(g2), __high2float(g2)};
-        if (g[0] == 0.0f && g[1] == 0.0f && !U.dense_adam) return;
-        U.tgrad[k] = __halves2half2(__float2half(0.0f), __float2half(0.0f));
-        const float sc = scale / kTableGradScale;
-        for (int c = 0; c < 2; ++c) g[c] = isfinite(g[c]) ? g[c] * sc : 0.0f;
-        float4 *rec = reinterpret_cast<float4 *>(U.ts + k);
-        const float4 wm = rec[0], vp = rec[1];
-        float w[2] = {wm.x, wm.y}, m[2] = {wm.z, wm.w}, v[2] = {vp.x, vp.y};
-        adam2(w, m, v, g, U.lr, U.b1, U.b2, U.eps, c1, c2);
-        rec[0] = make_float4(w[0], w[1], m[0], m[1]);
-        rec[1] = make_float4(v[0], v[1], 0.0f, 0.0f);
-        U.t16[k] = __halves2half2(__float2half(w[0]), __float2half(w[1]));
-        return;
     }
     // partial element f (fragment order): thread (sub, fl) sums partials sub,
     // sub + 16, ... four loads at a time; the sixteen sums combine in a fixed
@@ -453,6 +450,32 @@ __global__ __launch_bounds__(kUpdBlock) void nrc_fused_update(UpdArgs U) {
     U.w16n[nrc_layer_offset(l) + (rel % in) * out + rel / in] = (_Float16)wi;
 }
 
+// The table's lazy Adam over the dense lattices, one entry per thread; the
+// step's last kernel stores Adam's step count.
+__global__ __launch_bounds__(kUpdBlock) void nrc_fused_table_adam(UpdArgs U) {
+    const float scale = 1.0f / (3.0f * fmaxf((float)U.n, 1.0f));
+    const float c1 = U.as->c1, c2 = U.as->c2;
+    const uint32_t i = blockIdx.x * kUpdBlock + threadIdx.x;
+    if (i == 0) U.as->step = U.as->next_step;
+    if (i >= U.lattice_total) return;
+    uint32_t l = 0;
+    while (i >= U.lattice_begin[l + 1]) ++l;
+    const size_t k = (size_t)l * U.entries + (i - U.lattice_begin[l]);
+    const __half2 g2 = U.tgrad[k];
+    float g[2] = {__low2float(g2), __high2float(g2)};
+    if (g[0] == 0.0f && g[1] == 0.0f && !U.dense_adam) return;
+    U.tgrad[k] = __halves2half2(__float2half(0.0f), __float2half(0.0f));
+    const float sc = scale / kTableGradScale;
+    for (int c = 0; c < 2; ++c) g[c] = isfinite(g[c]) ? g[c] * sc : 0.0f;
+    float4 *rec = reinterpret_cast<float4 *>(U.ts + k);
+    const float4 wm = rec[0], vp = rec[1];
+    float w[2] = {wm.x, wm.y}, m[2] = {wm.z, wm.w}, v[2] = {vp.x, vp.y};
+    adam2(w, m, v, g, U.lr, U.b1, U.b2, U.eps, c1, c2);
+    rec[0] = make_float4(w[0], w[1], m[0], m[1]);
+    rec[1] = make_float4(v[0], v[1], 0.0f, 0.0f);
+    U.t16[k] = __halves2half2(__float2half(w[0]), __float2half(w[1]));
+}
+
 // Fixed-order sum of the per-wave weight-gradient partials, times scale.
 __global__ void nrc_fused_sum(const float *partial, uint32_t parts, float scale, float *out) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,8 +498,9 @@ FbArgs fb_args(NanNrc *c, const float *pos, const float *nrm, const float *alb, 
     A.valid = valid;
     A.n = n;
     A.groups = (n + 63u) / 64u;
+    A.cap = c->cap_train_groups * 64u;
     A.partial = c->wpart;
-    A.tgrad = c->tg;
+    A.gtab = reinterpret_cast<float2 *>(c->denc);
     A.lossp = c->lossp;
     A.as = reinterpret_cast<AdamState *>(c->adam);
     A.b1 = c->cfg.beta1;
@@ -503,11 +527,16 @@ int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *
     if (!nrc_fused_eligible(c, n))
         return fail(nullptr, 0, "nan_nrc: the fused training step needs densely indexed levels and at most %u records",
                     (uint32_t)kNrcBwdWaves * 64u);
+    if (nrc_reserve_train(c, n, s)) return -1;
     FbArgs A = fb_args(c, pos, nrm, alb, target, valid, n);
     const uint32_t parts = A.groups;
     hipLaunchKernelGGL(nrc_fused_fb<false>, dim3(parts), dim3(64), 0, s, A);
     const NanNrcConfig &cf = c->cfg;
     UpdArgs U;
+    U.E = A.E;
+    U.pos = pos;
+    U.gtab = A.gtab;
+    U.cap = A.cap;
     U.n = n;
     U.parts = parts;
     U.partial = c->wpart;
@@ -536,8 +565,10 @@ int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *
     U.eps = cf.epsilon;
     U.dense_adam = cf.dense_table_adam ? 1u : 0u;
     U.mlp_blocks = ((uint32_t)kNrcWeights + kMlpPerBlock - 1) / kMlpPerBlock;
-    const uint32_t table_blocks = (U.lattice_total + kUpdBlock - 1) / kUpdBlock;
-    hipLaunchKernelGGL(nrc_fused_update, dim3(U.mlp_blocks + table_blocks), dim3(kUpdBlock), 0, s, U);
+    U.scatter_blocks = (uint32_t)(((uint64_t)n * 8u + kUpdBlock - 1) / kUpdBlock);
+    hipLaunchKernelGGL(nrc_fused_update, dim3(U.mlp_blocks + U.scatter_blocks, cf.levels), dim3(kUpdBlock), 0, s, U);
+    hipLaunchKernelGGL(nrc_fused_table_adam, dim3((U.lattice_total + kUpdBlock - 1) / kUpdBlock), dim3(kUpdBlock), 0,
+                       s, U);
     return check_launch("nrc fused training step");
 }
 
@@ -550,7 +581,7 @@ int nrc_fused_grads(NanNrc *c, const float *pos, const float *nrm, const float *
     FbArgs A = fb_args(c, pos, nrm, alb, target, nullptr, n);
     const uint32_t parts = A.groups;
     const float scale = 1.0f / (3.0f * (float)n);
-    A.tgrad = nullptr;  // parameters and their gradient buffers stay untouched
+    A.gtab = nullptr;  // parameters and their gradient buffers stay untouched
     A.as = nullptr;
     A.input_grad = input_grad;
     A.input_scale = scale;

@@ -176,9 +176,9 @@ int nrc_infer_into(NanNrc *c, const float *pos, const float *nrm, const float *a
 // slice size.
 int nrc_train_step(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                    const uint32_t *valid, uint32_t n, hipStream_t stream);
-// One Adam step in two launches (nan_nrc_fused.hip), for caches whose levels
-// are all densely indexed and steps of at most kNrcBwdWaves * 64 records; the
-// same loss and update as the seven-launch step.
+// One Adam step in three launches (nan_nrc_fused.hip), for caches whose
+// levels are all densely indexed and steps of at most kNrcBwdWaves * 64
+// records; the same loss and update as the seven-launch step.
 bool nrc_fused_eligible(const NanNrc *c, uint32_t n);
 int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                     const uint32_t *valid, uint32_t n, hipStream_t stream);

@@ -1,5 +1,5 @@
-"""The two-launch fused Adam step of a densely indexed radiance cache (taken
-for minibatches of up to 8,192 records) against the seven-launch step and
+"""The fused Adam step of a densely indexed radiance cache (taken for
+minibatches of up to 32,768 records) against the seven-launch step and
 the float references of tests/nrc_reference.py: its forward and backward
 against autograd, the first step's update (Adam's first step moves every
 parameter with a nonzero gradient by the learning rate against the
@@ -85,7 +85,7 @@ def test_fused_backward_matches_autograd(device, n):
     assert rel_i < 1e-2, f"input gradient relative error {rel_i}"
 
 
-@pytest.mark.parametrize("n", [100, 4096, 8192])
+@pytest.mark.parametrize("n", [100, 4096, 32_768])
 def test_first_fused_step_moves_by_learning_rate(device, n):
     """Adam's first step: m_hat = g and v_hat = g^2, so a parameter with a
     nonzero gradient moves by exactly lr against its sign.  Weight gradients
