@@ -228,9 +228,9 @@ def cache_config(scene, layout, minibatches: int):
 
 # Adam steps per frame of the Cornell-box learned form, trained beside the
 # next frame's tracing.  Mean of three cache seeds against the plain tracer
-# at equal RMSE: 4 steps (358 us per frame) 0.37x at 256 frames, 0.15x at 64;
-# 8 steps (448 us) 0.31x and 0.20x; 4 steps without the overlap (415 us)
-# 0.33x and 0.14x; one run varies by about +-0.04 at 256 frames and +-0.07 at
+# at equal RMSE: 4 steps (311 us per frame) 0.44x at 256 frames, 0.15x at 64;
+# 8 steps (435 us) 0.33x and 0.25x; 4 steps without the overlap (387 us)
+# 0.33x and 0.14x; one run varies by about +-0.03 at 256 frames and +-0.1 at
 # 64.  More, smaller steps converge in fewer frames but cost more per frame.
 LEARNED_MINIBATCHES = 4
 
