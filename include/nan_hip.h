@@ -411,13 +411,15 @@ NAN_API int nan_nrc_encode(NanNrc *nrc, const float *pos, const float *nrm, cons
  * nan_tst_execute_counted (may be NULL).  NAN_LEARNED_RNG_PCG selects the
  * PCG streams of NAN_RNG_PCG (default: the reference's).  With
  * NAN_LEARNED_OVERLAP (one-bounce form, per-texel kernels, no counters) the
- * cache trains on the previous call's training records, on a second stream
- * beside this call's tracing (texel samples and training paths in one
- * launch), and this call's records are kept for the next call: the queries
- * see a cache trained through the previous frame, as in online neural
- * radiance caching.  Before the first such call the kept records are zeros
- * (none valid), so that call's step changes no parameter and only advances
- * Adam's step count.  The work is forked from and joined back into `stream`.
+ * call first copies the parameters its queries read, then the cache trains
+ * on the previous call's training records on a second stream, beside this
+ * call's tracing (texel samples and training paths in one launch) and its
+ * queries of the copy, and this call's records are kept for the next call:
+ * the queries see the cache as the previous call's training left it
+ * (trained on records up to two frames old) and never wait for the
+ * training.  Before the first such call the kept records are zeros (none
+ * valid), so that call's step changes no parameter and only advances Adam's
+ * step count.  The work is forked from and joined back into `stream`.
  * No reference counterpart. */
 enum { NAN_LEARNED_EMITTER_NEE = 1, NAN_LEARNED_RAMPED_MEAN = 2, NAN_LEARNED_TEXEL_QUERY = 4,
        NAN_LEARNED_RNG_PCG = 8, NAN_LEARNED_OVERLAP = 16 };

@@ -32,16 +32,17 @@ struct LearnedBuffers {
     uint32_t cap_train = 0;
     uint64_t cap_query = 0;
     uint32_t cap_prev = 0;     // records pv_rec holds
-    // NAN_LEARNED_OVERLAP: the training of the previous call's records runs on
-    // side_stream, forked from and joined into the caller's stream
+    // NAN_LEARNED_OVERLAP: the parameter snapshot and the training of the
+    // previous call's records run on side_stream, forked from and joined into
+    // the caller's stream
     hipStream_t side_stream = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_trained = nullptr, ev_traced = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_snap = nullptr, ev_traced = nullptr, ev_join = nullptr;
     ~LearnedBuffers() {
         for (void *p : {(void *)d.tr_pos, (void *)d.tr_nrm, (void *)d.tr_alb, (void *)d.tr_target,
                         (void *)d.tr_valid, (void *)d.q_pos, (void *)d.q_nrm, (void *)d.q_alb, (void *)d.q_w,
                         (void *)d.q_pred, (void *)d.partial, (void *)d.pv_rec})
             hipFree(p);
-        for (hipEvent_t e : {ev_fork, ev_trained, ev_traced, ev_join})
+        for (hipEvent_t e : {ev_fork, ev_snap, ev_traced, ev_join})
             if (e) hipEventDestroy(e);
         if (side_stream) hipStreamDestroy(side_stream);
     }
@@ -550,8 +551,10 @@ int reserve(NanTracer *t, uint32_t n_train, uint64_t n_query, hipStream_t s) {
 }
 
 // NAN_LEARNED_OVERLAP's working set: the previous call's records (zeroed, so
-// the first call trains on nothing) and the side stream with its events.
-int reserve_overlap(NanTracer *t, uint32_t n_train, hipStream_t s) {
+// the first call trains on nothing), the cache's parameter snapshot and the
+// side stream with its events.
+int reserve_overlap(NanTracer *t, NanNrc *c, uint32_t n_train, hipStream_t s) {
+    if (nrc_reserve_snapshot(c, s)) return -1;
     LearnedBuffers &B = *t->learned;
     if (n_train > B.cap_prev) {
         if (stream_capturing(s))
@@ -569,7 +572,7 @@ int reserve_overlap(NanTracer *t, uint32_t n_train, hipStream_t s) {
             return fail(nullptr, 0, "learned forms: overlap stream not created; call nan_tst_learned_reserve before "
                                     "graph capture");
         HIP_TRY(hipStreamCreateWithFlags(&B.side_stream, hipStreamNonBlocking), nullptr, 0);
-        for (hipEvent_t *e : {&B.ev_fork, &B.ev_trained, &B.ev_traced, &B.ev_join})
+        for (hipEvent_t *e : {&B.ev_fork, &B.ev_snap, &B.ev_traced, &B.ev_join})
             HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming), nullptr, 0);
     }
     return 0;
@@ -643,6 +646,10 @@ int launch_overlap(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration,
     hipStream_t side = LB.side_stream;
     HIP_TRY(hipEventRecord(LB.ev_fork, s), nullptr, 0);
     HIP_TRY(hipStreamWaitEvent(side, LB.ev_fork, 0), nullptr, 0);
+    // the queries read the parameters as the previous call left them, copied
+    // before this call's steps change them
+    if (nrc_snapshot(c, side)) return -1;
+    HIP_TRY(hipEventRecord(LB.ev_snap, side), nullptr, 0);
     // the previous call's records; zeros (no valid record, so a step that
     // changes no parameter) before the first overlapped call
     const uint32_t cap = LB.cap_prev;
@@ -653,7 +660,6 @@ int launch_overlap(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration,
                            reinterpret_cast<const uint32_t *>(pv + 12 * (size_t)cap), n_train, side))
             return -1;
     }
-    HIP_TRY(hipEventRecord(LB.ev_trained, side), nullptr, 0);
     {
         StageTimer tm(t, s, kStageFirstBounce);
         const uint32_t texel_blocks = grid_for(t->total_front, kTraceBlock);
@@ -666,11 +672,11 @@ int launch_overlap(NanTracer *t, NanNrc *c, float4 *payload, uint32_t iteration,
     if (n_train)
         hipLaunchKernelGGL(learned_keep_records, dim3(grid_for(n_train, 256)), dim3(256), 0, side, B, n_train, cap);
     HIP_TRY(hipEventRecord(LB.ev_join, side), nullptr, 0);
-    HIP_TRY(hipStreamWaitEvent(s, LB.ev_trained, 0), nullptr, 0);
+    HIP_TRY(hipStreamWaitEvent(s, LB.ev_snap, 0), nullptr, 0);
     {
         StageTimer tm(t, s, kStageCacheQuery);
         uint64_t nq = t->total_payload * nspp;
-        if (nrc_predict(c, B.q_pos, B.q_nrm, B.q_alb, (uint32_t)nq, B.q_pred, s)) return -1;
+        if (nrc_predict(c, B.q_pos, B.q_nrm, B.q_alb, (uint32_t)nq, B.q_pred, s, true)) return -1;
     }
     {
         StageTimer tm(t, s, kStageFinish);
@@ -787,7 +793,7 @@ NAN_API int nan_tst_execute_learned(NanTracer *tracer, NanNrc *nrc, void *payloa
     hipStream_t s = as_stream(stream);
     if (reserve(tracer, train_samples, nq, s) || nrc_reserve(nrc, (uint32_t)nq, s) ||
         nrc_reserve_train(nrc, train_samples, s) ||
-        ((flags & NAN_LEARNED_OVERLAP) && reserve_overlap(tracer, train_samples, s)))
+        ((flags & NAN_LEARNED_OVERLAP) && reserve_overlap(tracer, nrc, train_samples, s)))
         return -1;
     unsigned long long *c = reinterpret_cast<unsigned long long *>(counters);
     // device-frame mode: iteration and reset come from the tracer's device
@@ -822,7 +828,7 @@ NAN_API int nan_tst_learned_reserve(NanTracer *tracer, NanNrc *nrc, uint32_t sam
         return fail(nullptr, 0, "nan_tst_learned_reserve: called during graph capture");
     if (reserve(tracer, train_samples, nq, nullptr) || nrc_reserve(nrc, (uint32_t)std::max<uint64_t>(nq, 1), nullptr) ||
         nrc_reserve_train(nrc, std::max(train_samples, 1u), nullptr) ||
-        reserve_overlap(tracer, std::max(train_samples, 1u), nullptr))
+        reserve_overlap(tracer, nrc, std::max(train_samples, 1u), nullptr))
         return -1;
     return 0;
 }

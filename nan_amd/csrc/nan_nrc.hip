@@ -573,6 +573,31 @@ __global__ void nrc_add_loss(const float *stats, float *loss_out) {
     if (threadIdx.x == 0) *loss_out += stats[0] * grad_scale(stats);
 }
 
+// Level l's entries in 32-bit words (one FP16 pair each) that a query can
+// read: the (res+1)^3 lattice of a densely indexed level, the whole table of
+// a hashed one.
+__host__ __device__ inline uint32_t snapshot_words(const NrcGrid &E, uint32_t l) {
+    uint32_t r1 = E.res[l] + 1u;
+    return E.dense[l] ? r1 * r1 * r1 : E.entries;
+}
+
+// The snapshot copy: blockIdx.y < levels copies that level's used entries,
+// blockIdx.y == levels the FP16 weights; snap holds the weights, then the
+// table in its own layout.
+__global__ void nrc_snapshot_copy(NrcGrid E, const uint32_t *w16t, uint32_t *snap) {
+    const uint32_t l = blockIdx.y;
+    const uint32_t *src = w16t;
+    uint32_t *dst = snap;
+    uint32_t count = kNrcWeights / 2;
+    if (l < E.levels) {
+        const size_t at = (size_t)l * E.entries;
+        src = reinterpret_cast<const uint32_t *>(E.table) + at;
+        dst = snap + kNrcWeights / 2 + at;
+        count = snapshot_words(E, l);
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -626,12 +651,12 @@ void encode(NanNrc *c, const float *pos, const float *nrm, const float *alb, uin
 void forward_backward(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                       const uint32_t *valid, uint32_t n, hipStream_t s) {
     uint32_t groups = groups_for(n), np = groups * 64;
-    encode(c, pos, nrm, alb, n, nullptr, s);
-    hipLaunchKernelGGL((nrc_mlp_fwd<true>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, c->w16t, c->enc,
-                       groups, n, c->pred, c->save, c->cap_train_groups, c->stats);
-    hipLaunchKernelGGL(nrc_loss, dim3(grid_for(np, kLossBlock)), dim3(kLossBlock), 0, s, c->pred, target, valid, n,
+    encode_to(c, pos, nrm, alb, n, c->tenc, nullptr, s);
+    hipLaunchKernelGGL((nrc_mlp_fwd<true>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, c->w16t, c->tenc,
+                       groups, n, c->tpred, c->save, c->cap_train_groups, c->stats);
+    hipLaunchKernelGGL(nrc_loss, dim3(grid_for(np, kLossBlock)), dim3(kLossBlock), 0, s, c->tpred, target, valid, n,
                        np, c->dy, c->stats);
-    hipLaunchKernelGGL(nrc_mlp_bwd, dim3(bwd_waves(groups)), dim3(64), 0, s, c->w16n, c->enc, c->save,
+    hipLaunchKernelGGL(nrc_mlp_bwd, dim3(bwd_waves(groups)), dim3(64), 0, s, c->w16n, c->tenc, c->save,
                        c->cap_train_groups, c->dy, groups, c->wpart, c->denc);
 }
 
@@ -639,8 +664,8 @@ void forward_backward(NanNrc *c, const float *pos, const float *nrm, const float
 
 NanNrc::~NanNrc() {
     for (void *p : {(void *)w32, (void *)wm, (void *)wv, (void *)wpart, (void *)w16t, (void *)w16n, (void *)ts,
-                    (void *)tg, (void *)t16, (void *)enc, (void *)save, (void *)dy,
-                    (void *)denc, (void *)pred, (void *)stats, (void *)lossp, adam})
+                    (void *)tg, (void *)t16, (void *)enc, (void *)save, (void *)dy, (void *)denc, (void *)pred,
+                    (void *)tenc, (void *)tpred, (void *)snap, (void *)stats, (void *)lossp, adam})
         hipFree(p);
 }
 
@@ -670,21 +695,23 @@ int nrc_reserve(NanNrc *c, uint32_t n, hipStream_t s) {
     return 0;
 }
 
-// Training scratch (saved activations, gradients) for n samples, plus the
-// prediction scratch.
+// Training scratch (encoded input, outputs, saved activations, gradients)
+// for n samples.
 int nrc_reserve_train(NanNrc *c, uint32_t n, hipStream_t s) {
-    if (nrc_reserve(c, n, s)) return -1;
     uint32_t groups = groups_for(n);
     if (groups <= c->cap_train_groups) return 0;
     if (stream_capturing(s))
         return fail(nullptr, 0, "nan_nrc: %u training records exceed the %u reserved; reserve before graph capture",
                     n, c->cap_train_groups * 64);
     HIP_TRY(hipDeviceSynchronize(), nullptr, 0);
-    for (void *p : {(void *)c->save, (void *)c->dy, (void *)c->denc}) hipFree(p);
-    c->save = c->dy = nullptr;
+    for (void *p : {(void *)c->save, (void *)c->dy, (void *)c->denc, (void *)c->tenc, (void *)c->tpred}) hipFree(p);
+    c->save = c->dy = c->tenc = nullptr;
     c->denc = nullptr;
+    c->tpred = nullptr;
     c->cap_train_groups = 0;
     uint32_t cap = std::max<uint32_t>(groups, 1024);
+    HIP_TRY(hipMalloc((void **)&c->tenc, (size_t)cap * kEncGroup * 2), nullptr, 0);
+    HIP_TRY(hipMalloc((void **)&c->tpred, (size_t)cap * 64 * sizeof(float4)), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&c->save, (size_t)(kNrcLayers - 1) * cap * kActGroup * 2), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&c->dy, (size_t)cap * kActGroup * 2), nullptr, 0);
     HIP_TRY(hipMalloc((void **)&c->denc, (size_t)cap * 64 * kNrcIn * 4), nullptr, 0);
@@ -693,14 +720,41 @@ int nrc_reserve_train(NanNrc *c, uint32_t n, hipStream_t s) {
 }
 
 int nrc_predict(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float4 *pred,
-                hipStream_t s) {
+                hipStream_t s, bool from_snapshot) {
     if (n == 0) return 0;
+    if (from_snapshot && !c->snap) return fail(nullptr, 0, "nan_nrc: no parameter snapshot reserved");
     if (nrc_reserve(c, n, s)) return -1;
     uint32_t groups = groups_for(n);
-    encode(c, pos, nrm, alb, n, nullptr, s);
-    hipLaunchKernelGGL((nrc_mlp_fwd<false>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, c->w16t, c->enc,
-                       groups, n, pred, (_Float16 *)nullptr, c->cap_groups, (float *)nullptr);
+    NrcGrid E = grid_of(c);
+    const _Float16 *wt = c->w16t;
+    if (from_snapshot) {
+        wt = c->snap;
+        E.table = c->snap + kNrcWeights;
+    }
+    uint32_t np = groups * 64;
+    dim3 grid(grid_for(np, 256), (c->cfg.levels + kEncLevels - 1) / kEncLevels + 1);
+    hipLaunchKernelGGL(nrc_encode_fwd, grid, dim3(256), 0, s, E, pos, nrm, alb, n, np, c->enc, (float *)nullptr);
+    hipLaunchKernelGGL((nrc_mlp_fwd<false>), dim3(fwd_blocks(groups)), dim3(kFwdBlock), 0, s, wt, c->enc, groups, n,
+                       pred, (_Float16 *)nullptr, c->cap_groups, (float *)nullptr);
     return check_launch("nrc predict");
+}
+
+int nrc_reserve_snapshot(NanNrc *c, hipStream_t s) {
+    if (c->snap) return 0;
+    if (stream_capturing(s))
+        return fail(nullptr, 0, "nan_nrc: parameter snapshot not reserved; reserve before graph capture");
+    HIP_TRY(hipMalloc((void **)&c->snap, ((size_t)kNrcWeights + c->table_params) * 2), nullptr, 0);
+    return 0;
+}
+
+int nrc_snapshot(NanNrc *c, hipStream_t s) {
+    if (nrc_reserve_snapshot(c, s)) return -1;
+    uint32_t most = kNrcWeights / 2;
+    for (uint32_t l = 0; l < c->cfg.levels; ++l) most = std::max(most, snapshot_words(grid_of(c), l));
+    dim3 grid(std::min<uint32_t>(grid_for(most, 256), 64), c->cfg.levels + 1);
+    hipLaunchKernelGGL(nrc_snapshot_copy, grid, dim3(256), 0, s, grid_of(c), (const uint32_t *)c->w16t,
+                       (uint32_t *)c->snap);
+    return check_launch("nrc snapshot");
 }
 
 int nrc_infer_into(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, _Float16 *enc,

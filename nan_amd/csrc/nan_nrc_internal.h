@@ -146,6 +146,13 @@ struct NanNrc {
     _Float16 *dy = nullptr;    // [groups][4096] gradient wrt the current layer's output
     float *denc = nullptr;     // [groups * 64][48]
     float4 *pred = nullptr;    // [groups * 64]
+    // the training step's own encoded input and outputs (cap_train_groups), so
+    // that queries on another stream may run beside a step
+    _Float16 *tenc = nullptr;
+    float4 *tpred = nullptr;
+    // copy of what the queries read (FP16 weights [kNrcWeights], then the FP16
+    // table) for queries that run beside training (nrc_snapshot)
+    _Float16 *snap = nullptr;
     float *stats = nullptr;    // [0] loss sum, [1] normalising count, [2] valid records (floats), [3] spare
     float2 *lossp = nullptr;   // [kNrcBwdWaves] per-block loss sums of the fused step
     bool fused_train = true;   // two-launch step when every level is densely indexed
@@ -162,9 +169,15 @@ NrcGrid nrc_grid(const NanNrc *c);
 // `stream` is being captured into a graph: growing allocates).
 int nrc_reserve(NanNrc *c, uint32_t n, hipStream_t stream);
 int nrc_reserve_train(NanNrc *c, uint32_t n, hipStream_t stream);
-// Network prediction for n samples into pred[n] (RGB + one spare output).
+// Network prediction for n samples into pred[n] (RGB + one spare output),
+// from the live parameters or from the last nrc_snapshot.
 int nrc_predict(NanNrc *c, const float *pos, const float *nrm, const float *alb, uint32_t n, float4 *pred,
-                hipStream_t stream);
+                hipStream_t stream, bool from_snapshot = false);
+// Copy the parameters the queries read into c->snap (allocated by
+// nrc_reserve_snapshot, refused while `stream` is being captured), so that
+// queries can run while a training step on another stream changes them.
+int nrc_reserve_snapshot(NanNrc *c, hipStream_t stream);
+int nrc_snapshot(NanNrc *c, hipStream_t stream);
 // The cache query of nan_nrc_infer with its scratch passed in: encoded input
 // enc (enc_groups groups of 64 samples) and raw outputs pred (enc_groups * 64
 // float4), then the clamped RGB into out[n][3].  Allocates nothing.

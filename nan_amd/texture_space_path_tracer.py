@@ -145,12 +145,14 @@ class TextureSpacePathTracer:
         cache at the texel itself: the cache learns the indirect irradiance
         field and each texel adds only its sun and light-sampled emitter
         terms (no bounce ray; use a cache trained in this form only).
-        ``overlap`` (one-bounce form) trains the cache on the previous
-        call's path suffixes on a second stream while this call traces, so
-        the queries see the cache trained through the previous frame, as
-        online neural radiance caching does; before the first such call the
-        kept records are zeros (none valid), so its step leaves the
-        parameters unchanged.  No counterpart in the reference."""
+        ``overlap`` (one-bounce form) copies the cache's parameters at the
+        start of the call, then trains the cache on the previous call's path
+        suffixes on a second stream while this call traces and queries the
+        copy: the queries see the cache as the previous call's training left
+        it (trained on suffixes up to two frames old) and never wait for the
+        training.  Before the first such call the kept records are zeros
+        (none valid), so its step leaves the parameters unchanged.  No
+        counterpart in the reference."""
         as_payload_tensor(output, self.layout.total_payload_count)
         if counters is not None and (counters.dtype != torch.int64 or counters.numel() < _native.COUNTER_COUNT
                                      or not counters.is_cuda):

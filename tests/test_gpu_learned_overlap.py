@@ -1,8 +1,10 @@
 """The learned form with overlapped training (NAN_LEARNED_OVERLAP): a call
-trains the cache on the previous call's training records on a second stream
-while it traces, and keeps its own records for the next call.  Checked
-against the cache trained explicitly on those records, for convergence on
-the Cornell box, and under hipGraph replay against eager calls."""
+copies the cache's parameters for its queries, trains the cache on the
+previous call's training records on a second stream while it traces and
+queries the copy, and keeps its own records for the next call.  Checked
+against the cache trained explicitly on those records, against calls that
+do not train (what the queries read), for convergence on the Cornell box,
+and under hipGraph replay against eager calls."""
 
 from __future__ import annotations
 
@@ -77,6 +79,34 @@ def test_overlapped_call_trains_on_the_previous_calls_records(device, cornell):
     _same_updates((ta - t0).ravel(), (tb - t0).ravel())
     v = out.cpu().numpy()
     assert np.isfinite(v).all() and (v.view(np.uint32)[:, 3] == 2).all()
+
+
+def test_overlapped_queries_read_the_parameters_the_previous_call_left(device, cornell):
+    """The queries of an overlapped call read the parameters copied at the
+    call's start, before its own steps.  The first call's step sees no valid
+    record, so the first two calls both query the untrained cache: their
+    payload equals that of two calls that do not train, bit for bit, although
+    the second call's step did change the cache."""
+    from nan_amd.radiance_cache import RadianceCache
+    from nan_amd.texture_space_path_tracer import TextureSpacePathTracer
+
+    layout, sc, tracer, cfg = _setup(device, cornell)
+    n = 16384
+    caches = [RadianceCache.for_scene(device, sc, cfg) for _ in range(2)]
+    w0, _ = caches[0].get_params()
+    enc = device.create_command_encoder()
+    out = torch.zeros((layout.total_payload_count, 4), dtype=torch.float32, device=device.torch_device)
+    twin = torch.zeros_like(out)
+    twin_tracer = TextureSpacePathTracer(device, sc, layout, samples_per_texel=1, max_bounces=3, mode="megakernel")
+    _say("two overlapped calls vs two calls without training")
+    for it in range(2):
+        tracer.execute_learned(enc, out, it, caches[0], reset=(it == 0), train_samples=n, overlap=True)
+        twin_tracer.execute_learned(enc, twin, it, caches[1], reset=(it == 0), train_samples=0)
+    w2, _ = caches[0].get_params()
+    assert np.abs(w2 - w0).max() > 1e-3
+    a, b = out.cpu().numpy(), twin.cpu().numpy()
+    assert (a.view(np.uint32)[:, 3] == 2).all()
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_overlapped_training_converges_on_the_cornell_box(device, cornell):
