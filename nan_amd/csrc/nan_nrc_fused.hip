@@ -86,27 +86,31 @@ struct UpdArgs {
 
 // acc[t] = W^T x[t] for the four tiles (B/D fragment layout: lane%16 = record
 // of the tile, 4*(lane/16)+i = unit); each A fragment of W ([out][in],
-// global) feeds four MFMAs.
+// global) feeds four MFMAs.  The layer's fragments are loaded together
+// before the products: one round trip to L2 or memory per layer (the update
+// kernel has just rewritten the weights, so they are not in this XCD's L2).
 template <int IN, int OUT>
 __device__ __forceinline__ void fwd4(const _Float16 *w, const half4_t (&x)[4][IN / 16],
                                      floatx4_t (&acc)[4][OUT / 16]) {
+    constexpr int KB = IN / 16, KO = OUT / 16;
     const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
+    half4_t a[KO][KB];
 #pragma unroll
-    for (int ot = 0; ot < OUT / 16; ++ot) {
+    for (int ot = 0; ot < KO; ++ot)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) a[ot][kb] = *(const half4_t *)&w[(ot * 16 + r) * IN + kb * 16 + 4 * q];
+#pragma unroll
+    for (int ot = 0; ot < KO; ++ot) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t][ot] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-        const _Float16 *row = w + (ot * 16 + r) * IN + 4 * q;
 #pragma unroll
-        for (int kb = 0; kb + 1 < IN / 16; kb += 2) {
-            const half4_t a0 = *(const half4_t *)&row[kb * 16], a1 = *(const half4_t *)&row[kb * 16 + 16];
+        for (int kb = 0; kb + 1 < KB; kb += 2)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][ot] = mfma32(a0, a1, x[t][kb], x[t][kb + 1], acc[t][ot]);
-        }
-        if ((IN / 16) % 2) {
-            const half4_t a = *(const half4_t *)&row[(IN / 16 - 1) * 16];
+            for (int t = 0; t < 4; ++t)
+                acc[t][ot] = mfma32(a[ot][kb], a[ot][kb + 1], x[t][kb], x[t][kb + 1], acc[t][ot]);
+        if (KB % 2)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][ot] = mfma16(a, x[t][IN / 16 - 1], acc[t][ot]);
-        }
+            for (int t = 0; t < 4; ++t) acc[t][ot] = mfma16(a[ot][KB - 1], x[t][KB - 1], acc[t][ot]);
     }
 }
 
@@ -161,35 +165,45 @@ __device__ __forceinline__ void bwd_dw(const half4_t (&xin)[4][nrc_layer_in(L) /
     __syncthreads();  // the next layer rewrites the transposes
 }
 
-// W dY^T for input block it of layer L over the four tiles.
-template <int L>
-__device__ __forceinline__ void bwd_dx(const _Float16 *w16n, const half4_t (&dy)[4][nrc_layer_out(L) / 16], int it,
-                                       floatx4_t (&dx)[4]) {
+// The A fragments of W dY^T for input blocks [0, NI) of layer L (w16n is
+// [in][out]), loaded together.
+template <int L, int NI>
+__device__ __forceinline__ void load_wn(const _Float16 *w16n, half4_t (&a)[NI][nrc_layer_out(L) / 16]) {
     constexpr int OUT = nrc_layer_out(L), KO = OUT / 16;
     const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
-    const _Float16 *wn = w16n + nrc_layer_offset(L);  // [in][out]
+    const _Float16 *wn = w16n + nrc_layer_offset(L);
+#pragma unroll
+    for (int it = 0; it < NI; ++it)
+#pragma unroll
+        for (int ko = 0; ko < KO; ++ko) a[it][ko] = *(const half4_t *)&wn[(it * 16 + r) * OUT + ko * 16 + 4 * q];
+}
+
+// W dY^T for input block it of layer L over the four tiles.
+template <int KO>
+__device__ __forceinline__ void bwd_dx(const half4_t (&a)[KO], const half4_t (&dy)[4][KO], floatx4_t (&dx)[4]) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) dx[t] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int ko = 0; ko < KO; ++ko) {
-        const half4_t a = *(const half4_t *)&wn[(it * 16 + r) * OUT + ko * 16 + 4 * q];
+    for (int ko = 0; ko < KO; ++ko)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) dx[t] = mfma16(a, dy[t][ko], dx[t]);
-    }
+        for (int t = 0; t < 4; ++t) dx[t] = mfma16(a[ko], dy[t][ko], dx[t]);
 }
 
 // Backward of layer L >= 1: its weight gradient, and the dY of the layer
 // below: W dY^T where the layer's input (a ReLU output) is positive, rounded
-// to FP16.
+// to FP16.  The weights' loads are in flight during the weight gradient.
 template <int L>
 __device__ __forceinline__ void bwd4(const _Float16 *w16n, const half4_t (&xin)[4][4],
                                      const half4_t (&dy)[4][nrc_layer_out(L) / 16], _Float16 *tx, _Float16 *ty,
                                      float *part, half4_t (&dyn)[4][4]) {
+    constexpr int KO = nrc_layer_out(L) / 16;
+    half4_t a[4][KO];
+    load_wn<L, 4>(w16n, a);
     bwd_dw<L>(xin, dy, tx, ty, part);
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
         floatx4_t dx[4];
-        bwd_dx<L>(w16n, dy, it, dx);
+        bwd_dx<KO>(a[it], dy, dx);
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -232,31 +246,49 @@ __global__ __launch_bounds__(64) void nrc_fused_fb(FbArgs A) {
     const half4_t zero4 = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
     {
         const uint32_t g = blockIdx.x;  // one 64-record group per wave
-        {  // encoding: lane = record
+        {  // encoding: lane = record.  Every level is densely indexed here; the
+           // 32 gathers of four levels are issued together (lanes past n read
+           // the cell at the box corner and store zeros).
             const uint32_t s = g * 64u + (uint32_t)lane;
+            const bool live = s < A.n;
             _Float16 *row = x0s + lane * kX0Stride;
             float p[3] = {0.0f, 0.0f, 0.0f};
-            if (s < A.n)
+            if (live)
                 for (int a = 0; a < 3; ++a) p[a] = A.pos[3 * (size_t)s + a];
-#pragma unroll 4
-            for (uint32_t l = 0; l < (uint32_t)kNrcMaxLevels; ++l) {
-                float f0 = 0.0f, f1 = 0.0f;
-                if (s < A.n && l < A.E.levels) {
+            const uint32_t levels = A.E.levels;
+            for (uint32_t l0 = 0; l0 < levels; l0 += 4) {
+                half2_t v[4][8];
+                float w[4][8];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t l = min(l0 + (uint32_t)j, levels - 1u);
                     uint32_t base[3];
                     float f[3];
                     level_cell(A.E, l, p, base, f);
                     const half2_t *tab = reinterpret_cast<const half2_t *>(A.E.table) + (size_t)l * A.E.entries;
-                    for (uint32_t k = 0; k < 8; ++k) {
-                        uint32_t idx;
-                        float w;
-                        cell_corner(A.E, l, base, f, k, idx, w);
-                        const half2_t v = tab[idx];
-                        f0 += w * (float)v[0];
-                        f1 += w * (float)v[1];
+                    const uint32_t r1 = A.E.res[l] + 1u;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+                        v[j][k] = tab[(base[0] + dx) + r1 * ((base[1] + dy) + r1 * (base[2] + dz))];
+                        w[j][k] = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
                     }
                 }
-                *reinterpret_cast<half2_t *>(&row[2 * l]) = half2_t{(_Float16)f0, (_Float16)f1};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float f0 = 0.0f, f1 = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        f0 += w[j][k] * (float)v[j][k][0];
+                        f1 += w[j][k] * (float)v[j][k][1];
+                    }
+                    if (l0 + (uint32_t)j < levels)
+                        *reinterpret_cast<half2_t *>(&row[2 * (l0 + j)]) =
+                            live ? half2_t{(_Float16)f0, (_Float16)f1} : half2_t{(_Float16)0.0f, (_Float16)0.0f};
+                }
             }
+            for (uint32_t l = levels; l < (uint32_t)kNrcMaxLevels; ++l)
+                *reinterpret_cast<half2_t *>(&row[2 * l]) = half2_t{(_Float16)0.0f, (_Float16)0.0f};
             for (int a = 0; a < 3; ++a) {
                 row[32 + a] = (_Float16)(s < A.n ? A.nrm[3 * (size_t)s + a] : 0.0f);
                 row[35 + a] = (_Float16)(s < A.n ? A.alb[3 * (size_t)s + a] : 0.0f);
@@ -316,11 +348,13 @@ __global__ __launch_bounds__(64) void nrc_fused_fb(FbArgs A) {
             bwd4<2>(A.w16n, xin, dyb, tx, ty, part, dy);
             load_act(sv[0], xin);
             bwd4<1>(A.w16n, xin, dy, tx, ty, part, dyb);
+            half4_t a0[FULL ? 3 : 2][4];
+            load_wn<0, FULL ? 3 : 2>(A.w16n, a0);
             bwd_dw<0>(x0, dyb, tx, ty, part);
 #pragma unroll
             for (int it = 0; it < (FULL ? 3 : 2); ++it) {
                 floatx4_t dx[4];
-                bwd_dx<0>(A.w16n, dyb, it, dx);
+                bwd_dx<4>(a0[it], dyb, dx);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) dh[t][it] = dx[t];
             }
@@ -402,15 +436,20 @@ __global__ __launch_bounds__(kUpdBlock) void nrc_fused_update(UpdArgs U) {
     if (blockIdx.y != 0) return;
     const float scale = 1.0f / (3.0f * fmaxf((float)U.n, 1.0f));
     const float c1 = U.as->c1, c2 = U.as->c2;
-    if (blockIdx.x == 0 && tid == 0) {
-        float l = 0.0f, v = 0.0f;  // the step's loss statistics (fixed order)
-        for (uint32_t p = 0; p < U.parts; ++p) {
-            l += U.lossp[p].x;
-            v += U.lossp[p].y;
+    if (blockIdx.x == 0) {  // the step's loss statistics: a fixed-order tree over the per-wave sums
+        __shared__ float2 lred[kNrcBwdWaves];
+        for (uint32_t p = tid; p < (uint32_t)kNrcBwdWaves; p += kUpdBlock)
+            lred[p] = p < U.parts ? U.lossp[p] : make_float2(0.0f, 0.0f);
+        __syncthreads();
+        for (uint32_t w = kNrcBwdWaves / 2; w > 0; w >>= 1) {
+            if (tid < w) lred[tid] = make_float2(lred[tid].x + lred[tid + w].x, lred[tid].y + lred[tid + w].y);
+            __syncthreads();
         }
-        U.stats[0] = l;
-        U.stats[1] = (float)U.n;
-        U.stats[2] = v;
+        if (tid == 0) {
+            U.stats[0] = lred[0].x;
+            U.stats[1] = (float)U.n;
+            U.stats[2] = lred[0].y;
+        }
     }
     // partial element f (fragment order): thread (sub, fl) sums partials sub,
     // sub + 16, ... four loads at a time; the sixteen sums combine in a fixed
