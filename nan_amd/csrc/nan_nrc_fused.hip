@@ -2,13 +2,14 @@
 // levels are all densely indexed ((res+1)^3 entries fit the table: the grids
 // of small scenes).  Three launches per step instead of seven:
 //
-//   nrc_fused_fb          one wave per 64-record group (four 16-record
-//                         tiles): hash-grid encoding, the 48 -> 64x4 -> 16
+//   nrc_fused_fb          four waves per 64-record group (four 16-record
+//                         tiles), each owning one 16-unit block of every
+//                         layer: hash-grid encoding, the 48 -> 64x4 -> 16
 //                         ReLU MLP forward (the layers' inputs kept in LDS for
 //                         the backward), the relative L2 loss, and the
-//                         backward.  The wave's weight gradients over its 64
+//                         backward.  The group's weight gradients over its 64
 //                         records go to its own partial in MFMA fragment order
-//                         (no cross-wave reduction); each record's gradient
+//                         (no cross-group reduction); each record's gradient
 //                         with respect to its hash features goes to a
 //                         per-level array.
 //   nrc_fused_update      the weights' Adam on the fixed-order sum of the
@@ -84,307 +85,310 @@ struct UpdArgs {
     uint32_t dense_adam, mlp_blocks;
 };
 
-// acc[t] = W^T x[t] for the four tiles (B/D fragment layout: lane%16 = record
-// of the tile, 4*(lane/16)+i = unit); each A fragment of W ([out][in],
-// global) feeds four MFMAs.  The layer's fragments are loaded together
-// before the products: one round trip to L2 or memory per layer (the update
-// kernel has just rewritten the weights, so they are not in this XCD's L2).
-template <int IN, int OUT>
-__device__ __forceinline__ void fwd4(const _Float16 *w, const half4_t (&x)[4][IN / 16],
-                                     floatx4_t (&acc)[4][OUT / 16]) {
-    constexpr int KB = IN / 16, KO = OUT / 16;
-    const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
-    half4_t a[KO][KB];
-#pragma unroll
-    for (int ot = 0; ot < KO; ++ot)
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb) a[ot][kb] = *(const half4_t *)&w[(ot * 16 + r) * IN + kb * 16 + 4 * q];
-#pragma unroll
-    for (int ot = 0; ot < KO; ++ot) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t][ot] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int kb = 0; kb + 1 < KB; kb += 2)
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                acc[t][ot] = mfma32(a[ot][kb], a[ot][kb + 1], x[t][kb], x[t][kb + 1], acc[t][ot]);
-        if (KB % 2)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][ot] = mfma16(a[ot][KB - 1], x[t][KB - 1], acc[t][ot]);
-    }
+constexpr int kFbWaves = 4;  // waves per 64-record group of the forward-backward kernel
+
+// One K=32 product as two K=16 MFMAs (the same map of fragments to K as
+// mfma32).  In this kernel the compiler reused a K=32 MFMA's B registers for
+// the next tile's fragments a few instructions after issuing it, and the
+// products of tiles 1..3 came out wrong while tile 0's were exact; the K=16
+// form reads its operands at issue.
+__device__ __forceinline__ floatx4_t mfma2x16(half4_t a0, half4_t a1, half4_t b0, half4_t b1, floatx4_t c) {
+    return mfma16(a1, b1, mfma16(a0, b0, c));
 }
 
 // ReLU after the rounding to FP16 (rounding is monotone and keeps zero).
-template <int KB>
-__device__ __forceinline__ void relu4(const floatx4_t (&acc)[4][KB], half4_t (&x)[4][KB]) {
+__device__ __forceinline__ half4_t relu_h(const floatx4_t &acc) {
     const half4_t zero = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-            x[t][kb] = __builtin_elementwise_max(__builtin_convertvector(acc[t][kb], half4_t), zero);
+    return __builtin_elementwise_max(__builtin_convertvector(acc, half4_t), zero);
 }
 
-// Weight gradient of layer L over the wave's 64 records: dW^T = dY^T X with
-// the records as the MFMA reduction dimension (both operands transposed
-// through LDS), one 16x16 block at a time, stored to the wave's partial in
-// fragment order (see frag_weight).
-template <int L>
-__device__ __forceinline__ void bwd_dw(const half4_t (&xin)[4][nrc_layer_in(L) / 16],
-                                       const half4_t (&dy)[4][nrc_layer_out(L) / 16], _Float16 *tx, _Float16 *ty,
-                                       float *part) {
-    constexpr int IN = nrc_layer_in(L), OUT = nrc_layer_out(L), KI = IN / 16, KO = OUT / 16;
-    const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
+// dY of the layer below from W dY^T: kept where the layer's input (a ReLU
+// output) is positive, rounded to FP16.
+__device__ __forceinline__ half4_t relu_mask(const half4_t &xin, const floatx4_t &dx) {
+    half4_t o;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-#pragma unroll
-        for (int kb = 0; kb < KO; ++kb)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) ty[(kb * 16 + 4 * q + i) * kTS + t * 16 + r] = dy[t][kb][i];
-#pragma unroll
-        for (int kb = 0; kb < KI; ++kb)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) tx[(kb * 16 + 4 * q + i) * kTS + t * 16 + r] = xin[t][kb][i];
-    }
-    __syncthreads();  // one wave per block: orders the transposes' writes and reads
-    float4 *p = reinterpret_cast<float4 *>(part + nrc_layer_offset(L)) + lane;
-#pragma unroll
-    for (int ot = 0; ot < KO; ++ot) {
-        half4_t a[4];
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) a[ks] = *(const half4_t *)&ty[(ot * 16 + r) * kTS + ks * 16 + 4 * q];
-#pragma unroll
-        for (int it = 0; it < KI; ++it) {
-            floatx4_t d = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                d = mfma16(a[ks], *(const half4_t *)&tx[(it * 16 + r) * kTS + ks * 16 + 4 * q], d);
-            p[(ot * KI + it) * 64] = make_float4(d[0], d[1], d[2], d[3]);
-        }
-    }
-    __syncthreads();  // the next layer rewrites the transposes
+    for (int i = 0; i < 4; ++i) o[i] = (_Float16)((float)xin[i] > 0.0f ? dx[i] : 0.0f);
+    return o;
 }
 
-// The A fragments of W dY^T for input blocks [0, NI) of layer L (w16n is
-// [in][out]), loaded together.
-template <int L, int NI>
-__device__ __forceinline__ void load_wn(const _Float16 *w16n, half4_t (&a)[NI][nrc_layer_out(L) / 16]) {
-    constexpr int OUT = nrc_layer_out(L), KO = OUT / 16;
-    const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
-    const _Float16 *wn = w16n + nrc_layer_offset(L);
+// One 16x16 block of a weight gradient over the group's 64 records, dW^T =
+// dY^T X with the records as the MFMA reduction dimension: rows ot of the
+// transposed dY (ty) against rows it of the transposed X (tx).
+__device__ __forceinline__ floatx4_t dw_block(const _Float16 *ty, const _Float16 *tx, int ot, int it, int r, int q) {
+    floatx4_t d = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int it = 0; it < NI; ++it)
-#pragma unroll
-        for (int ko = 0; ko < KO; ++ko) a[it][ko] = *(const half4_t *)&wn[(it * 16 + r) * OUT + ko * 16 + 4 * q];
+    for (int ks = 0; ks < 4; ++ks)
+        d = mfma16(*(const half4_t *)&ty[(ot * 16 + r) * kTS + ks * 16 + 4 * q],
+                   *(const half4_t *)&tx[(it * 16 + r) * kTS + ks * 16 + 4 * q], d);
+    return d;
 }
 
-// W dY^T for input block it of layer L over the four tiles.
-template <int KO>
-__device__ __forceinline__ void bwd_dx(const half4_t (&a)[KO], const half4_t (&dy)[4][KO], floatx4_t (&dx)[4]) {
+// The 16 rows (units 16b..16b+15) of a transposed operand, written from the
+// four tiles' fragments of block b (lane%16 = record of the tile, 4*(lane/16)+i
+// = unit).
+__device__ __forceinline__ void put_t(_Float16 *t, int b, const half4_t (&v)[4], int r, int q) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) dx[t] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-    for (int ko = 0; ko < KO; ++ko)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) dx[t] = mfma16(a[ko], dy[t][ko], dx[t]);
-}
-
-// Backward of layer L >= 1: its weight gradient, and the dY of the layer
-// below: W dY^T where the layer's input (a ReLU output) is positive, rounded
-// to FP16.  The weights' loads are in flight during the weight gradient.
-template <int L>
-__device__ __forceinline__ void bwd4(const _Float16 *w16n, const half4_t (&xin)[4][4],
-                                     const half4_t (&dy)[4][nrc_layer_out(L) / 16], _Float16 *tx, _Float16 *ty,
-                                     float *part, half4_t (&dyn)[4][4]) {
-    constexpr int KO = nrc_layer_out(L) / 16;
-    half4_t a[4][KO];
-    load_wn<L, 4>(w16n, a);
-    bwd_dw<L>(xin, dy, tx, ty, part);
-#pragma unroll
-    for (int it = 0; it < 4; ++it) {
-        floatx4_t dx[4];
-        bwd_dx<KO>(a[it], dy, dx);
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                dyn[t][it][i] = (_Float16)((float)xin[t][it][i] > 0.0f ? dx[t][i] : 0.0f);
-    }
-}
-
-__device__ __forceinline__ void save_act(half4_t (&sv)[4][4][64], const half4_t (&x)[4][4]) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) sv[t][kb][lane] = x[t][kb];
-}
-
-__device__ __forceinline__ void load_act(const half4_t (&sv)[4][4][64], half4_t (&x)[4][4]) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) x[t][kb] = sv[t][kb][lane];
+        for (int i = 0; i < 4; ++i) t[(b * 16 + 4 * q + i) * kTS + tt * 16 + r] = v[tt][i];
 }
 
 // FULL: also the gradient with respect to all 48 inputs (a test hook).
+//
+// Four waves share a 64-record group, each owning one 16-unit block: in the
+// forward, wave w computes units 16w..16w+15 of every hidden layer for all
+// four 16-record tiles and the output layer for tile w; the layers'
+// activations are exchanged through LDS (the same copy the backward reads).
+// In the backward, wave w computes the weight-gradient rows of output block w
+// and the input gradient of input block w; the dY blocks are exchanged
+// through LDS in fragment order, and transposed through LDS for the weight
+// gradients.  A single wave per group spent most of its time issuing one
+// serial instruction stream while three of the CU's four SIMDs idled.
 template <bool FULL>
-__global__ __launch_bounds__(64) void nrc_fused_fb(FbArgs A) {
-    __shared__ __align__(16) _Float16 x0s[64 * kX0Stride];
+__global__ __launch_bounds__(64 * kFbWaves) void nrc_fused_fb(FbArgs A) {
+    // u: the encoded input rows (record-major) until every wave holds its
+    // fragments, then the backward's dY exchange [tile][block][lane]
+    __shared__ __align__(16) half4_t u[4][4][64];
     __shared__ __align__(16) _Float16 tx[64 * kTS], ty[64 * kTS];
     __shared__ half4_t sv[kNrcHidden][4][4][64];  // inputs of layers 1..4, fragment order
-    const int lane = __lane_id(), r = lane & 15, q = lane >> 4;
-    if (A.as && blockIdx.x == 0 && lane == 0) {  // this step's bias corrections
+    __shared__ float2 lred[kFbWaves];
+    static_assert(64 * kX0Stride * sizeof(_Float16) <= sizeof(u), "encoded rows must fit the exchange buffer");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
+    _Float16 *x0s = reinterpret_cast<_Float16 *>(&u[0][0][0]);
+    const uint32_t g = blockIdx.x;  // one 64-record group per block
+    if (A.as && g == 0 && threadIdx.x == 0) {  // this step's bias corrections
         const uint32_t step = A.as->step + 1u;
         A.as->next_step = step;
         A.as->c1 = bias_correction(A.b1, step);
         A.as->c2 = bias_correction(A.b2, step);
     }
-    float *part = A.partial + (size_t)blockIdx.x * kNrcWeights;
-    float loss = 0.0f, nvalid = 0.0f;
-    const half4_t zero4 = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+    float *part = A.partial + (size_t)g * kNrcWeights;
+    constexpr int NI0 = FULL ? 3 : 2;  // input blocks of layer 0 whose gradient is needed
+    // This wave's weight fragments, requested before the encoding's gathers:
+    // forward, output block w of each hidden layer ([out][in]) and all of the
+    // output layer; backward, input block w of each layer ([in][out]).
+    half4_t wf0[3], wf1[4], wf2[4], wf3[4], wf4[4], wb1[4], wb2[4], wb3[4], wb0[4], wb4;
     {
-        const uint32_t g = blockIdx.x;  // one 64-record group per wave
-        {  // encoding: lane = record.  Every level is densely indexed here; the
-           // 32 gathers of four levels are issued together (lanes past n read
-           // the cell at the box corner and store zeros).
-            const uint32_t s = g * 64u + (uint32_t)lane;
-            const bool live = s < A.n;
-            _Float16 *row = x0s + lane * kX0Stride;
-            float p[3] = {0.0f, 0.0f, 0.0f};
-            if (live)
-                for (int a = 0; a < 3; ++a) p[a] = A.pos[3 * (size_t)s + a];
-            const uint32_t levels = A.E.levels;
-            for (uint32_t l0 = 0; l0 < levels; l0 += 4) {
-                half2_t v[4][8];
-                float w[4][8];
+        const _Float16 *wt = A.w16t, *wn = A.w16n;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t l = min(l0 + (uint32_t)j, levels - 1u);
-                    uint32_t base[3];
-                    float f[3];
-                    level_cell(A.E, l, p, base, f);
-                    const half2_t *tab = reinterpret_cast<const half2_t *>(A.E.table) + (size_t)l * A.E.entries;
-                    const uint32_t r1 = A.E.res[l] + 1u;
+        for (int kb = 0; kb < 3; ++kb) wf0[kb] = *(const half4_t *)&wt[(w * 16 + r) * kNrcIn + kb * 16 + 4 * q];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
-                        v[j][k] = tab[(base[0] + dx) + r1 * ((base[1] + dy) + r1 * (base[2] + dz))];
-                        w[j][k] = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
-                    }
+        for (int kb = 0; kb < 4; ++kb) {
+            wf1[kb] = *(const half4_t *)&wt[nrc_layer_offset(1) + (w * 16 + r) * kNrcWidth + kb * 16 + 4 * q];
+            wf2[kb] = *(const half4_t *)&wt[nrc_layer_offset(2) + (w * 16 + r) * kNrcWidth + kb * 16 + 4 * q];
+            wf3[kb] = *(const half4_t *)&wt[nrc_layer_offset(3) + (w * 16 + r) * kNrcWidth + kb * 16 + 4 * q];
+            wf4[kb] = *(const half4_t *)&wt[nrc_layer_offset(4) + r * kNrcWidth + kb * 16 + 4 * q];
+            wb1[kb] = *(const half4_t *)&wn[nrc_layer_offset(1) + (w * 16 + r) * kNrcWidth + kb * 16 + 4 * q];
+            wb2[kb] = *(const half4_t *)&wn[nrc_layer_offset(2) + (w * 16 + r) * kNrcWidth + kb * 16 + 4 * q];
+            wb3[kb] = *(const half4_t *)&wn[nrc_layer_offset(3) + (w * 16 + r) * kNrcWidth + kb * 16 + 4 * q];
+            // layer 0 has 48 inputs: blocks 0..2 only
+            wb0[kb] = w < NI0 ? *(const half4_t *)&wn[(w * 16 + r) * kNrcWidth + kb * 16 + 4 * q] : half4_t{};
+        }
+        wb4 = *(const half4_t *)&wn[nrc_layer_offset(4) + (w * 16 + r) * kNrcOut + 4 * q];
+    }
+    // the targets of tile w, read by the loss
+    const uint32_t sw = g * 64u + (uint32_t)w * 16u + (uint32_t)r;
+    const bool counted = q == 0 && sw < A.n && (!A.valid || A.valid[sw]);
+    float tgt[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tgt[c] = counted ? A.target[3 * (size_t)sw + c] : 0.0f;
+    {  // encoding: lane = record, wave w the levels w, w + 4, ...  Every level
+       // is densely indexed here; lanes past n read the cell at the box corner
+       // and store zeros.
+        const uint32_t s = g * 64u + (uint32_t)lane;
+        const bool live = s < A.n;
+        _Float16 *row = x0s + lane * kX0Stride;
+        float p[3] = {0.0f, 0.0f, 0.0f};
+        if (live)
+            for (int a = 0; a < 3; ++a) p[a] = A.pos[3 * (size_t)s + a];
+        const uint32_t levels = A.E.levels;
+        for (uint32_t l = (uint32_t)w; l < (uint32_t)kNrcMaxLevels; l += kFbWaves) {
+            half2_t val = {(_Float16)0.0f, (_Float16)0.0f};
+            if (l < levels) {
+                uint32_t base[3];
+                float f[3];
+                level_cell(A.E, l, p, base, f);
+                const half2_t *tab = reinterpret_cast<const half2_t *>(A.E.table) + (size_t)l * A.E.entries;
+                const uint32_t r1 = A.E.res[l] + 1u;
+                half2_t v[8];
+                float wt[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+                    v[k] = tab[(base[0] + dx) + r1 * ((base[1] + dy) + r1 * (base[2] + dz))];
+                    wt[k] = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
                 }
+                float f0 = 0.0f, f1 = 0.0f;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    float f0 = 0.0f, f1 = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        f0 += w[j][k] * (float)v[j][k][0];
-                        f1 += w[j][k] * (float)v[j][k][1];
-                    }
-                    if (l0 + (uint32_t)j < levels)
-                        *reinterpret_cast<half2_t *>(&row[2 * (l0 + j)]) =
-                            live ? half2_t{(_Float16)f0, (_Float16)f1} : half2_t{(_Float16)0.0f, (_Float16)0.0f};
+                for (int k = 0; k < 8; ++k) {
+                    f0 += wt[k] * (float)v[k][0];
+                    f1 += wt[k] * (float)v[k][1];
                 }
+                if (live) val = half2_t{(_Float16)f0, (_Float16)f1};
             }
-            for (uint32_t l = levels; l < (uint32_t)kNrcMaxLevels; ++l)
-                *reinterpret_cast<half2_t *>(&row[2 * l]) = half2_t{(_Float16)0.0f, (_Float16)0.0f};
+            *reinterpret_cast<half2_t *>(&row[2 * l]) = val;
+        }
+        if (w == kFbWaves - 1) {
             for (int a = 0; a < 3; ++a) {
-                row[32 + a] = (_Float16)(s < A.n ? A.nrm[3 * (size_t)s + a] : 0.0f);
-                row[35 + a] = (_Float16)(s < A.n ? A.alb[3 * (size_t)s + a] : 0.0f);
+                row[32 + a] = (_Float16)(live ? A.nrm[3 * (size_t)s + a] : 0.0f);
+                row[35 + a] = (_Float16)(live ? A.alb[3 * (size_t)s + a] : 0.0f);
             }
             for (int k = 38; k < kNrcIn; ++k) row[k] = (_Float16)0.0f;
         }
-        __syncthreads();
-        half4_t x0[4][3];
+    }
+    __syncthreads();
+    half4_t x0[4][3];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int kb = 0; kb < 3; ++kb)
-                x0[t][kb] = *(const half4_t *)&x0s[(t * 16 + r) * kX0Stride + kb * 16 + 4 * q];
-        // forward; the inputs of layers 1..4 are kept in LDS for the backward
-        half4_t dy4[4][1];
-        {
-            half4_t x[4][4];
-            floatx4_t acc[4][4];
-            fwd4<kNrcIn, kNrcWidth>(A.w16t + nrc_layer_offset(0), x0, acc);
-            relu4(acc, x);
-            save_act(sv[0], x);
+        for (int kb = 0; kb < 3; ++kb) x0[t][kb] = *(const half4_t *)&x0s[(t * 16 + r) * kX0Stride + kb * 16 + 4 * q];
+    // forward: wave w's block of each hidden layer for the four tiles
 #pragma unroll
-            for (int l = 1; l < kNrcHidden; ++l) {
-                fwd4<kNrcWidth, kNrcWidth>(A.w16t + nrc_layer_offset(l), x, acc);
-                relu4(acc, x);
-                save_act(sv[l], x);
-            }
-            floatx4_t out[4][1];
-            fwd4<kNrcWidth, kNrcOut>(A.w16t + nrc_layer_offset(kNrcLayers - 1), x, out);
-            // loss: lanes of quarter 0 hold outputs 0..3 of record r of each tile
+    for (int t = 0; t < 4; ++t) {
+        floatx4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mfma2x16(wf0[0], wf0[1], x0[t][0], x0[t][1], acc);
+        acc = mfma16(wf0[2], x0[t][2], acc);
+        sv[0][t][w][lane] = relu_h(acc);
+    }
+    __syncthreads();  // also: every wave holds its x0 fragments, so u is free
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                dy4[t][0] = zero4;
-                const uint32_t s = g * 64u + (uint32_t)t * 16u + (uint32_t)r;
-                if (q == 0 && s < A.n && (!A.valid || A.valid[s])) {
-                    float gr[3];
-                    for (int c = 0; c < 3; ++c) {
-                        const float pc = out[t][0][c], tc = A.target[3 * (size_t)s + c];
-                        const float d = pc * pc + 0.01f, diff = pc - tc;
-                        loss += diff * diff / d;
-                        gr[c] = fminf(fmaxf(2.0f * diff / d, -kGradClamp), kGradClamp);
-                    }
-                    dy4[t][0] = half4_t{(_Float16)gr[0], (_Float16)gr[1], (_Float16)gr[2], (_Float16)0.0f};
-                    nvalid += 1.0f;
-                }
-            }
+    for (int l = 1; l < kNrcHidden; ++l) {
+        const half4_t(&wl)[4] = l == 1 ? wf1 : (l == 2 ? wf2 : wf3);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            half4_t x[4];
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) x[kb] = sv[l - 1][t][kb][lane];
+            floatx4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma2x16(wl[0], wl[1], x[0], x[1], acc);
+            acc = mfma2x16(wl[2], wl[3], x[2], x[3], acc);
+            sv[l][t][w][lane] = relu_h(acc);
         }
-        // backward, layer by layer
-        floatx4_t dh[4][FULL ? 3 : 2];  // gradients of the 32 hash features (FULL: all 48 inputs)
-        {
-            half4_t dy[4][4], dyb[4][4], xin[4][4];
-            load_act(sv[3], xin);
-            bwd4<4>(A.w16n, xin, dy4, tx, ty, part, dy);
-            load_act(sv[2], xin);
-            bwd4<3>(A.w16n, xin, dy, tx, ty, part, dyb);
-            load_act(sv[1], xin);
-            bwd4<2>(A.w16n, xin, dyb, tx, ty, part, dy);
-            load_act(sv[0], xin);
-            bwd4<1>(A.w16n, xin, dy, tx, ty, part, dyb);
-            half4_t a0[FULL ? 3 : 2][4];
-            load_wn<0, FULL ? 3 : 2>(A.w16n, a0);
-            bwd_dw<0>(x0, dyb, tx, ty, part);
+        __syncthreads();
+    }
+    // output layer and loss for tile w: lanes of quarter 0 hold outputs 0..3
+    // of record r
+    float loss = 0.0f, nvalid = 0.0f;
+    half4_t dyw[4];  // this wave's dY block of the layer being differentiated, per tile
+    half4_t dy4 = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+    {
+        half4_t x[4];
 #pragma unroll
-            for (int it = 0; it < (FULL ? 3 : 2); ++it) {
-                floatx4_t dx[4];
-                bwd_dx<4>(a0[it], dyb, dx);
-#pragma unroll
-                for (int t = 0; t < 4; ++t) dh[t][it] = dx[t];
+        for (int kb = 0; kb < 4; ++kb) x[kb] = sv[3][w][kb][lane];
+        floatx4_t out = {0.0f, 0.0f, 0.0f, 0.0f};
+        out = mfma2x16(wf4[0], wf4[1], x[0], x[1], out);
+        out = mfma2x16(wf4[2], wf4[3], x[2], x[3], out);
+        if (counted) {
+            float gr[3];
+            for (int c = 0; c < 3; ++c) {
+                const float pc = out[c], tc = tgt[c];
+                const float d = pc * pc + 0.01f, diff = pc - tc;
+                loss += diff * diff / d;
+                gr[c] = fminf(fmaxf(2.0f * diff / d, -kGradClamp), kGradClamp);
             }
+            dy4 = half4_t{(_Float16)gr[0], (_Float16)gr[1], (_Float16)gr[2], (_Float16)0.0f};
+            nvalid = 1.0f;
+        }
+    }
+    // backward of the output layer (16 outputs): dY of tile w transposed into
+    // ty rows 0..15, block w of its input (sv[3]) transposed into tx, dY
+    // fragments exchanged through u
+    half4_t xin[4];
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ty[(4 * q + i) * kTS + w * 16 + r] = dy4[i];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) xin[t] = sv[3][t][w][lane];
+        put_t(tx, w, xin, r, q);
+        u[w][0][lane] = dy4;
+        __syncthreads();
+        float4 *p = reinterpret_cast<float4 *>(part + nrc_layer_offset(4)) + lane;
+        const floatx4_t d = dw_block(ty, tx, 0, w, r, q);
+        p[w * 64] = make_float4(d[0], d[1], d[2], d[3]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            floatx4_t dx = {0.0f, 0.0f, 0.0f, 0.0f};
+            dx = mfma16(wb4, u[t][0][lane], dx);
+            dyw[t] = relu_mask(xin[t], dx);
+        }
+        __syncthreads();
+    }
+    // hidden layers 3, 2, 1
+#pragma unroll
+    for (int l = kNrcHidden - 1; l >= 1; --l) {
+        const half4_t(&wl)[4] = l == 1 ? wb1 : (l == 2 ? wb2 : wb3);
+        put_t(ty, w, dyw, r, q);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            u[t][w][lane] = dyw[t];
+            xin[t] = sv[l - 1][t][w][lane];
+        }
+        put_t(tx, w, xin, r, q);
+        __syncthreads();
+        float4 *p = reinterpret_cast<float4 *>(part + nrc_layer_offset(l)) + lane;
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const floatx4_t d = dw_block(ty, tx, w, it, r, q);
+            p[(w * 4 + it) * 64] = make_float4(d[0], d[1], d[2], d[3]);
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const uint32_t s = g * 64u + (uint32_t)t * 16u + (uint32_t)r;
-            if (s >= A.n) continue;
-            if (FULL && A.input_grad) {
+            floatx4_t dx = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int it = 0; it < (FULL ? 3 : 2); ++it)
-                    *reinterpret_cast<float4 *>(&A.input_grad[(size_t)s * kNrcIn + it * 16 + 4 * q]) =
-                        make_float4(dh[t][it][0] * A.input_scale, dh[t][it][1] * A.input_scale,
-                                    dh[t][it][2] * A.input_scale, dh[t][it][3] * A.input_scale);
-            }
-            if (!A.gtab) continue;
+            for (int ko = 0; ko < 4; ++ko) dx = mfma16(wl[ko], u[t][ko][lane], dx);
+            dyw[t] = relu_mask(xin[t], dx);
+        }
+        __syncthreads();
+    }
+    // layer 0 (48 inputs): weight-gradient rows of output block w, and the
+    // gradient of input block w < NI0 (the hash features; FULL: all inputs)
+    put_t(ty, w, dyw, r, q);
 #pragma unroll
-            for (int it = 0; it < 2; ++it)
+    for (int t = 0; t < 4; ++t) u[t][w][lane] = dyw[t];
+    if (w < 3) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const uint32_t lv = (uint32_t)(it * 8 + 2 * q + h);
-                    if (lv < A.E.levels) A.gtab[(size_t)lv * A.cap + s] = make_float2(dh[t][it][2 * h], dh[t][it][2 * h + 1]);
-                }
+        for (int t = 0; t < 4; ++t) xin[t] = w == 0 ? x0[t][0] : (w == 1 ? x0[t][1] : x0[t][2]);
+        put_t(tx, w, xin, r, q);
+    }
+    __syncthreads();
+    {
+        float4 *p = reinterpret_cast<float4 *>(part) + lane;
+#pragma unroll
+        for (int it = 0; it < 3; ++it) {
+            const floatx4_t d = dw_block(ty, tx, w, it, r, q);
+            p[(w * 3 + it) * 64] = make_float4(d[0], d[1], d[2], d[3]);
         }
     }
+    if (w < NI0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            floatx4_t dx = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ko = 0; ko < 4; ++ko) dx = mfma16(wb0[ko], u[t][ko][lane], dx);
+            const uint32_t s = g * 64u + (uint32_t)t * 16u + (uint32_t)r;
+            if (s >= A.n) continue;
+            if (FULL && A.input_grad)
+                *reinterpret_cast<float4 *>(&A.input_grad[(size_t)s * kNrcIn + w * 16 + 4 * q]) =
+                    make_float4(dx[0] * A.input_scale, dx[1] * A.input_scale, dx[2] * A.input_scale,
+                                dx[3] * A.input_scale);
+            if (!A.gtab || w >= 2) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t lv = (uint32_t)(w * 8 + 2 * q + h);
+                if (lv < A.E.levels) A.gtab[(size_t)lv * A.cap + s] = make_float2(dx[2 * h], dx[2 * h + 1]);
+            }
+        }
+    }
+    // the group's loss: each wave's lanes, then the four waves in order
     for (int off = 32; off > 0; off >>= 1) {
         loss += __shfl_down(loss, off);
         nvalid += __shfl_down(nvalid, off);
     }
-    if (lane == 0) A.lossp[blockIdx.x] = make_float2(loss, nvalid);
+    if (lane == 0) lred[w] = make_float2(loss, nvalid);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        A.lossp[g] = make_float2(((lred[0].x + lred[1].x) + lred[2].x) + lred[3].x,
+                                 ((lred[0].y + lred[1].y) + lred[2].y) + lred[3].y);
 }
 
 // Weight (index into [layer][out][in] order) of element f of a partial in
@@ -452,13 +456,19 @@ __global__ __launch_bounds__(kUpdBlock) void nrc_fused_update(UpdArgs U) {
         }
     }
     // partial element f (fragment order): thread (sub, fl) sums partials sub,
-    // sub + 16, ... four loads at a time; the sixteen sums combine in a fixed
-    // order
+    // sub + 16, ... eight loads at a time, then four, then one; the sixteen
+    // sums combine in a fixed order
     const uint32_t fl = tid % kMlpPerBlock, sub = tid / kMlpPerBlock;
     const uint32_t f = blockIdx.x * kMlpPerBlock + fl;
     float g = 0.0f;
     if (f < (uint32_t)kNrcWeights) {
         uint32_t p = sub;
+        for (; p + 7 * kMlpSplit < U.parts; p += 8 * kMlpSplit) {  // eight loads in flight
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = U.partial[(size_t)(p + j * kMlpSplit) * kNrcWeights + f];
+            g += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        }
         for (; p + 3 * kMlpSplit < U.parts; p += 4 * kMlpSplit) {
             const float a = U.partial[(size_t)p * kNrcWeights + f];
             const float b = U.partial[(size_t)(p + kMlpSplit) * kNrcWeights + f];
@@ -569,7 +579,7 @@ int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *
     if (nrc_reserve_train(c, n, s)) return -1;
     FbArgs A = fb_args(c, pos, nrm, alb, target, valid, n);
     const uint32_t parts = A.groups;
-    hipLaunchKernelGGL(nrc_fused_fb<false>, dim3(parts), dim3(64), 0, s, A);
+    hipLaunchKernelGGL(nrc_fused_fb<false>, dim3(parts), dim3(64 * kFbWaves), 0, s, A);
     const NanNrcConfig &cf = c->cfg;
     UpdArgs U;
     U.E = A.E;
@@ -624,7 +634,7 @@ int nrc_fused_grads(NanNrc *c, const float *pos, const float *nrm, const float *
     A.as = nullptr;
     A.input_grad = input_grad;
     A.input_scale = scale;
-    hipLaunchKernelGGL(nrc_fused_fb<true>, dim3(parts), dim3(64), 0, s, A);
+    hipLaunchKernelGGL(nrc_fused_fb<true>, dim3(parts), dim3(64 * kFbWaves), 0, s, A);
     hipLaunchKernelGGL(nrc_fused_sum, dim3((kNrcWeights + 255) / 256), dim3(256), 0, s, c->wpart, parts, scale,
                        weight_grad);
     return check_launch("nrc fused gradients");
