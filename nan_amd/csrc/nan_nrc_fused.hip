@@ -87,15 +87,6 @@ struct UpdArgs {
 
 constexpr int kFbWaves = 4;  // waves per 64-record group of the forward-backward kernel
 
-// One K=32 product as two K=16 MFMAs (the same map of fragments to K as
-// mfma32).  In this kernel the compiler reused a K=32 MFMA's B registers for
-// the next tile's fragments a few instructions after issuing it, and the
-// products of tiles 1..3 came out wrong while tile 0's were exact; the K=16
-// form reads its operands at issue.
-__device__ __forceinline__ floatx4_t mfma2x16(half4_t a0, half4_t a1, half4_t b0, half4_t b1, floatx4_t c) {
-    return mfma16(a1, b1, mfma16(a0, b0, c));
-}
-
 // ReLU after the rounding to FP16 (rounding is monotone and keeps zero).
 __device__ __forceinline__ half4_t relu_h(const floatx4_t &acc) {
     const half4_t zero = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
@@ -246,7 +237,7 @@ __global__ __launch_bounds__(64 * kFbWaves) void nrc_fused_fb(FbArgs A) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         floatx4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = mfma2x16(wf0[0], wf0[1], x0[t][0], x0[t][1], acc);
+        acc = mfma32(wf0[0], wf0[1], x0[t][0], x0[t][1], acc);
         acc = mfma16(wf0[2], x0[t][2], acc);
         sv[0][t][w][lane] = relu_h(acc);
     }
@@ -260,8 +251,8 @@ __global__ __launch_bounds__(64 * kFbWaves) void nrc_fused_fb(FbArgs A) {
 #pragma unroll
             for (int kb = 0; kb < 4; ++kb) x[kb] = sv[l - 1][t][kb][lane];
             floatx4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma2x16(wl[0], wl[1], x[0], x[1], acc);
-            acc = mfma2x16(wl[2], wl[3], x[2], x[3], acc);
+            acc = mfma32(wl[0], wl[1], x[0], x[1], acc);
+            acc = mfma32(wl[2], wl[3], x[2], x[3], acc);
             sv[l][t][w][lane] = relu_h(acc);
         }
         __syncthreads();
@@ -276,8 +267,8 @@ __global__ __launch_bounds__(64 * kFbWaves) void nrc_fused_fb(FbArgs A) {
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) x[kb] = sv[3][w][kb][lane];
         floatx4_t out = {0.0f, 0.0f, 0.0f, 0.0f};
-        out = mfma2x16(wf4[0], wf4[1], x[0], x[1], out);
-        out = mfma2x16(wf4[2], wf4[3], x[2], x[3], out);
+        out = mfma32(wf4[0], wf4[1], x[0], x[1], out);
+        out = mfma32(wf4[2], wf4[3], x[2], x[3], out);
         if (counted) {
             float gr[3];
             for (int c = 0; c < 3; ++c) {

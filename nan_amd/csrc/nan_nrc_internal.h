@@ -44,17 +44,17 @@ typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
-// One K=32 MFMA over two K=16 fragments: hardware K index 8q+i of lane
-// quarter q stands for fragment 0's 4q+i (i < 4) and fragment 1's 4q+i-4,
-// the same map in both operands, so the product sums the same 32 terms.
-__device__ __forceinline__ floatx4_t mfma32(half4_t a0, half4_t a1, half4_t b0, half4_t b1, floatx4_t c) {
-    half8_t a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
-    half8_t b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
+// A K=32 product over two K=16 fragments, issued as two K=16 MFMAs.  The
+// single K=32 instruction (v_mfma_f32_16x16x32_f16) was measured wrong when
+// the compiler rewrote the upper half of its A or B registers a few
+// instructions after issue (the products of every tile but the first came
+// out wrong in a four-wave kernel; the K=16 form is exact there).
 __device__ __forceinline__ floatx4_t mfma16(half4_t a, half4_t b, floatx4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ floatx4_t mfma32(half4_t a0, half4_t a1, half4_t b0, half4_t b1, floatx4_t c) {
+    return mfma16(a1, b1, mfma16(a0, b0, c));
 }
 
 // Per-record gradient bound of the relative L2 loss (before the 1/3n scale).
