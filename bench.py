@@ -195,6 +195,11 @@ def tinybvh_baseline(node, threads: int, rays: int = 4_000_000) -> dict | None:
 
 
 MFMA_F16_PEAK_TFLOPS = 2500.0     # dense FP16 MFMA peak of MI355X
+# The network's products run on v_mfma_f32_16x16x16_f16 (512 FLOP per SIMD
+# cycle, about half the dense peak): the doubled-K form gave wrong products
+# where the compiler rewrote its operand registers shortly after issue.
+MFMA_INSTRUCTION = "v_mfma_f32_16x16x16_f16"
+MFMA_K16_PEAK_TFLOPS = 512 * 1024 * 2.4e9 / 1e12  # 1,024 SIMDs at 2.4 GHz
 MLP_FLOPS_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 16)   # executed (padded) forward FLOPs: 32,768
 
 
@@ -227,11 +232,11 @@ def cache_config(scene, layout, minibatches: int):
 
 
 # Adam steps per frame of the Cornell-box learned form, trained beside the
-# next frame's tracing.  Mean of three cache seeds against the plain tracer
-# at equal RMSE: 4 steps (311 us per frame) 0.44x at 256 frames, 0.15x at 64;
-# 8 steps (435 us) 0.33x and 0.25x; 4 steps without the overlap (387 us)
-# 0.33x and 0.14x; one run varies by about +-0.03 at 256 frames and +-0.1 at
-# 64.  More, smaller steps converge in fewer frames but cost more per frame.
+# next frame's tracing.  Against the plain tracer at equal RMSE after 256
+# frames (two cache seeds each, one run varies by about +-0.1): 4 steps of
+# 16,384 records (0.22 ms per frame) 0.46-0.68x; 3 steps 0.44-0.59x; 2 steps
+# of 32,768 (0.18-0.20 ms) 0.29-0.61x.  More, smaller steps converge in fewer
+# frames; the split of the 65,536 records barely changes the frame's cost.
 LEARNED_MINIBATCHES = 4
 
 
@@ -366,6 +371,8 @@ def learned_measurement(device, frames=(16, 64, 256), bake_frames: int = 4096, p
             "bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
             "flops_per_query": MLP_FLOPS_PER_QUERY, "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": tflops / MFMA_F16_PEAK_TFLOPS,
+            "instruction": MFMA_INSTRUCTION, "instruction_peak": MFMA_K16_PEAK_TFLOPS,
+            "instruction_frac": tflops / MFMA_K16_PEAK_TFLOPS,
         },
     }
 
@@ -689,7 +696,9 @@ def combined_leg(args, device) -> dict:
         "plain_and_learned_one_graph": one_graph,
         "mlp_roofline": {"bound": "mfma", "kernel": "nrc_mlp_fwd<false>", "queries": q, "avg_launch_ms": mlp_ms,
                          "achieved": tflops, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": tflops / MFMA_F16_PEAK_TFLOPS},
+                         "frac": tflops / MFMA_F16_PEAK_TFLOPS, "instruction": MFMA_INSTRUCTION,
+                         "instruction_peak": MFMA_K16_PEAK_TFLOPS,
+                         "instruction_frac": tflops / MFMA_K16_PEAK_TFLOPS},
     }
 
 
