@@ -1,6 +1,6 @@
-// Fused Adam step of the learned radiance cache, for caches whose hash-grid
-// levels are all densely indexed ((res+1)^3 entries fit the table: the grids
-// of small scenes).  Three launches per step instead of seven:
+// Fused Adam step of the learned radiance cache, for minibatches of up to
+// kNrcBwdWaves * 64 records, on dense and hashed grid levels alike.  Three
+// launches per step instead of seven:
 //
 //   nrc_fused_fb          four waves per 64-record group (four 16-record
 //                         tiles), each owning one 16-unit block of every
@@ -17,7 +17,8 @@
 //                         step's loss statistics, beside the table-gradient
 //                         scatter: one thread per (record, level, corner), one
 //                         packed FP16 atomic each.
-//   nrc_fused_table_adam  the table's lazy Adam over the dense lattices.
+//   nrc_fused_table_adam  the table's lazy Adam over each dense level's
+//                         lattice and each hashed level's whole table.
 //
 // The step computes what nrc_train_one does: the same loss, gradient clamp,
 // rounding points (activations and dY in FP16, FP32 accumulation), FP16
@@ -77,7 +78,7 @@ struct UpdArgs {
     TableState *ts;
     __half2 *tgrad, *t16;
     uint32_t entries, levels, lattice_total;
-    uint32_t lattice_begin[kNrcMaxLevels + 1];  // prefix of the levels' (res+1)^3
+    uint32_t lattice_begin[kNrcMaxLevels + 1];  // prefix of the levels' spans: (res+1)^3 dense, entries hashed
     const float2 *lossp;
     float *stats;
     AdamState *as;
@@ -186,9 +187,9 @@ __global__ __launch_bounds__(64 * kFbWaves) void nrc_fused_fb(FbArgs A) {
     float tgt[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) tgt[c] = counted ? A.target[3 * (size_t)sw + c] : 0.0f;
-    {  // encoding: lane = record, wave w the levels w, w + 4, ...  Every level
-       // is densely indexed here; lanes past n read the cell at the box corner
-       // and store zeros.
+    {  // encoding: lane = record, wave w the levels w, w + 4, ... (dense or
+       // hashed corner indexing per level); lanes past n read the cell at the
+       // box corner and store zeros.
         const uint32_t s = g * 64u + (uint32_t)lane;
         const bool live = s < A.n;
         _Float16 *row = x0s + lane * kX0Stride;
@@ -203,14 +204,13 @@ __global__ __launch_bounds__(64 * kFbWaves) void nrc_fused_fb(FbArgs A) {
                 float f[3];
                 level_cell(A.E, l, p, base, f);
                 const half2_t *tab = reinterpret_cast<const half2_t *>(A.E.table) + (size_t)l * A.E.entries;
-                const uint32_t r1 = A.E.res[l] + 1u;
                 half2_t v[8];
                 float wt[8];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
-                    v[k] = tab[(base[0] + dx) + r1 * ((base[1] + dy) + r1 * (base[2] + dz))];
-                    wt[k] = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
+                    uint32_t idx;
+                    cell_corner(A.E, l, base, f, (uint32_t)k, idx, wt[k]);
+                    v[k] = tab[idx];
                 }
                 float f0 = 0.0f, f1 = 0.0f;
 #pragma unroll
@@ -493,8 +493,9 @@ __global__ __launch_bounds__(kUpdBlock) void nrc_fused_update(UpdArgs U) {
     U.w16n[nrc_layer_offset(l) + (rel % in) * out + rel / in] = (_Float16)wi;
 }
 
-// The table's lazy Adam over the dense lattices, one entry per thread; the
-// step's last kernel stores Adam's step count.
+// The table's lazy Adam, one entry per thread: a dense level's lattice, a
+// hashed level's whole table; the step's last kernel stores Adam's step
+// count.
 __global__ __launch_bounds__(kUpdBlock) void nrc_fused_table_adam(UpdArgs U) {
     const float scale = 1.0f / (3.0f * fmaxf((float)U.n, 1.0f));
     const float c1 = U.as->c1, c2 = U.as->c2;
@@ -558,17 +559,15 @@ FbArgs fb_args(NanNrc *c, const float *pos, const float *nrm, const float *alb, 
 namespace nanx {
 
 bool nrc_fused_eligible(const NanNrc *c, uint32_t n) {
-    if (n > (uint32_t)kNrcBwdWaves * 64u) return false;  // one 64-record group per wave and partial
-    for (uint32_t l = 0; l < c->cfg.levels; ++l)
-        if (!c->level_dense[l]) return false;
-    return true;
+    (void)c;
+    return n <= (uint32_t)kNrcBwdWaves * 64u;  // one 64-record group per block and partial
 }
 
 int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                     const uint32_t *valid, uint32_t n, hipStream_t s) {
     if (n == 0) return 0;
     if (!nrc_fused_eligible(c, n))
-        return fail(nullptr, 0, "nan_nrc: the fused training step needs densely indexed levels and at most %u records",
+        return fail(nullptr, 0, "nan_nrc: the fused training step takes at most %u records",
                     (uint32_t)kNrcBwdWaves * 64u);
     if (nrc_reserve_train(c, n, s)) return -1;
     FbArgs A = fb_args(c, pos, nrm, alb, target, valid, n);
@@ -596,7 +595,8 @@ int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *
     U.lattice_begin[0] = 0;
     for (uint32_t l = 0; l < (uint32_t)kNrcMaxLevels; ++l) {
         const uint32_t r1 = c->level_res[l] + 1u;
-        U.lattice_begin[l + 1] = U.lattice_begin[l] + (l < cf.levels ? r1 * r1 * r1 : 0u);
+        const uint32_t span = c->level_dense[l] ? r1 * r1 * r1 : c->table_entries;
+        U.lattice_begin[l + 1] = U.lattice_begin[l] + (l < cf.levels ? span : 0u);
     }
     U.lattice_total = U.lattice_begin[cf.levels];
     U.lossp = c->lossp;
@@ -619,7 +619,7 @@ int nrc_fused_grads(NanNrc *c, const float *pos, const float *nrm, const float *
                     float *weight_grad, float *input_grad, hipStream_t s) {
     if (n == 0) return 0;
     if (!nrc_fused_eligible(c, n))
-        return fail(nullptr, 0, "nan_nrc: the fused gradients need densely indexed levels and at most %u records",
+        return fail(nullptr, 0, "nan_nrc: the fused gradients take at most %u records",
                     (uint32_t)kNrcBwdWaves * 64u);
     FbArgs A = fb_args(c, pos, nrm, alb, target, nullptr, n);
     const uint32_t parts = A.groups;

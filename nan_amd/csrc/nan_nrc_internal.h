@@ -155,7 +155,7 @@ struct NanNrc {
     _Float16 *snap = nullptr;
     float *stats = nullptr;    // [0] loss sum, [1] normalising count, [2] valid records (floats), [3] spare
     float2 *lossp = nullptr;   // [kNrcBwdWaves] per-block loss sums of the fused step
-    bool fused_train = true;   // two-launch step when every level is densely indexed
+    bool fused_train = true;   // three-launch step for minibatches of at most kNrcBwdWaves * 64 records
     void *adam = nullptr;      // Adam step count and bias corrections (16 bytes, device)
     ~NanNrc();
 };
@@ -189,9 +189,9 @@ int nrc_infer_into(NanNrc *c, const float *pos, const float *nrm, const float *a
 // slice size.
 int nrc_train_step(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                    const uint32_t *valid, uint32_t n, hipStream_t stream);
-// One Adam step in three launches (nan_nrc_fused.hip), for caches whose
-// levels are all densely indexed and steps of at most kNrcBwdWaves * 64
-// records; the same loss and update as the seven-launch step.
+// One Adam step in three launches (nan_nrc_fused.hip), for steps of at most
+// kNrcBwdWaves * 64 records on dense and hashed levels; the same loss and
+// update as the seven-launch step.
 bool nrc_fused_eligible(const NanNrc *c, uint32_t n);
 int nrc_train_fused(NanNrc *c, const float *pos, const float *nrm, const float *alb, const float *target,
                     const uint32_t *valid, uint32_t n, hipStream_t stream);

@@ -235,10 +235,10 @@ class RadianceCache:
         return out.view(self.config.levels, -1, 2)
 
     def set_fused_training(self, fused: bool) -> None:
-        """Training kernels of a cache whose levels are all densely indexed:
-        the two-launch fused step (default) or the seven-launch step with
-        packed FP16 table-gradient atomics (hashed caches always take that
-        one)."""
+        """Training kernels: the three-launch fused step (default; minibatches
+        of up to 32,768 records, dense and hashed levels) or the seven-launch
+        step with packed FP16 table-gradient atomics (larger minibatches
+        always take that one)."""
         _native.check(_native.lib().nan_nrc_set_fused_training(self.handle, 1 if fused else 0),
                       "nan_nrc_set_fused_training")
 

@@ -1,5 +1,6 @@
-"""The fused Adam step of a densely indexed radiance cache (taken for
-minibatches of up to 32,768 records) against the seven-launch step and
+"""The fused Adam step of a radiance cache, on densely indexed and on
+mixed dense and hashed grids (taken for minibatches of up to 32,768
+records), against the seven-launch step and
 the float references of tests/nrc_reference.py: its forward and backward
 against autograd, the first step's update (Adam's first step moves every
 parameter with a nonzero gradient by the learning rate against the
@@ -22,6 +23,10 @@ from tests import nrc_reference as ref  # noqa: E402
 BBOX_MIN = np.array([-1.0, 0.0, -1.0])
 BBOX_MAX = np.array([1.0, 2.0, 1.0])
 DENSE = dict(levels=4, base_resolution=16.0, max_resolution=32.0, log2_table_size=19)
+# 8 levels of 16 to 256 cells over a 2^14-entry table: levels 0-1 dense, 2-7
+# hashed (the fused step indexes each level as the library does)
+MIXED = dict(levels=8, base_resolution=16.0, max_resolution=256.0, log2_table_size=14)
+GRIDS = {"dense": DENSE, "mixed": MIXED}
 
 
 def _say(msg: str) -> None:
@@ -29,10 +34,10 @@ def _say(msg: str) -> None:
     sys.stdout.flush()
 
 
-def _cache(device, fused=True, **kw):
+def _cache(device, fused=True, grid=DENSE, **kw):
     from nan_amd.radiance_cache import RadianceCache, RadianceCacheConfig
 
-    cache = RadianceCache(device, BBOX_MIN, BBOX_MAX, RadianceCacheConfig(**{**DENSE, **kw}))
+    cache = RadianceCache(device, BBOX_MIN, BBOX_MAX, RadianceCacheConfig(**{**grid, **kw}))
     cache.set_fused_training(fused)
     return cache
 
@@ -47,20 +52,24 @@ def _inputs(n, seed, device):
     return (pos, nrm, alb), (t(pos), t(nrm), t(alb))
 
 
-def test_every_level_of_the_grid_is_dense():
+def test_grid_levels_are_dense_or_hashed_as_configured():
     _, dense = ref.level_grid(DENSE["levels"], DENSE["log2_table_size"], DENSE["base_resolution"],
                               DENSE["max_resolution"])
     assert all(dense)
+    _, dense = ref.level_grid(MIXED["levels"], MIXED["log2_table_size"], MIXED["base_resolution"],
+                              MIXED["max_resolution"])
+    assert dense == [True, True] + [False] * 6
 
 
+@pytest.mark.parametrize("grid", sorted(GRIDS))
 @pytest.mark.parametrize("n", [64, 100, 5000, 32_768])
-def test_fused_backward_matches_autograd(device, n):
+def test_fused_backward_matches_autograd(device, n, grid):
     """The fused forward and backward (the backward_raw hook of a dense cache
     runs them) against autograd through the float reference, which rounds
     where the kernels round: weight gradients per layer and the gradients of
     all 48 inputs."""
-    _say(f"fused backward n={n}")
-    cache = _cache(device)
+    _say(f"fused backward n={n} grid={grid}")
+    cache = _cache(device, grid=GRIDS[grid])
     rng = np.random.default_rng(5)
     w, t = cache.get_params()
     w = rng.uniform(-0.3, 0.3, size=w.shape).astype(np.float32)
@@ -85,16 +94,18 @@ def test_fused_backward_matches_autograd(device, n):
     assert rel_i < 1e-2, f"input gradient relative error {rel_i}"
 
 
+@pytest.mark.parametrize("grid", sorted(GRIDS))
 @pytest.mark.parametrize("n", [100, 4096, 32_768])
-def test_first_fused_step_moves_by_learning_rate(device, n):
+def test_first_fused_step_moves_by_learning_rate(device, n, grid):
     """Adam's first step: m_hat = g and v_hat = g^2, so a parameter with a
     nonzero gradient moves by exactly lr against its sign.  Weight gradients
     come from the seven-launch backward (checked against autograd
     elsewhere); table gradients from its input gradients spread over the cell
     corners in float64.  Entries no record touches keep their values."""
-    _say(f"first fused step n={n}")
+    _say(f"first fused step n={n} grid={grid}")
     lr = 1e-2
-    cache = _cache(device, learning_rate=lr)
+    g_cfg = GRIDS[grid]
+    cache = _cache(device, grid=g_cfg, learning_rate=lr)
     rng = np.random.default_rng(20)
     w0, t0 = cache.get_params()
     w0 = rng.uniform(-0.3, 0.3, size=w0.shape).astype(np.float32)
@@ -104,8 +115,8 @@ def test_first_fused_step_moves_by_learning_rate(device, n):
     target = torch.from_numpy(rng.uniform(0.0, 2.0, size=(n, 3)).astype(np.float32)).to(device.torch_device)
     g, ig = cache.backward_raw(*dev_in, target)
     g = g.cpu().numpy()
-    want = ref.table_gradient(pos, ig.cpu().numpy(), BBOX_MIN, BBOX_MAX, DENSE["levels"], t0.shape[1],
-                              DENSE["base_resolution"], DENSE["max_resolution"], DENSE["log2_table_size"])
+    want = ref.table_gradient(pos, ig.cpu().numpy(), BBOX_MIN, BBOX_MAX, g_cfg["levels"], t0.shape[1],
+                              g_cfg["base_resolution"], g_cfg["max_resolution"], g_cfg["log2_table_size"])
     cache.train_step(*dev_in, target)
     loss, count = cache.last_step_stats()
     assert np.isfinite(loss) and loss > 0 and count == n
