@@ -353,13 +353,13 @@ NAN_API int nan_nrc_infer_noalloc(NanNrc *nrc, const float *pos, const float *nr
  * last step's mean loss to *loss_out (device float, may be NULL). */
 NAN_API int nan_nrc_train(NanNrc *nrc, const float *pos, const float *nrm, const float *alb, const float *target,
                           uint32_t n, float *loss_out, void *stream);
-/* Which kernels a training step runs: when every level of the cache is
- * densely indexed ((res + 1)^3 entries fit the table), fused = 1 (the
- * default) takes the three-launch step for minibatches of up to 32,768
- * records (encoding, forward, loss and backward in one launch; the weights'
- * Adam beside the table-gradient atomics; the table's Adam over the dense
- * lattices), fused = 0 the seven-launch step.  Both sum table gradients with
- * packed FP16 atomics.  Hashed caches always take the seven-launch step. */
+/* Which kernels a training step runs: fused = 1 (the default) takes the
+ * three-launch step for minibatches of up to 32,768 records, on dense and
+ * hashed levels alike (encoding, forward, loss and backward in one launch;
+ * the weights' Adam beside the table-gradient atomics; the table's lazy
+ * Adam over each dense level's lattice and each hashed level's table),
+ * fused = 0 the seven-launch step.  Both sum table gradients with packed
+ * FP16 atomics.  Larger minibatches always take the seven-launch step. */
 NAN_API int nan_nrc_set_fused_training(NanNrc *nrc, int fused);
 /* Testing hooks: the raw first four network outputs float[n][4], and the
  * gradients of the mean loss for targets float[n][3] with respect to the MLP
